@@ -1,0 +1,62 @@
+"""ctypes binding of libbats_amd.so (the C ABI declared in include/bats_amd.h).
+
+The library is loaded lazily on first use. There is no fallback: if the library has not
+been built, or no GPU is visible, every operator raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libbats_amd.so"
+
+_vp = C.c_void_p
+_i = C.c_int
+_i64 = C.c_int64
+_f = C.c_float
+_d = C.c_double
+_sz = C.c_size_t
+
+# name -> (restype, argtypes); must match include/bats_amd.h
+SIGNATURES = {
+    "bats_last_error": (C.c_char_p, []),
+    "bats_version": (C.c_char_p, []),
+    "bats_lif_forward_workspace_size": (_sz, [_i, _i, _i, _i]),
+    "bats_lif_forward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
+                              _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i]),
+    "bats_lif_backward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp, _sz]),
+    "bats_count_loss": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _vp, _vp, _vp]),
+    "bats_count_loss_targets": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "bats_ttfs_loss": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _vp, _vp, _vp]),
+    "bats_adam_step": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f,
+                            _d, _d]),
+    "bats_nadam_step": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f,
+                             _d, _d]),
+    "bats_sgd_step": (_i, [_vp, _i64, _vp, _vp, _vp, _f]),
+    "bats_add_deltas": (_i, [_vp, _i64, _vp, _vp]),
+    "bats_events_to_per_neuron": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbats_amd.so and declare every exported signature."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m bats_amd.build`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError("bats_amd: " + load().bats_last_error().decode())

@@ -1,0 +1,66 @@
+// Shared helpers of the BATS HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+namespace bats {
+
+// ---------------------------------------------------------------- error reporting
+void set_error(const std::string &msg);
+int fail(const std::string &msg);
+int check_launch(const char *what);
+
+#define BATS_REQUIRE(cond, msg)                                   \
+  do {                                                            \
+    if (!(cond)) return ::bats::fail(std::string(__func__) + ": " + (msg)); \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// ---------------------------------------------------------------- device helpers
+// Monotone map of a float's bits to an unsigned key (ascending float order = ascending key).
+__device__ __forceinline__ uint32_t float_key(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+
+__device__ __forceinline__ bool is_inf_f(float v) { return isinf(v); }
+
+constexpr float kInfF = __builtin_huge_valf();
+constexpr double kInfD = __builtin_huge_val();
+
+// One time-sorted input event of one sample, read uniformly by every lane of a wave.
+// es/e are the float32 exps exactly as stored for the backward; zq/zl are float64
+// exp(t/tau) and exp(min(t_next, t_max)/tau); czl2 = c*zl*zl; tl = min(t_next, t_max).
+struct alignas(16) Event {
+  float es;
+  float e;
+  int idx;
+  float t;
+  double zq;
+  double zl;
+  double czl2;
+  float tl;
+  float pad;
+};
+static_assert(sizeof(Event) == 48, "Event layout");
+
+// Fill kernel (used instead of hipMemsetAsync so that every call stays graph-capturable).
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace bats

@@ -1,0 +1,281 @@
+// Dense LIF layer BATS backward for gfx950.
+//
+// Both the weight gradient (reference compute_weights_gradient.cu:33-45) and the errors of
+// the pre-synaptic spikes (propagate_errors_to_pre_spikes.cu:41-53) are functions of the
+// same quantity: for a pre spike at time t_m and a post neuron j of the same sample,
+//   V1_j(t_m) = sum_{k: t_jk > t_m} e_jk f1_jk,   V2_j(t_m) = sum_{k: t_jk > t_m} e_jk f2_jk.
+// Regrouping the reference's per-post-spike sums gives
+//   dW[j,i]      = sum_b sum_m  es_m V1_j(t_m) - e_m V2_j(t_m)       (m: spikes of pre i)
+//   pre_err[b,i,m] = sum_j W[j,i] (V1_j(t_m) es_m / tau_s - V2_j(t_m) e_m / tau)
+// so the backward is three kernels:
+//   1. post_prep_kernel  one lane per (sample, post neuron): the reset-chain error scan
+//      (propagate_recurrent_errors.cu:17-22, written back in place like the reference),
+//      the factors f1, f2 (compute_factors.cu:20-30, rounded to float32 as stored there),
+//      and the float64 suffix sums S1[k] = sum_{k'>=k} e f1, S2[k] = sum_{k'>=k} e f2.
+//      V_j(t) is then S[first k with t_jk > t].
+//   2. wgrad_kernel      lanes over pre neurons i, a tile of post neurons j, a slice of the
+//      batch; the post neuron's spike list is wave-uniform (scalar loads), the lookup is a
+//      select chain over its at most K spikes; float64 accumulation; a deterministic
+//      fixed-order reduction over batch slices (wgrad_reduce_kernel) finishes the mean.
+//   3. pre_err_kernel    lanes over pre neurons i of one sample, loop over post neurons j.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace bats {
+
+constexpr int kBwdThreads = 256;
+constexpr int kJTile = 8;
+
+__global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
+    int BJ, int K, const int *__restrict__ n, const float *__restrict__ t,
+    const float *__restrict__ a, const float *__restrict__ x, const float *__restrict__ z,
+    float *__restrict__ errors, double c, double tau, double dtt, int *__restrict__ n_eff,
+    double *__restrict__ S1, double *__restrict__ S2) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= BJ) return;
+  const size_t row = (size_t)lane * K;
+  const int nk = min(n[lane], K);
+  // reset chain, reverse order (propagate_recurrent_errors.cu:17-22)
+  double C = 0.0;
+  for (int k = nk - 1; k >= 0; --k) {
+    const float zk = z[row + k];
+    if (isinf(zk)) continue;
+    const double ek = (double)errors[row + k] + C * (double)zk;
+    const float ek32 = (float)ek;
+    errors[row + k] = ek32;
+    C = C + (double)ek32 * dtt / (double)x[row + k];
+  }
+  // spikes used by the gradient: those before the first infinite time
+  int ne = 0;
+  while (ne < nk && !isinf(t[row + ne])) ++ne;
+  double s1 = 0.0, s2 = 0.0;
+  for (int k = ne - 1; k >= 0; --k) {
+    const double ak = (double)a[row + k], xk = (double)x[row + k], zk = (double)z[row + k];
+    const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
+    const float f2 = (float)(tau / xk);
+    const double ek = (double)errors[row + k];
+    s1 += ek * (double)f1;
+    s2 += ek * (double)f2;
+    S1[row + k] = s1;
+    S2[row + k] = s2;
+  }
+  n_eff[lane] = ne;
+}
+
+// V(t) lookup: S[first k with tpost[k] > tp] (0 when none), post times ascending.
+__device__ __forceinline__ void lookup(const float *__restrict__ tpost,
+                                       const double *__restrict__ s1,
+                                       const double *__restrict__ s2, int nj, float tp,
+                                       double &v1, double &v2) {
+  v1 = 0.0;
+  v2 = 0.0;
+  for (int k = nj - 1; k >= 0; --k) {
+    if (tpost[k] > tp) {
+      v1 = s1[k];
+      v2 = s2[k];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBwdThreads) void wgrad_kernel(
+    int B, int I, int K_pre, int J, int K, int b_per_slice, const float *__restrict__ pre_t,
+    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
+    const float *__restrict__ post_t, const int *__restrict__ n_eff,
+    const double *__restrict__ S1, const double *__restrict__ S2, int per_sample,
+    float *__restrict__ grad, double *__restrict__ partial) {
+  const int i = blockIdx.x * kBwdThreads + threadIdx.x;
+  const bool active = i < I;
+  const int ic = active ? i : 0;
+  const int j0 = blockIdx.y * kJTile;
+  const int slice = blockIdx.z;
+  const int b0 = slice * b_per_slice;
+  const int b1 = min(B, b0 + b_per_slice);
+  double acc[kJTile];
+#pragma unroll
+  for (int u = 0; u < kJTile; ++u) acc[u] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const size_t pbase = ((size_t)b * I + ic) * K_pre;
+    for (int m = 0; m < K_pre; ++m) {
+      const float tp = active ? pre_t[pbase + m] : kInfF;
+      if (!isfinite(tp)) break;
+      const double es = (double)pre_es[pbase + m];
+      const double e = (double)pre_e[pbase + m];
+#pragma unroll
+      for (int u = 0; u < kJTile; ++u) {
+        const int j = j0 + u;
+        if (j >= J) break;
+        const size_t prow = (size_t)b * J + j;
+        const int nj = n_eff[prow];
+        if (nj == 0) continue;
+        double v1, v2;
+        lookup(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
+        acc[u] = fma(es, v1, acc[u]);
+        acc[u] = fma(-e, v2, acc[u]);
+      }
+    }
+    if (per_sample) {
+#pragma unroll
+      for (int u = 0; u < kJTile; ++u) {
+        const int j = j0 + u;
+        if (j < J && active) grad[((size_t)b * J + j) * I + i] = (float)acc[u];
+        acc[u] = 0.0;
+      }
+    }
+  }
+  if (!per_sample && active) {
+#pragma unroll
+    for (int u = 0; u < kJTile; ++u) {
+      const int j = j0 + u;
+      if (j < J) partial[((size_t)slice * J + j) * I + i] = acc[u];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBwdThreads) void wgrad_reduce_kernel(
+    int slices, int64_t JI, double inv_b, const double *__restrict__ partial,
+    float *__restrict__ grad) {
+  const int64_t e = (int64_t)blockIdx.x * kBwdThreads + threadIdx.x;
+  if (e >= JI) return;
+  double s = 0.0;
+  for (int z = 0; z < slices; ++z) s += partial[(size_t)z * JI + e];
+  grad[e] = (float)(s * inv_b);
+}
+
+__global__ __launch_bounds__(kBwdThreads) void pre_err_kernel(
+    int I, int K_pre, int J, int K, const float *__restrict__ pre_t,
+    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
+    const float *__restrict__ W, const float *__restrict__ post_t,
+    const int *__restrict__ n_eff, const double *__restrict__ S1,
+    const double *__restrict__ S2, double tau_s, double tau, float *__restrict__ pre_errors) {
+  const int i = blockIdx.x * kBwdThreads + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= I) return;
+  const size_t pbase = ((size_t)b * I + i) * K_pre;
+  for (int m = 0; m < K_pre; ++m) {
+    const float tp = pre_t[pbase + m];
+    if (!isfinite(tp)) {
+      pre_errors[pbase + m] = 0.f;
+      continue;
+    }
+    const double es = (double)pre_es[pbase + m];
+    const double e = (double)pre_e[pbase + m];
+    double acc = 0.0;
+    for (int j = 0; j < J; ++j) {
+      const size_t prow = (size_t)b * J + j;
+      const int nj = n_eff[prow];
+      if (nj == 0) continue;
+      double v1, v2;
+      lookup(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
+      const double w = (double)W[(size_t)j * I + i];
+      acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
+    }
+    pre_errors[pbase + m] = (float)acc;
+  }
+}
+
+struct BwdLayout {
+  size_t neff, s1, s2, partial, total;
+  int slices, b_per_slice;
+};
+
+static BwdLayout bwd_layout(int B, int I, int J, int K) {
+  BwdLayout L{};
+  const int gi = (I + kBwdThreads - 1) / kBwdThreads;
+  const int gj = (J + kJTile - 1) / kJTile;
+  const int base = std::max(1, gi * gj);
+  int slices = std::max(1, std::min(B, (2048 + base - 1) / base));
+  L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
+  L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
+  size_t off = 0;
+  L.neff = off;
+  off = align_up(off + sizeof(int) * (size_t)B * J, 256);
+  L.s1 = off;
+  off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  L.s2 = off;
+  off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  L.partial = off;
+  off = align_up(off + sizeof(double) * (size_t)L.slices * J * I, 256);
+  L.total = off;
+  return L;
+}
+
+}  // namespace bats
+
+using namespace bats;
+
+extern "C" size_t bats_lif_backward_workspace_size(int B, int I, int J, int K_post) {
+  if (B < 0 || I < 0 || J < 0 || K_post < 0) return 0;
+  return bwd_layout(B, I, J, K_post).total;
+}
+
+extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, int K_post,
+                                 const int *n, const float *t, const float *a, const float *x,
+                                 const float *z, float *errors, const float *pre_times,
+                                 const float *pre_exp_tau_s, const float *pre_exp_tau,
+                                 const float *weights, float tau_s, float tau, float c,
+                                 float delta_theta_tau, float *grad, int grad_mode,
+                                 float *pre_errors, void *workspace, size_t workspace_bytes) {
+  BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0 && K_post >= 1, "bad sizes");
+  BATS_REQUIRE(grad_mode == 0 || grad_mode == 1, "grad_mode must be 0 (mean) or 1 (per sample)");
+  hipStream_t st = as_stream(stream);
+  const int64_t JI = (int64_t)J * I;
+  if (B == 0) {
+    if (grad_mode == 0 && JI > 0) {
+      BATS_REQUIRE(grad, "null grad");
+      hipLaunchKernelGGL(fill_kernel<float>, dim3((unsigned)std::min<int64_t>((JI + 255) / 256, 65535)),
+                         dim3(256), 0, st, grad, JI, 0.f);
+      return check_launch("fill grad");
+    }
+    return 0;
+  }
+  BATS_REQUIRE(n && t && a && x && z && errors, "null post-layer pointer");
+  BATS_REQUIRE(grad || JI == 0, "null grad");
+  const BwdLayout L = bwd_layout(B, I, J, K_post);
+  BATS_REQUIRE(workspace != nullptr || L.total == 0, "null workspace");
+  BATS_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+  char *ws = static_cast<char *>(workspace);
+  int *n_eff = reinterpret_cast<int *>(ws + L.neff);
+  double *S1 = reinterpret_cast<double *>(ws + L.s1);
+  double *S2 = reinterpret_cast<double *>(ws + L.s2);
+  double *partial = reinterpret_cast<double *>(ws + L.partial);
+
+  const int BJ = B * J;
+  if (BJ > 0) {
+    hipLaunchKernelGGL(post_prep_kernel, dim3((BJ + kBwdThreads - 1) / kBwdThreads),
+                       dim3(kBwdThreads), 0, st, BJ, K_post, n, t, a, x, z, errors, (double)c,
+                       (double)tau, (double)delta_theta_tau, n_eff, S1, S2);
+    if (int r = check_launch("post_prep_kernel")) return r;
+  }
+  if (JI > 0) {
+    BATS_REQUIRE(K_pre == 0 || (pre_times && pre_exp_tau_s && pre_exp_tau), "null pre pointer");
+    const int gi = (I + kBwdThreads - 1) / kBwdThreads;
+    const int gj = (J + kJTile - 1) / kJTile;
+    if (grad_mode == 1) {
+      // one sample per workgroup
+      dim3 g(gi, gj, B);
+      hipLaunchKernelGGL(wgrad_kernel, g, dim3(kBwdThreads), 0, st, B, I, K_pre, J, K_post, 1,
+                         pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1, S2, 1, grad,
+                         partial);
+      if (int r = check_launch("wgrad_kernel")) return r;
+    } else {
+      dim3 g(gi, gj, L.slices);
+      hipLaunchKernelGGL(wgrad_kernel, g, dim3(kBwdThreads), 0, st, B, I, K_pre, J, K_post,
+                         L.b_per_slice, pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1,
+                         S2, 0, grad, partial);
+      if (int r = check_launch("wgrad_kernel")) return r;
+      hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
+                         dim3(kBwdThreads), 0, st, L.slices, JI, 1.0 / (double)B, partial, grad);
+      if (int r = check_launch("wgrad_reduce_kernel")) return r;
+    }
+  }
+  if (pre_errors && I > 0 && K_pre > 0) {
+    BATS_REQUIRE(weights, "null weights");
+    dim3 g((I + kBwdThreads - 1) / kBwdThreads, B);
+    hipLaunchKernelGGL(pre_err_kernel, g, dim3(kBwdThreads), 0, st, I, K_pre, J, K_post,
+                       pre_times, pre_exp_tau_s, pre_exp_tau, weights, t, n_eff, S1, S2,
+                       (double)tau_s, (double)tau, pre_errors);
+    if (int r = check_launch("pre_err_kernel")) return r;
+  }
+  return 0;
+}

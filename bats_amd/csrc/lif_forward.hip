@@ -1,0 +1,344 @@
+// Dense LIF layer forward for gfx950: per-sample event sort + exact multi-spike solver.
+//
+// Data flow of one call (all on the caller's stream, no host synchronisation):
+//   1. transpose_kernel    W [J,I] -> W^T [I,J] so that the lanes of a wave (consecutive
+//                          post neurons) read one input's weights with one coalesced load.
+//   2. sort_events_kernel  one workgroup per sample: compacts the finite input slots,
+//                          bitonic-sorts (time, slot) keys in LDS and writes the sample's
+//                          time-ordered Event list (exps in float32 exactly as stored for
+//                          the backward, exp(t/tau) in float64 for the crossing filter).
+//                          It also writes the per-slot exps the backward reads.
+//   3. lif_solve_seq_kernel  one lane per (sample, post neuron): walks the sample's events
+//                          in order, accumulates a and b in float64 (products of two
+//                          float32 are exact in float64, so the running sums equal a
+//                          float64 evaluation in the reference's order), evaluates a
+//                          conservative crossing filter per event and runs the closed-form
+//                          root (the reference's compute_spikes loop) in float64 only when
+//                          the filter cannot exclude a crossing before the next event.
+//   4. lif_solve_wave_kernel one wave per (sample, post neuron) for layers with few post
+//                          neurons: the same decision sequence with the event axis spread
+//                          over the 64 lanes (prefix sums by wave scan).
+//
+// Reference: bats/Layers/LIFLayer.py:63-91, bats/CudaKernels/CuFiles/Inference/
+// compute_pre_exps.cu, get_spike_weights.cu, compute_spike_times.cu:34-138,
+// bats/CudaKernels/Wrappers/Inference/get_sorted_spikes_indices.py:7-16.
+#include "common.hpp"
+
+#include <algorithm>
+#include <mutex>
+
+namespace bats {
+
+// ------------------------------------------------------------------ transpose
+__global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict__ W,
+                                                        float *__restrict__ WT, int J, int I) {
+  __shared__ float tile[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + r, i = i0 + tx;
+    if (j < J && i < I) tile[r][tx] = W[(size_t)j * I + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, j = j0 + tx;
+    if (j < J && i < I) WT[(size_t)i * J + j] = tile[tx][r];
+  }
+}
+
+// ------------------------------------------------------------------ per-sample sort
+constexpr int kSortThreads = 512;
+
+__device__ __forceinline__ void bitonic_sort(uint64_t *keys, int N, bool global_mem) {
+  for (int k = 2; k <= N; k <<= 1) {
+    for (int jstride = k >> 1; jstride > 0; jstride >>= 1) {
+      for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const int ixj = i ^ jstride;
+        if (ixj > i) {
+          const uint64_t x = keys[i], y = keys[ixj];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            keys[i] = y;
+            keys[ixj] = x;
+          }
+        }
+      }
+      if (global_mem) __threadfence();
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
+    const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
+    float tmax, int Pcap, Event *__restrict__ events, int *__restrict__ counts,
+    float *__restrict__ pre_exp_s, float *__restrict__ pre_exp, uint64_t *__restrict__ gkeys,
+    int gcap, int lds_cap) {
+  extern __shared__ uint64_t lkeys[];
+  __shared__ int s_count;
+  __shared__ int s_fill;
+  const int b = blockIdx.x;
+  const float *times = pre_times + (size_t)b * S;
+  if (threadIdx.x == 0) {
+    s_count = 0;
+    s_fill = 0;
+  }
+  __syncthreads();
+  int local = 0;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) local += isfinite(times[s]) ? 1 : 0;
+  if (local) atomicAdd(&s_count, local);
+  __syncthreads();
+  const int P = s_count;
+  int N = 1;
+  while (N < P) N <<= 1;
+  const bool use_global = N > lds_cap;
+  uint64_t *keys = use_global ? gkeys + (size_t)b * gcap : lkeys;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    const float t = times[s];
+    if (isfinite(t)) {
+      const int pos = atomicAdd(&s_fill, 1);
+      keys[pos] = ((uint64_t)float_key(t) << 32) | (uint32_t)s;
+    }
+  }
+  __syncthreads();
+  for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
+  if (use_global) __threadfence();
+  __syncthreads();
+  if (P > 1) bitonic_sort(keys, N, use_global);
+
+  Event *ev = events + (size_t)b * Pcap;
+  const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
+  for (int q = threadIdx.x; q < P; q += blockDim.x) {
+    const uint64_t key = keys[q];
+    const float t = key_float((uint32_t)(key >> 32));
+    const int s = (int)(uint32_t)key;
+    const float tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
+    const float tl = fminf(tn, tmax);
+    const double es64 = exp((double)t / dtau_s);
+    const double zq = exp((double)t / dtau);
+    const double zl = exp((double)tl / dtau);
+    Event e;
+    e.es = (float)es64;
+    e.e = (float)zq;
+    e.idx = s / K_pre;
+    e.t = t;
+    e.zq = zq;
+    e.zl = zl;
+    e.czl2 = dc * zl * zl;
+    e.tl = tl;
+    e.pad = 0.f;
+    ev[q] = e;
+    pre_exp_s[(size_t)b * S + s] = e.es;
+    pre_exp[(size_t)b * S + s] = e.e;
+  }
+  if (threadIdx.x == 0) counts[b] = P;
+}
+
+// ------------------------------------------------------------------ crossing filter
+// Necessary condition (with a relative margin) for the reference's smaller root
+// z = 2a/(b + sqrt(b^2 - 4ac)) to be accepted in (t_q, min(t_next, t_max)]:
+// a > 0 and b > 0, and either g(zl) = c zl^2 - b zl + a <= 0 (one root inside) or the
+// vertex b/(2c) of g lies in [zq, zl] with a non-negative discriminant (two roots inside).
+constexpr double kFilterEps = 1e-10;
+
+__device__ __forceinline__ bool crossing_candidate(double a, double b, const Event &e,
+                                                   double c4, double inv2c) {
+  if (!(a > 0.0) || !(b > 0.0)) return false;
+  const double bb_hi = b * b * (1.0 + kFilterEps);
+  if (isinf(e.zl)) return bb_hi >= c4 * a;
+  const double g = (a + e.czl2) - b * e.zl * (1.0 + kFilterEps);
+  if (g <= 0.0) return true;
+  const double v = b * inv2c;
+  if (v < e.zq * (1.0 - kFilterEps) || v > e.zl * (1.0 + kFilterEps)) return false;
+  return bb_hi >= c4 * a;
+}
+
+// The reference's compute_spikes loop (compute_spike_times.cu:53-83) in float64.
+// Returns false when the neuron's buffer became full (its event loop ends).
+__device__ __forceinline__ bool solve_crossings(double a, double &b, int &n, double last,
+                                                double tl, double c, double dtt, double tau,
+                                                int K, size_t row, float *__restrict__ out_t,
+                                                float *__restrict__ out_a,
+                                                float *__restrict__ out_x,
+                                                float *__restrict__ out_z) {
+  while (true) {
+    const double disc = b * b - (4.0 * a) * c;
+    if (disc < 0.0) return true;
+    const double x = sqrt(disc);
+    const double den = x + b;
+    if (den == 0.0) return true;
+    const double z = (2.0 * a) / den;
+    if (z < 0.0) return true;
+    const double t = tau * log(z);
+    if (t <= last || t > tl) return true;
+    out_a[row + n] = (float)a;
+    out_x[row + n] = (float)x;
+    out_t[row + n] = (float)t;
+    out_z[row + n] = (float)z;
+    last = t;
+    b = b - dtt * z;
+    ++n;
+    if (n >= K) return false;
+  }
+}
+
+// ------------------------------------------------------------------ solver, lane per neuron
+constexpr int kSolveThreads = 256;
+constexpr int kUnroll = 8;
+
+__global__ __launch_bounds__(kSolveThreads) void lif_solve_seq_kernel(
+    const Event *__restrict__ events, const int *__restrict__ counts, int Pcap,
+    const float *__restrict__ WT, int J, int K, double c, double dtt, double tau,
+    int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
+    float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
+  __shared__ int s_n[kSolveThreads];
+  const int b = blockIdx.y;
+  const int j0 = blockIdx.x * kSolveThreads;
+  const int j = j0 + threadIdx.x;
+  const bool active = j < J;
+  const int jc = active ? j : 0;
+  const int P = counts[b];
+  const Event *ev = events + (size_t)b * Pcap;
+  const double c4 = 4.0 * c, inv2c = 0.5 / c;
+  const size_t row = ((size_t)b * J + jc) * K;
+  double a = 0.0, bb = 0.0;
+  int n = 0;
+  bool live = active;
+  for (int q0 = 0; q0 < P; q0 += kUnroll) {
+    float w[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int q = q0 + u;
+      w[u] = (q < P) ? WT[(size_t)ev[q].idx * J + jc] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int q = q0 + u;
+      if (q < P) {
+        const Event e = ev[q];
+        a = fma((double)w[u], (double)e.es, a);
+        bb = fma((double)w[u], (double)e.e, bb);
+        const bool cand = live && (always_exact || crossing_candidate(a, bb, e, c4, inv2c));
+        if (cand)
+          live = solve_crossings(a, bb, n, (double)e.t, (double)e.tl, c, dtt, tau, K, row,
+                                 out_t, out_a, out_x, out_z);
+      }
+    }
+  }
+  s_n[threadIdx.x] = active ? n : 0;
+  if (active) out_n[(size_t)b * J + j] = n;
+  __syncthreads();
+  const int jcount = min(kSolveThreads, J - j0);
+  float *tb = out_t + ((size_t)b * J + j0) * K;
+  for (int e = threadIdx.x; e < jcount * K; e += kSolveThreads) {
+    const int jj = e / K;
+    if (e - jj * K >= s_n[jj]) tb[e] = kInfF;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+static int lds_capacity_entries() {
+  static int cached[64];
+  static std::once_flag once[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 4096;
+  std::call_once(once[dev], [dev]() {
+    int bytes = 0;
+    if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+        hipSuccess)
+      bytes = 65536;
+    bytes -= 1024;  // static LDS of the kernel + slack
+    int cap = 1;
+    while ((cap << 1) * 8 <= bytes) cap <<= 1;
+    cached[dev] = cap;
+    if (cap * 8 > 65536)
+      (void)hipFuncSetAttribute((const void *)sort_events_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, cap * 8);
+  });
+  return cached[dev];
+}
+
+struct FwdLayout {
+  size_t wt, events, counts, gkeys, total;
+  int Pcap, gcap;
+};
+
+static FwdLayout fwd_layout(int B, int I, int K_pre, int J, int lds_cap) {
+  FwdLayout L{};
+  const int S = I * K_pre;
+  L.Pcap = std::max(S, 1);
+  int N = 1;
+  while (N < S) N <<= 1;
+  L.gcap = (N > lds_cap) ? N : 0;
+  size_t off = 0;
+  L.wt = off;
+  off = align_up(off + sizeof(float) * (size_t)I * J, 256);
+  L.events = off;
+  off = align_up(off + sizeof(Event) * (size_t)B * L.Pcap, 256);
+  L.counts = off;
+  off = align_up(off + sizeof(int) * (size_t)B, 256);
+  L.gkeys = off;
+  off = align_up(off + sizeof(uint64_t) * (size_t)B * L.gcap, 256);
+  L.total = off;
+  return L;
+}
+
+}  // namespace bats
+
+using namespace bats;
+
+extern "C" size_t bats_lif_forward_workspace_size(int B, int I, int K_pre, int J) {
+  if (B < 0 || I < 0 || K_pre < 0 || J < 0) return 0;
+  return fwd_layout(B, I, K_pre, J, lds_capacity_entries()).total;
+}
+
+extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
+                                const float *pre_times, const float *weights, float tau_s,
+                                float tau, float c, float delta_theta_tau,
+                                float max_simulation, int *out_n, float *out_t, float *out_a,
+                                float *out_x, float *out_z, float *pre_exp_tau_s,
+                                float *pre_exp_tau, void *workspace, size_t workspace_bytes) {
+  BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0, "negative size");
+  BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
+  BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
+  if (B == 0 || J == 0) return 0;
+  BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
+  BATS_REQUIRE(tau_s > 0.f && tau > 0.f, "tau_s and tau must be positive");
+  const int S = I * K_pre;
+  hipStream_t st = as_stream(stream);
+  const int cap = lds_capacity_entries();
+  const FwdLayout L = fwd_layout(B, I, K_pre, J, cap);
+  BATS_REQUIRE(workspace != nullptr || L.total == 0, "null workspace");
+  BATS_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+  char *ws = static_cast<char *>(workspace);
+  float *WT = reinterpret_cast<float *>(ws + L.wt);
+  Event *events = reinterpret_cast<Event *>(ws + L.events);
+  int *counts = reinterpret_cast<int *>(ws + L.counts);
+  uint64_t *gkeys = reinterpret_cast<uint64_t *>(ws + L.gkeys);
+
+  if (S > 0) {
+    BATS_REQUIRE(pre_times && weights && pre_exp_tau_s && pre_exp_tau, "null input pointer");
+    dim3 tg((I + 31) / 32, (J + 31) / 32);
+    hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I);
+    if (int r = check_launch("transpose_kernel")) return r;
+    int N = 1;
+    while (N < S) N <<= 1;
+    const int lds_entries = std::min(N, cap);
+    hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
+                       sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s, tau, c,
+                       max_simulation, L.Pcap, events, counts, pre_exp_tau_s, pre_exp_tau,
+                       gkeys, L.gcap, lds_entries);
+    if (int r = check_launch("sort_events_kernel")) return r;
+  } else {
+    // no input neurons: every sample has zero events
+    hipLaunchKernelGGL(fill_kernel<int>, dim3((B + 255) / 256), dim3(256), 0, st, counts,
+                       (int64_t)B, 0);
+    if (int r = check_launch("zero_counts")) return r;
+  }
+  const int always_exact = (c > 0.f) ? 0 : 1;
+  dim3 sg((J + kSolveThreads - 1) / kSolveThreads, B);
+  hipLaunchKernelGGL(lif_solve_seq_kernel, sg, dim3(kSolveThreads), 0, st, events, counts,
+                     L.Pcap, WT, J, K_post, (double)c, (double)delta_theta_tau, (double)tau,
+                     always_exact, out_n, out_t, out_a, out_x, out_z);
+  return check_launch("lif_solve_seq_kernel");
+}

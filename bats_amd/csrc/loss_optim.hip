@@ -1,0 +1,279 @@
+// Losses, optimizer steps and event-list conversion for gfx950.
+//
+// Each kernel computes in float64 and stores float32 at the points where the reference
+// keeps float32 arrays, so that results match a float64 evaluation of the reference's
+// expressions in the reference's operation order.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace bats {
+
+constexpr int kThreads = 256;
+
+static unsigned grid_for(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 1 << 20));
+}
+
+// ---------------------------------------------------------------- spike-count loss
+// bats/Losses/SpikeCountLoss.py:17-38, targets from SpikeCountClassLoss.py:15-18.
+__global__ __launch_bounds__(kThreads) void count_loss_kernel(
+    int B, int O, int K, const int *__restrict__ n, const int64_t *__restrict__ labels,
+    const float *__restrict__ targets, float target_true, float target_false,
+    float *__restrict__ loss, float *__restrict__ errors, int64_t *__restrict__ pred) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int64_t lab = labels ? labels[b] : -1;
+  double sq = 0.0;
+  int best = 0;
+  int bestn = 0;
+  for (int j = 0; j < O; ++j) {
+    const int nj = n[(size_t)b * O + j];
+    const float tg = targets ? targets[(size_t)b * O + j] : (j == lab ? target_true : target_false);
+    const double d = (double)tg - (double)(float)nj;
+    sq += d * d;
+    if (errors) {
+      const float d32 = (float)d;
+      float *er = errors + ((size_t)b * O + j) * K;
+      for (int k = 0; k < K; ++k) er[k] = d32;
+    }
+    if (j == 0 || nj > bestn) {
+      bestn = nj;
+      best = j;
+    }
+  }
+  if (loss) loss[b] = (float)(sq / 2.0);
+  if (pred) pred[b] = best;
+}
+
+// ---------------------------------------------------------------- TTFS softmax CE
+// bats/Losses/TTFSSoftmaxCrossEntropy.py:20-56.
+__device__ __forceinline__ float nan_to_num(float v) {
+  if (isnan(v)) return 0.f;
+  if (isinf(v)) return v > 0.f ? 3.4028234663852886e38f : -3.4028234663852886e38f;
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void ttfs_loss_kernel(
+    int B, int O, int K, const float *__restrict__ t, const int64_t *__restrict__ labels,
+    double tau, float *__restrict__ loss, float *__restrict__ errors, int64_t *__restrict__ pred) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int64_t lab = labels ? labels[b] : 0;
+  double sum = 0.0;
+  for (int j = 0; j < O; ++j) {
+    const float e = (float)exp(-(double)t[((size_t)b * O + j) * K] / tau);
+    sum += (double)e;
+  }
+  const float sums = (float)sum;
+  if (loss) {
+    if (lab < 0 || lab >= O) {
+      loss[b] = __builtin_nanf("");
+    } else {
+      const float el = (float)exp(-(double)t[((size_t)b * O + lab) * K] / tau);
+      loss[b] = (float)(-log((double)el / (double)sums));
+    }
+  }
+  if (errors) {
+    for (int j = 0; j < O; ++j) {
+      const float e = (float)exp(-(double)t[((size_t)b * O + j) * K] / tau);
+      float neg = (float)(-(double)e / (double)sums);
+      if (j == lab) neg = (float)((double)neg + 1.0);
+      neg = (float)((double)neg / tau);
+      float *er = errors + ((size_t)b * O + j) * K;
+      er[0] = nan_to_num(neg);
+      for (int k = 1; k < K; ++k) er[k] = 0.f;
+    }
+  }
+  if (pred) {
+    // argmin over neurons for every slot k (the reference flattens [B, K])
+    for (int k = 0; k < K; ++k) {
+      int best = 0;
+      float bv = t[((size_t)b * O) * K + k];
+      for (int j = 1; j < O; ++j) {
+        const float v = t[((size_t)b * O + j) * K + k];
+        if (v < bv) {
+          bv = v;
+          best = j;
+        }
+      }
+      pred[(size_t)b * K + k] = best;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- optimizers
+// bats/Optimizers/AdamOptimizer.py:7-19,35-55.
+__global__ __launch_bounds__(kThreads) void adam_kernel(
+    int64_t n, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+    float *__restrict__ delta, float *__restrict__ w, double lr, double b1, double omb1,
+    double b2, double omb2, double eps, double bias1, double bias2, int nadam) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double gi = (double)g[i];
+    const float mi = (float)(b1 * (double)m[i] + omb1 * gi);
+    const float vi = (float)(b2 * (double)v[i] + omb2 * gi * gi);
+    m[i] = mi;
+    v[i] = vi;
+    const double mh = (double)(float)((double)mi / bias1);
+    const double vh = (double)(float)((double)vi / bias2);
+    float d;
+    if (nadam)
+      d = (float)(-(lr / (sqrt(vh) + eps)) * (b1 * mh + omb1 * gi / bias1));
+    else
+      d = (float)(-(lr * mh / (sqrt(vh) + eps)));
+    if (delta) delta[i] = d;
+    if (w) w[i] = w[i] + d;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void sgd_kernel(int64_t n, const float *__restrict__ g,
+                                                        float *__restrict__ delta,
+                                                        float *__restrict__ w, double lr) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float d = (float)(-lr * (double)g[i]);
+    if (delta) delta[i] = d;
+    if (w) w[i] = w[i] + d;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void add_kernel(int64_t n, float *__restrict__ w,
+                                                       const float *__restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = w[i] + d[i];
+}
+
+// ---------------------------------------------------------------- event lists
+// bats/CudaKernels/CuFiles/Inference/convert_to_spike_per_neuron.cu:9-24. One thread per
+// sample keeps the reference's sequential per-sample order (the list is time-ordered, so
+// each neuron's buffer comes out ascending); the +inf fill is a separate coalesced pass.
+__global__ __launch_bounds__(kThreads) void events_to_neuron_kernel(
+    int B, int P, int N, int K, const int *__restrict__ idx, const float *__restrict__ times,
+    float *__restrict__ out_t, int *__restrict__ out_n) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  int *cnt = out_n + (size_t)b * N;
+  float *ot = out_t + (size_t)b * N * K;
+  for (int p = 0; p < P; ++p) {
+    const int i = idx[(size_t)b * P + p];
+    if (i < 0) break;
+    if (i >= N) continue;
+    const int c = cnt[i];
+    if (c < K) ot[(size_t)i * K + c] = times[(size_t)b * P + p];
+    cnt[i] = c + 1;
+  }
+}
+
+}  // namespace bats
+
+using namespace bats;
+
+extern "C" int bats_count_loss(void *stream, int B, int O, int K, const int *n,
+                               const int64_t *labels, float target_true, float target_false,
+                               float *loss, float *errors, int64_t *pred) {
+  BATS_REQUIRE(B >= 0 && O >= 1 && K >= 0, "bad sizes");
+  if (B == 0) return 0;
+  BATS_REQUIRE(n && labels, "null n or labels");
+  hipLaunchKernelGGL(count_loss_kernel, dim3(grid_for(B)), dim3(kThreads), 0, as_stream(stream),
+                     B, O, K, n, labels, (const float *)nullptr, target_true, target_false, loss,
+                     errors, pred);
+  return check_launch("count_loss_kernel");
+}
+
+extern "C" int bats_count_loss_targets(void *stream, int B, int O, int K, const int *n,
+                                       const float *targets, float *loss, float *errors,
+                                       int64_t *pred) {
+  BATS_REQUIRE(B >= 0 && O >= 1 && K >= 0, "bad sizes");
+  if (B == 0) return 0;
+  BATS_REQUIRE(n && targets, "null n or targets");
+  hipLaunchKernelGGL(count_loss_kernel, dim3(grid_for(B)), dim3(kThreads), 0, as_stream(stream),
+                     B, O, K, n, (const int64_t *)nullptr, targets, 0.f, 0.f, loss, errors, pred);
+  return check_launch("count_loss_kernel");
+}
+
+extern "C" int bats_ttfs_loss(void *stream, int B, int O, int K, const float *t,
+                              const int64_t *labels, float tau, float *loss, float *errors,
+                              int64_t *pred) {
+  BATS_REQUIRE(B >= 0 && O >= 1 && K >= 1, "bad sizes");
+  if (B == 0) return 0;
+  BATS_REQUIRE(t, "null t");
+  BATS_REQUIRE(labels || (!loss && !errors), "labels required for loss/errors");
+  hipLaunchKernelGGL(ttfs_loss_kernel, dim3(grid_for(B)), dim3(kThreads), 0, as_stream(stream), B,
+                     O, K, t, labels, (double)tau, loss, errors, pred);
+  return check_launch("ttfs_loss_kernel");
+}
+
+static int adam_like(void *stream, int64_t n, const float *grad, float *m, float *v, float *delta,
+                     float *weights, float lr, float beta_1, float omb1, float beta_2, float omb2,
+                     float eps, double bias1, double bias2, int nadam) {
+  BATS_REQUIRE(n >= 0, "negative size");
+  if (n == 0) return 0;
+  BATS_REQUIRE(grad && m && v && (delta || weights), "null pointer");
+  hipLaunchKernelGGL(adam_kernel, dim3(std::min<unsigned>(grid_for(n), 4096)), dim3(kThreads), 0,
+                     as_stream(stream), n, grad, m, v, delta, weights, (double)lr,
+                     (double)beta_1, (double)omb1, (double)beta_2, (double)omb2, (double)eps,
+                     bias1, bias2, nadam);
+  return check_launch("adam_kernel");
+}
+
+extern "C" int bats_adam_step(void *stream, int64_t n, const float *grad, float *m, float *v,
+                              float *delta, float *weights, float lr, float beta_1,
+                              float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                              float epsilon, double bias1, double bias2) {
+  return adam_like(stream, n, grad, m, v, delta, weights, lr, beta_1, one_minus_beta_1, beta_2,
+                   one_minus_beta_2, epsilon, bias1, bias2, 0);
+}
+
+extern "C" int bats_nadam_step(void *stream, int64_t n, const float *grad, float *m, float *v,
+                               float *delta, float *weights, float lr, float beta_1,
+                               float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                               float epsilon, double bias1, double bias2) {
+  return adam_like(stream, n, grad, m, v, delta, weights, lr, beta_1, one_minus_beta_1, beta_2,
+                   one_minus_beta_2, epsilon, bias1, bias2, 1);
+}
+
+extern "C" int bats_sgd_step(void *stream, int64_t n, const float *grad, float *delta,
+                             float *weights, float lr) {
+  BATS_REQUIRE(n >= 0, "negative size");
+  if (n == 0) return 0;
+  BATS_REQUIRE(grad && (delta || weights), "null pointer");
+  hipLaunchKernelGGL(sgd_kernel, dim3(std::min<unsigned>(grid_for(n), 4096)), dim3(kThreads), 0,
+                     as_stream(stream), n, grad, delta, weights, (double)lr);
+  return check_launch("sgd_kernel");
+}
+
+extern "C" int bats_add_deltas(void *stream, int64_t n, float *weights, const float *delta) {
+  BATS_REQUIRE(n >= 0, "negative size");
+  if (n == 0) return 0;
+  BATS_REQUIRE(weights && delta, "null pointer");
+  hipLaunchKernelGGL(add_kernel, dim3(std::min<unsigned>(grid_for(n), 4096)), dim3(kThreads), 0,
+                     as_stream(stream), n, weights, delta);
+  return check_launch("add_kernel");
+}
+
+extern "C" int bats_events_to_per_neuron(void *stream, int B, int P, int N, int K,
+                                         const int *indices, const float *times,
+                                         float *out_times, int *out_counts) {
+  BATS_REQUIRE(B >= 0 && P >= 0 && N >= 0 && K >= 0, "bad sizes");
+  if (B == 0 || N == 0) return 0;
+  BATS_REQUIRE(out_times && out_counts, "null output");
+  BATS_REQUIRE(P == 0 || (indices && times), "null input");
+  hipStream_t st = as_stream(stream);
+  const int64_t nt = (int64_t)B * N * K, nc = (int64_t)B * N;
+  if (nt > 0) {
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(std::min<unsigned>(grid_for(nt), 4096)),
+                       dim3(kThreads), 0, st, out_times, nt, kInfF);
+    if (int r = check_launch("fill times")) return r;
+  }
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(std::min<unsigned>(grid_for(nc), 4096)), dim3(kThreads),
+                     0, st, out_counts, nc, 0);
+  if (int r = check_launch("fill counts")) return r;
+  if (P > 0) {
+    hipLaunchKernelGGL(events_to_neuron_kernel, dim3(grid_for(B)), dim3(kThreads), 0, st, B, P, N,
+                       K, indices, times, out_times, out_counts);
+    if (int r = check_launch("events_to_neuron_kernel")) return r;
+  }
+  return 0;
+}

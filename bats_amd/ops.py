@@ -1,0 +1,211 @@
+"""Functional operators over torch device tensors, each one C-ABI call into libbats_amd.
+
+These are the replacements of the reference's kernel wrappers
+(bats/CudaKernels/Wrappers/**). Every operator allocates its outputs with torch's caching
+allocator, launches on torch's current stream and returns without synchronising.
+"""
+from __future__ import annotations
+
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+
+F32 = torch.float32
+I32 = torch.int32
+I64 = torch.int64
+
+
+def _require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("bats_amd operators need a ROCm GPU; none is visible")
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t: Optional[torch.Tensor], dtype=None) -> Optional[int]:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def as_device(x, dtype, device=None) -> torch.Tensor:
+    """Host/any tensor -> contiguous device tensor of ``dtype`` (no copy when already so)."""
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device).contiguous()
+
+
+def _workspace(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+# ---------------------------------------------------------------- dense LIF
+def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float32,
+                tau: np.float32, c: np.float32, delta_theta_tau: np.float32,
+                max_simulation: float, max_n_post: int):
+    """Dense LIF forward (bats/Layers/LIFLayer.py:63-91).
+
+    Returns (n [B,J] i32, t [B,J,K] f32 +inf padded, a, x, z [B,J,K], pre_exp_tau_s,
+    pre_exp_tau [B,I,K_pre]).
+    """
+    _require_gpu()
+    B, I, Kp = pre_times.shape
+    J, I2 = weights.shape
+    if I2 != I:
+        raise ValueError(f"weights are [{J},{I2}] but the previous layer has {I} neurons")
+    K = int(max_n_post)
+    dev = pre_times.device
+    n = torch.empty((B, J), dtype=I32, device=dev)
+    t = torch.empty((B, J, K), dtype=F32, device=dev)
+    a = torch.empty((B, J, K), dtype=F32, device=dev)
+    x = torch.empty((B, J, K), dtype=F32, device=dev)
+    z = torch.empty((B, J, K), dtype=F32, device=dev)
+    es = torch.empty((B, I, Kp), dtype=F32, device=dev)
+    e = torch.empty((B, I, Kp), dtype=F32, device=dev)
+    lib = _lib.load()
+    ws_bytes = lib.bats_lif_forward_workspace_size(B, I, Kp, J)
+    ws = _workspace(ws_bytes, dev)
+    _lib.check(lib.bats_lif_forward(
+        _stream(), B, I, Kp, J, K, _ptr(pre_times, F32), _ptr(weights, F32),
+        float(tau_s), float(tau), float(c), float(delta_theta_tau), float(np.float32(max_simulation)),
+        _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), _ptr(es), _ptr(e), _ptr(ws), ws_bytes))
+    return n, t, a, x, z, es, e
+
+
+def lif_backward(n, t, a, x, z, errors, pre_times, pre_es, pre_e, weights, tau_s, tau, c,
+                 delta_theta_tau, per_sample: bool = False, pre_trainable: bool = True):
+    """Dense LIF BATS backward (bats/Layers/LIFLayer.py:93-110). ``errors`` is updated in
+    place with the reset-chain errors. Returns (grad, pre_errors | None); grad is the batch
+    mean [J,I] unless ``per_sample`` ([B,J,I])."""
+    _require_gpu()
+    B, J, K = t.shape
+    _, I, Kp = pre_times.shape
+    dev = t.device
+    grad = torch.empty((B, J, I) if per_sample else (J, I), dtype=F32, device=dev)
+    pre_err = torch.empty((B, I, Kp), dtype=F32, device=dev) if pre_trainable else None
+    lib = _lib.load()
+    ws_bytes = lib.bats_lif_backward_workspace_size(B, I, J, K)
+    ws = _workspace(ws_bytes, dev)
+    _lib.check(lib.bats_lif_backward(
+        _stream(), B, I, Kp, J, K, _ptr(n, I32), _ptr(t, F32), _ptr(a, F32), _ptr(x, F32),
+        _ptr(z, F32), _ptr(errors, F32), _ptr(pre_times, F32), _ptr(pre_es, F32),
+        _ptr(pre_e, F32), _ptr(weights, F32), float(tau_s), float(tau), float(c),
+        float(delta_theta_tau), _ptr(grad), 1 if per_sample else 0, _ptr(pre_err), _ptr(ws),
+        ws_bytes))
+    return grad, pre_err
+
+
+# ---------------------------------------------------------------- losses
+def count_loss(n: torch.Tensor, K: int, labels=None, target_true=None, target_false=None,
+               targets=None, want_loss=True, want_errors=True, want_pred=False):
+    """Spike-count loss (bats/Losses/SpikeCountLoss.py:17-38); targets either from labels
+    (SpikeCountClassLoss.py:15-18) or given [B,O]."""
+    _require_gpu()
+    B, O = n.shape
+    dev = n.device
+    loss = torch.empty((B,), dtype=F32, device=dev) if want_loss else None
+    err = torch.empty((B, O, int(K)), dtype=F32, device=dev) if want_errors else None
+    pred = torch.empty((B,), dtype=I64, device=dev) if want_pred else None
+    lib = _lib.load()
+    if targets is None:
+        lab = as_device(labels, I64, dev)
+        _lib.check(lib.bats_count_loss(_stream(), B, O, int(K), _ptr(n, I32), _ptr(lab),
+                                       float(np.float32(target_true)),
+                                       float(np.float32(target_false)), _ptr(loss), _ptr(err),
+                                       _ptr(pred)))
+    else:
+        tg = as_device(targets, F32, dev)
+        _lib.check(lib.bats_count_loss_targets(_stream(), B, O, int(K), _ptr(n, I32), _ptr(tg),
+                                               _ptr(loss), _ptr(err), _ptr(pred)))
+    return loss, err, pred
+
+
+def ttfs_loss(t: torch.Tensor, labels=None, tau: float = 1.0, want_loss=True,
+              want_errors=True, want_pred=False):
+    """Softmax cross-entropy on first spike times (bats/Losses/TTFSSoftmaxCrossEntropy.py:20-56)."""
+    _require_gpu()
+    B, O, K = t.shape
+    dev = t.device
+    lab = as_device(labels, I64, dev) if labels is not None else None
+    loss = torch.empty((B,), dtype=F32, device=dev) if want_loss else None
+    err = torch.empty((B, O, K), dtype=F32, device=dev) if want_errors else None
+    pred = torch.empty((B * K,), dtype=I64, device=dev) if want_pred else None
+    lib = _lib.load()
+    _lib.check(lib.bats_ttfs_loss(_stream(), B, O, K, _ptr(t, F32), _ptr(lab),
+                                  float(np.float32(tau)), _ptr(loss), _ptr(err), _ptr(pred)))
+    return loss, err, pred
+
+
+# ---------------------------------------------------------------- optimizers
+def adam_step(grad, m, v, lr, beta_1, omb1, beta_2, omb2, eps, bias1, bias2, nadam=False,
+              weights=None, want_delta=True):
+    _require_gpu()
+    delta = torch.empty_like(grad) if want_delta else None
+    lib = _lib.load()
+    fn = lib.bats_nadam_step if nadam else lib.bats_adam_step
+    _lib.check(fn(_stream(), grad.numel(), _ptr(grad, F32), _ptr(m, F32), _ptr(v, F32),
+                  _ptr(delta), _ptr(weights, F32) if weights is not None else None, float(lr),
+                  float(beta_1), float(omb1), float(beta_2), float(omb2), float(eps),
+                  float(bias1), float(bias2)))
+    return delta
+
+
+def sgd_step(grad, lr, weights=None, want_delta=True):
+    _require_gpu()
+    delta = torch.empty_like(grad) if want_delta else None
+    lib = _lib.load()
+    _lib.check(lib.bats_sgd_step(_stream(), grad.numel(), _ptr(grad, F32), _ptr(delta),
+                                 _ptr(weights, F32) if weights is not None else None, float(lr)))
+    return delta
+
+
+def add_deltas(weights: torch.Tensor, delta: torch.Tensor) -> None:
+    _require_gpu()
+    if delta.numel() != weights.numel():
+        raise ValueError("delta and weights differ in size")
+    d = delta.reshape(weights.shape).contiguous()
+    lib = _lib.load()
+    _lib.check(lib.bats_add_deltas(_stream(), weights.numel(), _ptr(weights, F32), _ptr(d, F32)))
+
+
+# ---------------------------------------------------------------- event lists
+def events_to_per_neuron(indices, times, n_neurons: int, max_per_neuron: Optional[int] = None):
+    """Event list -> per-neuron buffers (bats/CudaKernels/Wrappers/Inference/
+    convert_to_spike_per_neuron.py:20-33). When ``max_per_neuron`` is None the buffer depth
+    is the largest per-neuron multiplicity, which needs one device-to-host read (the
+    reference does a per-sample host loop for the same number)."""
+    _require_gpu()
+    idx = as_device(indices, I32)
+    tim = as_device(times, F32, idx.device)
+    B, P = idx.shape
+    N = int(n_neurons)
+    if max_per_neuron is None:
+        valid = idx >= 0
+        ends = torch.where(valid.all(dim=1), torch.full((B,), P, device=idx.device),
+                           (~valid).int().argmax(dim=1))
+        live = torch.arange(P, device=idx.device)[None, :] < ends[:, None]
+        flat = torch.where(live, idx.long() + torch.arange(B, device=idx.device)[:, None] * N,
+                           torch.full_like(idx, -1, dtype=torch.long))
+        flat = flat[flat >= 0]
+        K = int(torch.bincount(flat, minlength=B * N).max().item()) if flat.numel() else 0
+    else:
+        K = int(max_per_neuron)
+    out_t = torch.empty((B, N, K), dtype=F32, device=idx.device)
+    out_n = torch.empty((B, N), dtype=I32, device=idx.device)
+    lib = _lib.load()
+    _lib.check(lib.bats_events_to_per_neuron(_stream(), B, P, N, K, _ptr(idx), _ptr(tim),
+                                             _ptr(out_t), _ptr(out_n)))
+    return out_t, out_n

@@ -1,0 +1,158 @@
+/*
+ * bats_amd.h -- C ABI of the MI355X (gfx950) BATS hot path.
+ *
+ * One shared library, libbats_amd.so, built with hipcc for gfx950. Every entry point
+ * takes device pointers, sizes and scalars plus a HIP stream (passed as an opaque
+ * pointer; NULL is the default stream) and returns 0 on success or a nonzero status,
+ * with the message available from bats_last_error(). Outputs and workspaces are
+ * allocated by the caller; the library allocates no device memory, never synchronises
+ * the stream and issues no memset, so every call can be captured in a HIP graph.
+ *
+ * Layouts (row-major, float32 unless stated):
+ *   spike times  [B, N, K]  ascending per neuron, +inf padded ("no spike")
+ *   spike counts [B, N]     int32
+ *   dense weights [J, I]    post-major, as the reference's LIFLayer weights
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * reference repository root, Florian-BACHO/bats).
+ */
+#ifndef BATS_AMD_H
+#define BATS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char *bats_last_error(void);
+
+/* Library version string. */
+const char *bats_version(void);
+
+/* ------------------------------------------------------------------ dense LIF forward */
+
+/* Workspace bytes bats_lif_forward needs for these sizes. */
+size_t bats_lif_forward_workspace_size(int B, int I, int K_pre, int J);
+
+/*
+ * Dense LIF layer forward: pre-spike exps, per-sample time sort, weight gather and the
+ * exact multi-spike spike-time solver, fused.
+ * Replaces, in one call, bats/Layers/LIFLayer.py:63-91 and the operators it launches:
+ *   compute_pre_exps        bats/CudaKernels/Wrappers/Inference/compute_pre_exps.py:10
+ *   get_sorted_spikes_indices  bats/CudaKernels/Wrappers/Inference/get_sorted_spikes_indices.py:7
+ *   get_spike_weights       bats/CudaKernels/Wrappers/Inference/get_spike_weights.py:9
+ *   compute_spike_times     bats/CudaKernels/Wrappers/Inference/compute_spike_times.py:11
+ * Inputs : pre_times [B,I,K_pre]; weights [J,I]; tau_s, tau = 2 tau_s, c = theta/tau,
+ *          delta_theta_tau = delta_theta/tau (float32 as the layer stores them);
+ *          max_simulation (+inf allowed).
+ * Outputs: out_n [B,J] int32; out_t [B,J,K_post] (+inf padded); out_a, out_x, out_z
+ *          [B,J,K_post] (valid for slots < out_n, other slots untouched);
+ *          pre_exp_tau_s, pre_exp_tau [B,I,K_pre] (valid for finite pre slots only).
+ */
+int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
+                     const float *pre_times, const float *weights,
+                     float tau_s, float tau, float c, float delta_theta_tau,
+                     float max_simulation,
+                     int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                     float *pre_exp_tau_s, float *pre_exp_tau,
+                     void *workspace, size_t workspace_bytes);
+
+/* ------------------------------------------------------------------ dense LIF backward */
+
+size_t bats_lif_backward_workspace_size(int B, int I, int J, int K_post);
+
+/*
+ * Dense LIF layer BATS backward.
+ * Replaces bats/Layers/LIFLayer.py:93-110 and its operators:
+ *   propagate_recurrent_errors bats/CudaKernels/Wrappers/Backpropagation/propagate_recurrent_errors.py:9
+ *   compute_factors            bats/CudaKernels/Wrappers/Backpropagation/compute_factors.py:11
+ *   compute_weights_gradient   bats/CudaKernels/Wrappers/Backpropagation/compute_weights_gradient.py:10
+ *   propagate_errors_to_pre_spikes bats/CudaKernels/Wrappers/Backpropagation/propagate_errors_to_pre_spikes.py:11
+ * and the caller's batch mean (experiments/mnist/train_spike_count.py:167).
+ * errors [B,J,K_post] is updated in place with the reset-chain errors, as the reference's
+ * propagate_recurrent_errors does (only slots holding a spike change).
+ * grad_mode 0: grad is [J,I] = batch mean of the per-sample gradients.
+ * grad_mode 1: grad is [B,J,I], the per-sample gradients of the reference.
+ * pre_errors [B,I,K_pre] may be NULL (previous layer not trainable).
+ */
+int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, int K_post,
+                      const int *n, const float *t, const float *a, const float *x,
+                      const float *z, float *errors,
+                      const float *pre_times, const float *pre_exp_tau_s,
+                      const float *pre_exp_tau, const float *weights,
+                      float tau_s, float tau, float c, float delta_theta_tau,
+                      float *grad, int grad_mode, float *pre_errors,
+                      void *workspace, size_t workspace_bytes);
+
+/* ------------------------------------------------------------------ losses */
+
+/*
+ * Spike-count class loss. Replaces bats/Losses/SpikeCountClassLoss.py:15-33 with
+ * bats/Losses/SpikeCountLoss.py:17-38: loss[b] = 0.5*sum_j (target - n)^2,
+ * errors[b,j,k] = target - n, pred[b] = argmax_j n. Any output pointer may be NULL.
+ */
+int bats_count_loss(void *stream, int B, int O, int K, const int *n, const int64_t *labels,
+                    float target_true, float target_false,
+                    float *loss, float *errors, int64_t *pred);
+
+/*
+ * Generic spike-count loss with explicit targets [B,O] (bats/Losses/SpikeCountLoss.py:20-38).
+ */
+int bats_count_loss_targets(void *stream, int B, int O, int K, const int *n,
+                            const float *targets, float *loss, float *errors, int64_t *pred);
+
+/*
+ * Softmax cross-entropy on the first spike time. Replaces
+ * bats/Losses/TTFSSoftmaxCrossEntropy.py:20-56. t is [B,O,K]; errors [B,O,K] (slot 0 set,
+ * others 0); pred has B*K entries (argmin over neurons, as the reference flattens it).
+ */
+int bats_ttfs_loss(void *stream, int B, int O, int K, const float *t, const int64_t *labels,
+                   float tau, float *loss, float *errors, int64_t *pred);
+
+/* ------------------------------------------------------------------ optimizers */
+
+/*
+ * Adam deltas. Replaces bats/Optimizers/AdamOptimizer.py:35-55 for one tensor of n
+ * elements: m, v updated in place, delta = -lr * m_hat / (sqrt(v_hat) + eps) with
+ * bias1 = 1 - beta_1^t and bias2 = 1 - beta_2^t. If weights is not NULL the delta is also
+ * added to it (the fused form of bats/Network.py:50-54).
+ */
+int bats_adam_step(void *stream, int64_t n, const float *grad, float *m, float *v,
+                   float *delta, float *weights, float lr, float beta_1,
+                   float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                   float epsilon, double bias1, double bias2);
+
+/* NAdam deltas. Replaces bats/Optimizers/NAdamOptimizer.py:36-58. */
+int bats_nadam_step(void *stream, int64_t n, const float *grad, float *m, float *v,
+                    float *delta, float *weights, float lr, float beta_1,
+                    float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                    float epsilon, double bias1, double bias2);
+
+/* Gradient-descent deltas. Replaces bats/Optimizers/GradientDescentOptimizer.py:8-9. */
+int bats_sgd_step(void *stream, int64_t n, const float *grad, float *delta,
+                  float *weights, float lr);
+
+/* weights += delta. Replaces bats/Layers/LIFLayer.py:112-113 (add_deltas). */
+int bats_add_deltas(void *stream, int64_t n, float *weights, const float *delta);
+
+/* ------------------------------------------------------------------ event lists */
+
+/*
+ * Time-ordered event list -> per-neuron buffers. Replaces
+ * bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33.
+ * indices [B,P] int32 (a negative index ends the list), times [B,P].
+ * out_times [B,N,K] is fully written (+inf padded); out_counts [B,N] counts every
+ * event of the neuron, events beyond K are dropped.
+ */
+int bats_events_to_per_neuron(void *stream, int B, int P, int N, int K,
+                              const int *indices, const float *times,
+                              float *out_times, int *out_counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BATS_AMD_H */

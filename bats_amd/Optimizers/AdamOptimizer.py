@@ -13,8 +13,13 @@ from ..AbstractOptimizer import AbstractOptimizer
 class AdamOptimizer(AbstractOptimizer):
     _nadam = False
 
-    def __init__(self, beta_1=0.9, beta_2=0.999, epsilon=1e-8, **kwargs):
+    def __init__(self, beta_1=0.9, beta_2=0.999, epsilon=1e-8, capturable: bool = False,
+                 **kwargs):
+        """``capturable=True`` keeps the step counter on the device so that a HIP graph of the
+        whole training step can be replayed (the reference keeps it on the host)."""
         super().__init__(**kwargs)
+        self._capturable = bool(capturable)
+        self._t_dev: Optional[torch.Tensor] = None
         self._beta_1 = np.float32(beta_1)
         self._one_minus_beta_1 = np.float32(1.0 - beta_1)
         self._beta_2 = np.float32(beta_2)
@@ -27,7 +32,8 @@ class AdamOptimizer(AbstractOptimizer):
     @property
     def state(self):
         """(m, v, t) -- the moment estimates and the step counter."""
-        return self._m, self._v, self._t
+        t = int(self._t_dev.item()) if self._t_dev is not None else self._t
+        return self._m, self._v, t
 
     def load_state(self, m, v, t) -> None:
         self._m, self._v, self._t = m, v, int(t)
@@ -45,6 +51,13 @@ class AdamOptimizer(AbstractOptimizer):
         if self._m is None:
             self._m = [None if g is None else torch.zeros_like(g, dtype=torch.float32) for g in gradient]
             self._v = [None if g is None else torch.zeros_like(g, dtype=torch.float32) for g in gradient]
+        step_dev = None
+        if self._capturable:
+            if self._t_dev is None:
+                dev = next(g.device for g in gradient if g is not None)
+                self._t_dev = torch.zeros((), dtype=torch.int64, device=dev)
+            self._t_dev.add_(1)
+            step_dev = self._t_dev
         b1, b2 = self._bias()
         out = []
         for k, g in enumerate(gradient):
@@ -56,5 +69,5 @@ class AdamOptimizer(AbstractOptimizer):
             out.append(ops.adam_step(g, self._m[k], self._v[k], self._learning_rate,
                                      self._beta_1, self._one_minus_beta_1, self._beta_2,
                                      self._one_minus_beta_2, self._epsilon, b1, b2,
-                                     nadam=self._nadam, weights=w))
+                                     nadam=self._nadam, weights=w, step_dev=step_dev))
         return out

@@ -24,7 +24,8 @@ SIGNATURES = {
     "bats_lif_forward_workspace_size": (_sz, [_i, _i, _i, _i]),
     "bats_lif_forward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
-    "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i]),
+    "bats_set_solver_mode": (_i, [_i]),
+    "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i, _i]),
     "bats_lif_backward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp, _sz]),
     "bats_count_loss": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _vp, _vp, _vp]),
@@ -34,9 +35,14 @@ SIGNATURES = {
                             _d, _d]),
     "bats_nadam_step": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f,
                              _d, _d]),
+    "bats_adam_step_dev": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f,
+                                _vp, _i]),
     "bats_sgd_step": (_i, [_vp, _i64, _vp, _vp, _vp, _f]),
     "bats_add_deltas": (_i, [_vp, _i64, _vp, _vp]),
     "bats_events_to_per_neuron": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "bats_timing_enable": (None, [_i]),
+    "bats_timing_reset": (None, []),
+    "bats_timing_query": (_i, [C.c_char_p, C.POINTER(_d), C.POINTER(C.c_longlong)]),
 }
 
 _lib = None

@@ -21,6 +21,21 @@ int check_launch(const char *what);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Optional per-kernel timing with HIP events (off by default; not used under graph capture).
+bool timing_enabled();
+void timing_begin(const char *name, hipStream_t st, hipEvent_t *e0, hipEvent_t *e1);
+void timing_end(const char *name, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+
+// Launch wrapper: records start/stop events around the launch when timing is enabled.
+#define BATS_TIMED(name, st, ...)                               \
+  do {                                                          \
+    hipEvent_t _e0 = nullptr, _e1 = nullptr;                    \
+    const bool _t = ::bats::timing_enabled();                   \
+    if (_t) ::bats::timing_begin(name, st, &_e0, &_e1);         \
+    __VA_ARGS__;                                                \
+    if (_t) ::bats::timing_end(name, st, _e0, _e1);             \
+  } while (0)
+
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // ---------------------------------------------------------------- device helpers
@@ -40,20 +55,21 @@ constexpr float kInfF = __builtin_huge_valf();
 constexpr double kInfD = __builtin_huge_val();
 
 // One time-sorted input event of one sample, read uniformly by every lane of a wave.
-// es/e are the float32 exps exactly as stored for the backward; zq/zl are float64
-// exp(t/tau) and exp(min(t_next, t_max)/tau); czl2 = c*zl*zl; tl = min(t_next, t_max).
+// es/e are the float32 exps exactly as stored for the backward, widened to float64; zq/zl
+// are float64 exp(t/tau) and exp(min(t_next, t_max)/tau); czl2 = c*zl*zl;
+// tl = min(t_next, t_max).
 struct alignas(16) Event {
-  float es;
-  float e;
-  int idx;
-  float t;
+  double es;
+  double e;
   double zq;
   double zl;
   double czl2;
+  int idx;
+  float t;
   float tl;
   float pad;
 };
-static_assert(sizeof(Event) == 48, "Event layout");
+static_assert(sizeof(Event) == 64, "Event layout");
 
 // Fill kernel (used instead of hipMemsetAsync so that every call stays graph-capturable).
 template <typename T>

@@ -25,7 +25,6 @@
 namespace bats {
 
 constexpr int kBwdThreads = 256;
-constexpr int kJTile = 8;
 
 __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
     int BJ, int K, const int *__restrict__ n, const float *__restrict__ t,
@@ -78,23 +77,93 @@ __device__ __forceinline__ void lookup(const float *__restrict__ tpost,
   }
 }
 
-__global__ __launch_bounds__(kBwdThreads) void wgrad_kernel(
+// Post-neuron rows (b, j0 .. j0+jn-1) staged in LDS: spike counts, times and suffix sums of
+// the valid slots only.
+struct RowStage {
+  int *ne;
+  float *t;
+  double *s1;
+  double *s2;
+};
+
+__device__ __forceinline__ RowStage row_stage(char *smem, int JT, int K) {
+  RowStage r;
+  r.s1 = reinterpret_cast<double *>(smem);
+  r.s2 = r.s1 + (size_t)JT * K;
+  r.t = reinterpret_cast<float *>(r.s2 + (size_t)JT * K);
+  r.ne = reinterpret_cast<int *>(r.t + (size_t)JT * K);
+  return r;
+}
+
+__host__ __device__ constexpr size_t row_stage_bytes(int JT, int K) {
+  return (size_t)JT * K * (2 * sizeof(double) + sizeof(float)) + (size_t)JT * sizeof(int);
+}
+
+// Loads rows (b, j0 ..) into the stage. Contains barriers: call from every thread.
+__device__ __forceinline__ void load_rows(const RowStage &st, int b, int J, int j0, int jn, int K,
+                                          const float *__restrict__ post_t,
+                                          const int *__restrict__ n_eff,
+                                          const double *__restrict__ S1,
+                                          const double *__restrict__ S2) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  __syncthreads();  // previous contents consumed
+  if (tid < jn) st.ne[tid] = n_eff[(size_t)b * J + j0 + tid];
+  __syncthreads();
+  const size_t base = ((size_t)b * J + j0) * K;
+  for (int e = tid; e < jn * K; e += nth) {
+    const int r = e / K;
+    const int k = e - r * K;
+    if (k < st.ne[r]) {
+      st.t[e] = post_t[base + e];
+      st.s1[e] = S1[base + e];
+      st.s2[e] = S2[base + e];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void lookup_lds(const RowStage &st, int r, int K, float tp,
+                                           double &v1, double &v2) {
+  v1 = 0.0;
+  v2 = 0.0;
+  const int nj = st.ne[r];
+  const float *tr = st.t + r * K;
+  const double *s1 = st.s1 + r * K;
+  const double *s2 = st.s2 + r * K;
+  for (int k = nj - 1; k >= 0; --k) {
+    if (tr[k] > tp) {
+      v1 = s1[k];
+      v2 = s2[k];
+    }
+  }
+}
+
+// Lanes are pre neurons (a block covers up to 1024 of them), a block owns kWJT post neurons
+// and a slice of the batch. Per sample the block stages its post rows in LDS; every lane
+// accumulates es*V1 - e*V2 over its pre spikes in float64.
+constexpr int kWJT = 16;
+
+__global__ __launch_bounds__(1024) void wgrad_kernel(
     int B, int I, int K_pre, int J, int K, int b_per_slice, const float *__restrict__ pre_t,
     const float *__restrict__ pre_es, const float *__restrict__ pre_e,
     const float *__restrict__ post_t, const int *__restrict__ n_eff,
     const double *__restrict__ S1, const double *__restrict__ S2, int per_sample,
-    float *__restrict__ grad, double *__restrict__ partial) {
-  const int i = blockIdx.x * kBwdThreads + threadIdx.x;
+    float *__restrict__ grad, float *__restrict__ partial) {
+  extern __shared__ __align__(16) char smem[];
+  const RowStage st = row_stage(smem, kWJT, K);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < I;
   const int ic = active ? i : 0;
-  const int j0 = blockIdx.y * kJTile;
+  const int j0 = blockIdx.y * kWJT;
+  const int jn = min(kWJT, J - j0);
   const int slice = blockIdx.z;
   const int b0 = slice * b_per_slice;
   const int b1 = min(B, b0 + b_per_slice);
-  double acc[kJTile];
+  double acc[kWJT];
 #pragma unroll
-  for (int u = 0; u < kJTile; ++u) acc[u] = 0.0;
+  for (int u = 0; u < kWJT; ++u) acc[u] = 0.0;
   for (int b = b0; b < b1; ++b) {
+    load_rows(st, b, J, j0, jn, K, post_t, n_eff, S1, S2);
     const size_t pbase = ((size_t)b * I + ic) * K_pre;
     for (int m = 0; m < K_pre; ++m) {
       const float tp = active ? pre_t[pbase + m] : kInfF;
@@ -102,89 +171,106 @@ __global__ __launch_bounds__(kBwdThreads) void wgrad_kernel(
       const double es = (double)pre_es[pbase + m];
       const double e = (double)pre_e[pbase + m];
 #pragma unroll
-      for (int u = 0; u < kJTile; ++u) {
-        const int j = j0 + u;
-        if (j >= J) break;
-        const size_t prow = (size_t)b * J + j;
-        const int nj = n_eff[prow];
-        if (nj == 0) continue;
-        double v1, v2;
-        lookup(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
-        acc[u] = fma(es, v1, acc[u]);
-        acc[u] = fma(-e, v2, acc[u]);
+      for (int u = 0; u < kWJT; ++u) {
+        if (u < jn && st.ne[u] > 0) {
+          double v1, v2;
+          lookup_lds(st, u, K, tp, v1, v2);
+          acc[u] = fma(es, v1, acc[u]);
+          acc[u] = fma(-e, v2, acc[u]);
+        }
       }
     }
     if (per_sample) {
 #pragma unroll
-      for (int u = 0; u < kJTile; ++u) {
-        const int j = j0 + u;
-        if (j < J && active) grad[((size_t)b * J + j) * I + i] = (float)acc[u];
+      for (int u = 0; u < kWJT; ++u) {
+        if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
         acc[u] = 0.0;
       }
     }
   }
   if (!per_sample && active) {
 #pragma unroll
-    for (int u = 0; u < kJTile; ++u) {
-      const int j = j0 + u;
-      if (j < J) partial[((size_t)slice * J + j) * I + i] = acc[u];
-    }
+    for (int u = 0; u < kWJT; ++u)
+      if (u < jn) partial[((size_t)slice * J + j0 + u) * I + i] = (float)acc[u];
   }
 }
 
 __global__ __launch_bounds__(kBwdThreads) void wgrad_reduce_kernel(
-    int slices, int64_t JI, double inv_b, const double *__restrict__ partial,
+    int slices, int64_t JI, double inv_b, const float *__restrict__ partial,
     float *__restrict__ grad) {
   const int64_t e = (int64_t)blockIdx.x * kBwdThreads + threadIdx.x;
   if (e >= JI) return;
   double s = 0.0;
-  for (int z = 0; z < slices; ++z) s += partial[(size_t)z * JI + e];
+  for (int z = 0; z < slices; ++z) s += (double)partial[(size_t)z * JI + e];
   grad[e] = (float)(s * inv_b);
 }
 
-__global__ __launch_bounds__(kBwdThreads) void pre_err_kernel(
+// Lanes are pre neurons of one sample; the post rows are staged kWJT at a time and every
+// pre spike of the lane accumulates W[j,i] (V1 es/tau_s - V2 e/tau) over all post neurons.
+// Pre-spike sums of one lane are kept in float64 scratch rows across the row chunks.
+__global__ __launch_bounds__(1024) void pre_err_kernel(
     int I, int K_pre, int J, int K, const float *__restrict__ pre_t,
     const float *__restrict__ pre_es, const float *__restrict__ pre_e,
     const float *__restrict__ W, const float *__restrict__ post_t,
     const int *__restrict__ n_eff, const double *__restrict__ S1,
-    const double *__restrict__ S2, double tau_s, double tau, float *__restrict__ pre_errors) {
-  const int i = blockIdx.x * kBwdThreads + threadIdx.x;
+    const double *__restrict__ S2, double tau_s, double tau, double *__restrict__ acc_scratch,
+    float *__restrict__ pre_errors) {
+  extern __shared__ __align__(16) char smem[];
+  const RowStage st = row_stage(smem, kWJT, K);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
-  if (i >= I) return;
-  const size_t pbase = ((size_t)b * I + i) * K_pre;
-  for (int m = 0; m < K_pre; ++m) {
-    const float tp = pre_t[pbase + m];
-    if (!isfinite(tp)) {
+  const bool active = i < I;
+  const int ic = active ? i : 0;
+  const size_t pbase = ((size_t)b * I + ic) * K_pre;
+  // number of leading finite pre spikes of this lane
+  int np = 0;
+  if (active)
+    while (np < K_pre && isfinite(pre_t[pbase + np])) ++np;
+  const bool single_chunk = J <= kWJT;
+  for (int j0 = 0; j0 < J; j0 += kWJT) {
+    const int jn = min(kWJT, J - j0);
+    load_rows(st, b, J, j0, jn, K, post_t, n_eff, S1, S2);
+    for (int m = 0; m < np; ++m) {
+      const float tp = pre_t[pbase + m];
+      const double es = (double)pre_es[pbase + m];
+      const double e = (double)pre_e[pbase + m];
+      double acc = (j0 == 0) ? 0.0 : acc_scratch[pbase + m];
+      for (int u = 0; u < jn; ++u) {
+        if (st.ne[u] == 0) continue;
+        double v1, v2;
+        lookup_lds(st, u, K, tp, v1, v2);
+        const double w = (double)W[(size_t)(j0 + u) * I + i];
+        acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
+      }
+      if (single_chunk || j0 + kWJT >= J)
+        pre_errors[pbase + m] = (float)acc;
+      else
+        acc_scratch[pbase + m] = acc;
+    }
+  }
+  if (active) {
+    // slots without a spike (the reference's pre_errors are zero there)
+    for (int m = np; m < K_pre; ++m) {
+      const float tp = pre_t[pbase + m];
       pre_errors[pbase + m] = 0.f;
-      continue;
+      (void)tp;
     }
-    const double es = (double)pre_es[pbase + m];
-    const double e = (double)pre_e[pbase + m];
-    double acc = 0.0;
-    for (int j = 0; j < J; ++j) {
-      const size_t prow = (size_t)b * J + j;
-      const int nj = n_eff[prow];
-      if (nj == 0) continue;
-      double v1, v2;
-      lookup(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
-      const double w = (double)W[(size_t)j * I + i];
-      acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
-    }
-    pre_errors[pbase + m] = (float)acc;
   }
 }
 
 struct BwdLayout {
-  size_t neff, s1, s2, partial, total;
-  int slices, b_per_slice;
+  size_t neff, s1, s2, partial, scratch, total;
+  int slices, b_per_slice, lanes, gi;
 };
 
-static BwdLayout bwd_layout(int B, int I, int J, int K) {
+static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K) {
   BwdLayout L{};
-  const int gi = (I + kBwdThreads - 1) / kBwdThreads;
-  const int gj = (J + kJTile - 1) / kJTile;
-  const int base = std::max(1, gi * gj);
-  int slices = std::max(1, std::min(B, (2048 + base - 1) / base));
+  L.gi = std::max(1, (I + 1023) / 1024);
+  const int per_block = (I + L.gi - 1) / L.gi;
+  L.lanes = std::max(64, (per_block + 63) / 64 * 64);
+  const int gj = (J + kWJT - 1) / kWJT;
+  const int base = std::max(1, L.gi * gj);
+  int slices = std::max(1, std::min(B, (512 + base - 1) / base));
   L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
   L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
   size_t off = 0;
@@ -195,7 +281,9 @@ static BwdLayout bwd_layout(int B, int I, int J, int K) {
   L.s2 = off;
   off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
   L.partial = off;
-  off = align_up(off + sizeof(double) * (size_t)L.slices * J * I, 256);
+  off = align_up(off + sizeof(float) * (size_t)L.slices * J * I, 256);
+  L.scratch = off;
+  if (J > kWJT) off = align_up(off + sizeof(double) * (size_t)B * I * K_pre, 256);
   L.total = off;
   return L;
 }
@@ -204,10 +292,13 @@ static BwdLayout bwd_layout(int B, int I, int J, int K) {
 
 using namespace bats;
 
-extern "C" size_t bats_lif_backward_workspace_size(int B, int I, int J, int K_post) {
-  if (B < 0 || I < 0 || J < 0 || K_post < 0) return 0;
-  return bwd_layout(B, I, J, K_post).total;
+extern "C" size_t bats_lif_backward_workspace_size(int B, int I, int K_pre, int J, int K_post) {
+  if (B < 0 || I < 0 || K_pre < 0 || J < 0 || K_post < 0) return 0;
+  return bwd_layout(B, I, K_pre, J, K_post).total;
 }
+
+// Largest post-spike buffer the backward's LDS row stage supports.
+constexpr int kMaxBackwardK = 384;
 
 extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, int K_post,
                                  const int *n, const float *t, const float *a, const float *x,
@@ -217,6 +308,7 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                                  float delta_theta_tau, float *grad, int grad_mode,
                                  float *pre_errors, void *workspace, size_t workspace_bytes) {
   BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0 && K_post >= 1, "bad sizes");
+  BATS_REQUIRE(K_post <= kMaxBackwardK, "max_n_spike above 384 is not supported by the backward");
   BATS_REQUIRE(grad_mode == 0 || grad_mode == 1, "grad_mode must be 0 (mean) or 1 (per sample)");
   hipStream_t st = as_stream(stream);
   const int64_t JI = (int64_t)J * I;
@@ -231,14 +323,16 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   }
   BATS_REQUIRE(n && t && a && x && z && errors, "null post-layer pointer");
   BATS_REQUIRE(grad || JI == 0, "null grad");
-  const BwdLayout L = bwd_layout(B, I, J, K_post);
+  const BwdLayout L = bwd_layout(B, I, K_pre, J, K_post);
   BATS_REQUIRE(workspace != nullptr || L.total == 0, "null workspace");
   BATS_REQUIRE(workspace_bytes >= L.total, "workspace too small");
   char *ws = static_cast<char *>(workspace);
   int *n_eff = reinterpret_cast<int *>(ws + L.neff);
   double *S1 = reinterpret_cast<double *>(ws + L.s1);
   double *S2 = reinterpret_cast<double *>(ws + L.s2);
-  double *partial = reinterpret_cast<double *>(ws + L.partial);
+  float *partial = reinterpret_cast<float *>(ws + L.partial);
+  double *scratch = reinterpret_cast<double *>(ws + L.scratch);
+  const size_t lds = row_stage_bytes(kWJT, K_post);
 
   const int BJ = B * J;
   if (BJ > 0) {
@@ -249,20 +343,19 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   }
   if (JI > 0) {
     BATS_REQUIRE(K_pre == 0 || (pre_times && pre_exp_tau_s && pre_exp_tau), "null pre pointer");
-    const int gi = (I + kBwdThreads - 1) / kBwdThreads;
-    const int gj = (J + kJTile - 1) / kJTile;
+    const int gj = (J + kWJT - 1) / kWJT;
     if (grad_mode == 1) {
-      // one sample per workgroup
-      dim3 g(gi, gj, B);
-      hipLaunchKernelGGL(wgrad_kernel, g, dim3(kBwdThreads), 0, st, B, I, K_pre, J, K_post, 1,
+      dim3 g(L.gi, gj, B);  // one sample per block
+      hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds, st, B, I, K_pre, J, K_post, 1,
                          pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1, S2, 1, grad,
                          partial);
       if (int r = check_launch("wgrad_kernel")) return r;
     } else {
-      dim3 g(gi, gj, L.slices);
-      hipLaunchKernelGGL(wgrad_kernel, g, dim3(kBwdThreads), 0, st, B, I, K_pre, J, K_post,
-                         L.b_per_slice, pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1,
-                         S2, 0, grad, partial);
+      dim3 g(L.gi, gj, L.slices);
+      BATS_TIMED("wgrad_kernel", st,
+                 hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds, st, B, I, K_pre, J,
+                                    K_post, L.b_per_slice, pre_times, pre_exp_tau_s, pre_exp_tau,
+                                    t, n_eff, S1, S2, 0, grad, partial));
       if (int r = check_launch("wgrad_kernel")) return r;
       hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
                          dim3(kBwdThreads), 0, st, L.slices, JI, 1.0 / (double)B, partial, grad);
@@ -271,10 +364,11 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   }
   if (pre_errors && I > 0 && K_pre > 0) {
     BATS_REQUIRE(weights, "null weights");
-    dim3 g((I + kBwdThreads - 1) / kBwdThreads, B);
-    hipLaunchKernelGGL(pre_err_kernel, g, dim3(kBwdThreads), 0, st, I, K_pre, J, K_post,
-                       pre_times, pre_exp_tau_s, pre_exp_tau, weights, t, n_eff, S1, S2,
-                       (double)tau_s, (double)tau, pre_errors);
+    dim3 g(L.gi, B);
+    BATS_TIMED("pre_err_kernel", st,
+               hipLaunchKernelGGL(pre_err_kernel, g, dim3(L.lanes), lds, st, I, K_pre, J, K_post,
+                                  pre_times, pre_exp_tau_s, pre_exp_tau, weights, t, n_eff, S1,
+                                  S2, (double)tau_s, (double)tau, scratch, pre_errors));
     if (int r = check_launch("pre_err_kernel")) return r;
   }
   return 0;

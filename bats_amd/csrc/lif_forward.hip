@@ -118,8 +118,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const double zq = exp((double)t / dtau);
     const double zl = exp((double)tl / dtau);
     Event e;
-    e.es = (float)es64;
-    e.e = (float)zq;
+    const float es32 = (float)es64, e32 = (float)zq;
+    e.es = (double)es32;
+    e.e = (double)e32;
     e.idx = s / K_pre;
     e.t = t;
     e.zq = zq;
@@ -128,8 +129,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     e.tl = tl;
     e.pad = 0.f;
     ev[q] = e;
-    pre_exp_s[(size_t)b * S + s] = e.es;
-    pre_exp[(size_t)b * S + s] = e.e;
+    pre_exp_s[(size_t)b * S + s] = es32;
+    pre_exp[(size_t)b * S + s] = e32;
   }
   if (threadIdx.x == 0) counts[b] = P;
 }
@@ -160,7 +161,7 @@ __device__ __forceinline__ bool solve_crossings(double a, double &b, int &n, dou
                                                 int K, size_t row, float *__restrict__ out_t,
                                                 float *__restrict__ out_a,
                                                 float *__restrict__ out_x,
-                                                float *__restrict__ out_z) {
+                                                float *__restrict__ out_z, bool writer = true) {
   while (true) {
     const double disc = b * b - (4.0 * a) * c;
     if (disc < 0.0) return true;
@@ -171,10 +172,12 @@ __device__ __forceinline__ bool solve_crossings(double a, double &b, int &n, dou
     if (z < 0.0) return true;
     const double t = tau * log(z);
     if (t <= last || t > tl) return true;
-    out_a[row + n] = (float)a;
-    out_x[row + n] = (float)x;
-    out_t[row + n] = (float)t;
-    out_z[row + n] = (float)z;
+    if (writer) {
+      out_a[row + n] = (float)a;
+      out_x[row + n] = (float)x;
+      out_t[row + n] = (float)t;
+      out_z[row + n] = (float)z;
+    }
     last = t;
     b = b - dtt * z;
     ++n;
@@ -182,15 +185,38 @@ __device__ __forceinline__ bool solve_crossings(double a, double &b, int &n, dou
   }
 }
 
+// Out-of-line copy of the closed-form loop for the lane-per-neuron solver, whose event loop
+// is fully unrolled per chunk (the loop is entered rarely: only when the filter passes).
+struct SolveState {
+  double b;
+  int n;
+  int live;
+};
+
+__device__ __attribute__((noinline)) SolveState solve_crossings_ool(
+    double a, double b, int n, double last, double tl, double c, double dtt, double tau, int K,
+    size_t row, float *out_t, float *out_a, float *out_x, float *out_z) {
+  SolveState r;
+  r.live = solve_crossings(a, b, n, last, tl, c, dtt, tau, K, row, out_t, out_a, out_x, out_z);
+  r.b = b;
+  r.n = n;
+  return r;
+}
+
 // ------------------------------------------------------------------ solver, lane per neuron
+// A workgroup is one sample and 256 consecutive post neurons (one lane each). Events are
+// staged through LDS kChunk at a time (one coalesced copy per chunk, read back as wave-wide
+// broadcasts); the weights of a whole chunk are requested before the chunk is processed, so
+// kChunk loads per lane are in flight at once.
 constexpr int kSolveThreads = 256;
-constexpr int kUnroll = 8;
+constexpr int kChunk = 32;
 
 __global__ __launch_bounds__(kSolveThreads) void lif_solve_seq_kernel(
     const Event *__restrict__ events, const int *__restrict__ counts, int Pcap,
     const float *__restrict__ WT, int J, int K, double c, double dtt, double tau,
     int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
     float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
+  __shared__ Event sev[kChunk];
   __shared__ int s_n[kSolveThreads];
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * kSolveThreads;
@@ -199,29 +225,40 @@ __global__ __launch_bounds__(kSolveThreads) void lif_solve_seq_kernel(
   const int jc = active ? j : 0;
   const int P = counts[b];
   const Event *ev = events + (size_t)b * Pcap;
+  const float *wcol = WT + jc;
   const double c4 = 4.0 * c, inv2c = 0.5 / c;
   const size_t row = ((size_t)b * J + jc) * K;
   double a = 0.0, bb = 0.0;
   int n = 0;
   bool live = active;
-  for (int q0 = 0; q0 < P; q0 += kUnroll) {
-    float w[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int q = q0 + u;
-      w[u] = (q < P) ? WT[(size_t)ev[q].idx * J + jc] : 0.f;
+  for (int q0 = 0; q0 < P; q0 += kChunk) {
+    const int cnt = min(kChunk, P - q0);
+    __syncthreads();  // the previous chunk has been consumed by every wave
+    {
+      const uint4 *src = reinterpret_cast<const uint4 *>(ev + q0);
+      uint4 *dst = reinterpret_cast<uint4 *>(sev);
+      constexpr int kVec = sizeof(Event) / sizeof(uint4);
+      for (int v = threadIdx.x; v < cnt * kVec; v += kSolveThreads) dst[v] = src[v];
     }
+    __syncthreads();
+    float w[kChunk];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int q = q0 + u;
-      if (q < P) {
-        const Event e = ev[q];
-        a = fma((double)w[u], (double)e.es, a);
-        bb = fma((double)w[u], (double)e.e, bb);
-        const bool cand = live && (always_exact || crossing_candidate(a, bb, e, c4, inv2c));
-        if (cand)
-          live = solve_crossings(a, bb, n, (double)e.t, (double)e.tl, c, dtt, tau, K, row,
-                                 out_t, out_a, out_x, out_z);
+    for (int u = 0; u < kChunk; ++u)
+      w[u] = (u < cnt) ? wcol[(size_t)sev[u].idx * J] : 0.f;
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      if (u < cnt) {
+        const double w64 = (double)w[u];
+        a = fma(w64, sev[u].es, a);
+        bb = fma(w64, sev[u].e, bb);
+        if (live && (always_exact || crossing_candidate(a, bb, sev[u], c4, inv2c))) {
+          const SolveState r = solve_crossings_ool(a, bb, n, (double)sev[u].t,
+                                                   (double)sev[u].tl, c, dtt, tau, K, row,
+                                                   out_t, out_a, out_x, out_z);
+          bb = r.b;
+          n = r.n;
+          live = r.live != 0;
+        }
       }
     }
   }
@@ -236,7 +273,92 @@ __global__ __launch_bounds__(kSolveThreads) void lif_solve_seq_kernel(
   }
 }
 
+// ------------------------------------------------------------------ solver, wave per neuron
+// One wave per (sample, post neuron); lane l holds event q0 + l of the current 64-event chunk.
+// a_q and b_q come from float64 inclusive wave scans of w*exp terms plus the carry of the
+// previous chunks; every lane evaluates the crossing filter for its own event, and the
+// candidates are resolved in time order (lowest lane first) with the reference's closed-form
+// loop on wave-uniform values. A reset lowers b for all later events of the chunk, after
+// which the filter is re-evaluated for them.
+constexpr int kWaveThreads = 256;
+
+__device__ __forceinline__ double wave_scan_incl(double v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(v, d, 64);
+    if (lane >= d) v += y;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
+    const Event *__restrict__ events, const int *__restrict__ counts, int Pcap,
+    const float *__restrict__ W, int I, int J, int BJ, int K, double c, double dtt, double tau,
+    int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
+    float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (kWaveThreads / 64) + (threadIdx.x >> 6);
+  if (wave >= BJ) return;  // uniform per wave; the kernel has no block-level barrier
+  const int b = wave / J;
+  const int j = wave - b * J;
+  const int P = counts[b];
+  const Event *ev = events + (size_t)b * Pcap;
+  const float *wrow = W + (size_t)j * I;
+  const size_t row = ((size_t)b * J + j) * K;
+  const double c4 = 4.0 * c, inv2c = 0.5 / c;
+  double carry_a = 0.0, carry_b = 0.0;
+  int n = 0;
+  bool live = true;
+  for (int q0 = 0; q0 < P && live; q0 += 64) {
+    const int q = q0 + lane;
+    const bool valid = q < P;
+    Event e{};
+    double pa = 0.0, pb = 0.0;
+    if (valid) {
+      e = ev[q];
+      const double w = (double)wrow[e.idx];
+      pa = w * e.es;
+      pb = w * e.e;
+    }
+    double A = wave_scan_incl(pa, lane) + carry_a;
+    double Bv = wave_scan_incl(pb, lane) + carry_b;
+    bool cand = valid && (always_exact || crossing_candidate(A, Bv, e, c4, inv2c));
+    unsigned long long mask = __ballot(cand);
+    while (mask) {
+      const int f = __ffsll(mask) - 1;
+      const double af = __shfl(A, f, 64);
+      const double bin = __shfl(Bv, f, 64);
+      const double tq = (double)__shfl(e.t, f, 64);
+      const double tl = (double)__shfl(e.tl, f, 64);
+      double bf = bin;
+      const int n0 = n;
+      live = solve_crossings(af, bf, n, tq, tl, c, dtt, tau, K, row, out_t, out_a, out_x, out_z,
+                             lane == 0);
+      if (!live) break;
+      if (n != n0) {
+        const double delta = bin - bf;
+        if (lane > f) Bv -= delta;
+        if (lane == f) Bv = bf;
+        cand = valid && lane > f && (always_exact || crossing_candidate(A, Bv, e, c4, inv2c));
+        mask = __ballot(cand);
+      } else {
+        mask &= ~(1ull << f);
+      }
+    }
+    const int last_lane = min(63, P - 1 - q0);
+    carry_a = __shfl(A, last_lane, 64);
+    carry_b = __shfl(Bv, last_lane, 64);
+  }
+  if (lane == 0) out_n[(size_t)b * J + j] = n;
+  for (int k = n + lane; k < K; k += 64) out_t[row + k] = kInfF;
+}
+
 // ------------------------------------------------------------------ host side
+static int g_solver_mode = 0;  // 0 auto, 1 lane per neuron, 2 wave per neuron
+
+// Lane-per-neuron keeps the event walk sequential in each lane and needs many (sample,
+// neuron) pairs to fill 256 CUs; below this many pairs the wave-per-neuron form is used.
+constexpr long long kSeqMinLanes = 32768;
 static int lds_capacity_entries() {
   static int cached[64];
   static std::once_flag once[64];
@@ -324,10 +446,11 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
     int N = 1;
     while (N < S) N <<= 1;
     const int lds_entries = std::min(N, cap);
-    hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
-                       sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s, tau, c,
-                       max_simulation, L.Pcap, events, counts, pre_exp_tau_s, pre_exp_tau,
-                       gkeys, L.gcap, lds_entries);
+    BATS_TIMED("sort_events_kernel", st,
+               hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
+                                  sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,
+                                  tau, c, max_simulation, L.Pcap, events, counts, pre_exp_tau_s,
+                                  pre_exp_tau, gkeys, L.gcap, lds_entries));
     if (int r = check_launch("sort_events_kernel")) return r;
   } else {
     // no input neurons: every sample has zero events
@@ -336,9 +459,29 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
     if (int r = check_launch("zero_counts")) return r;
   }
   const int always_exact = (c > 0.f) ? 0 : 1;
+  const long long BJ = (long long)B * J;
+  const bool wave_mode = g_solver_mode == 2 || (g_solver_mode == 0 && BJ < kSeqMinLanes);
+  if (wave_mode && S > 0) {
+    BATS_REQUIRE(BJ < 2147483647LL, "too many (sample, neuron) pairs");
+    const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));
+    BATS_TIMED("lif_solve_wave_kernel", st,
+               hipLaunchKernelGGL(lif_solve_wave_kernel, dim3(blocks), dim3(kWaveThreads), 0, st,
+                                  events, counts, L.Pcap, weights, I, J, (int)BJ, K_post,
+                                  (double)c, (double)delta_theta_tau, (double)tau, always_exact,
+                                  out_n, out_t, out_a, out_x, out_z));
+    return check_launch("lif_solve_wave_kernel");
+  }
   dim3 sg((J + kSolveThreads - 1) / kSolveThreads, B);
-  hipLaunchKernelGGL(lif_solve_seq_kernel, sg, dim3(kSolveThreads), 0, st, events, counts,
-                     L.Pcap, WT, J, K_post, (double)c, (double)delta_theta_tau, (double)tau,
-                     always_exact, out_n, out_t, out_a, out_x, out_z);
+  BATS_TIMED("lif_solve_seq_kernel", st,
+             hipLaunchKernelGGL(lif_solve_seq_kernel, sg, dim3(kSolveThreads), 0, st, events,
+                                counts, L.Pcap, WT, J, K_post, (double)c,
+                                (double)delta_theta_tau, (double)tau, always_exact, out_n, out_t,
+                                out_a, out_x, out_z));
   return check_launch("lif_solve_seq_kernel");
+}
+
+extern "C" int bats_set_solver_mode(int mode) {
+  BATS_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0 (auto), 1 (lane per neuron) or 2 (wave per neuron)");
+  g_solver_mode = mode;
+  return 0;
 }

@@ -104,10 +104,24 @@ __global__ __launch_bounds__(kThreads) void ttfs_loss_kernel(
 
 // ---------------------------------------------------------------- optimizers
 // bats/Optimizers/AdamOptimizer.py:7-19,35-55.
+// When ``step`` is not NULL the bias corrections 1 - beta^t are computed on the device from
+// the step counter it points to (graph-capturable form); otherwise bias1/bias2 are used.
 __global__ __launch_bounds__(kThreads) void adam_kernel(
     int64_t n, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
     float *__restrict__ delta, float *__restrict__ w, double lr, double b1, double omb1,
-    double b2, double omb2, double eps, double bias1, double bias2, int nadam) {
+    double b2, double omb2, double eps, double bias1, double bias2,
+    const int64_t *__restrict__ step, int nadam) {
+  if (step) {
+    __shared__ double s_bias[2];
+    if (threadIdx.x == 0) {
+      const double t = (double)*step;
+      s_bias[0] = 1.0 - pow(b1, t);
+      s_bias[1] = 1.0 - pow(b2, t);
+    }
+    __syncthreads();
+    bias1 = s_bias[0];
+    bias2 = s_bias[1];
+  }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double gi = (double)g[i];
@@ -207,15 +221,25 @@ extern "C" int bats_ttfs_loss(void *stream, int B, int O, int K, const float *t,
 
 static int adam_like(void *stream, int64_t n, const float *grad, float *m, float *v, float *delta,
                      float *weights, float lr, float beta_1, float omb1, float beta_2, float omb2,
-                     float eps, double bias1, double bias2, int nadam) {
+                     float eps, double bias1, double bias2, int nadam,
+                     const int64_t *step = nullptr) {
   BATS_REQUIRE(n >= 0, "negative size");
   if (n == 0) return 0;
   BATS_REQUIRE(grad && m && v && (delta || weights), "null pointer");
   hipLaunchKernelGGL(adam_kernel, dim3(std::min<unsigned>(grid_for(n), 4096)), dim3(kThreads), 0,
                      as_stream(stream), n, grad, m, v, delta, weights, (double)lr,
                      (double)beta_1, (double)omb1, (double)beta_2, (double)omb2, (double)eps,
-                     bias1, bias2, nadam);
+                     bias1, bias2, step, nadam);
   return check_launch("adam_kernel");
+}
+
+extern "C" int bats_adam_step_dev(void *stream, int64_t n, const float *grad, float *m, float *v,
+                                  float *delta, float *weights, float lr, float beta_1,
+                                  float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                                  float epsilon, const int64_t *step, int nadam) {
+  BATS_REQUIRE(step != nullptr, "null step counter");
+  return adam_like(stream, n, grad, m, v, delta, weights, lr, beta_1, one_minus_beta_1, beta_2,
+                   one_minus_beta_2, epsilon, 0.0, 0.0, nadam ? 1 : 0, step);
 }
 
 extern "C" int bats_adam_step(void *stream, int64_t n, const float *grad, float *m, float *v,

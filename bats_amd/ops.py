@@ -97,7 +97,7 @@ def lif_backward(n, t, a, x, z, errors, pre_times, pre_es, pre_e, weights, tau_s
     grad = torch.empty((B, J, I) if per_sample else (J, I), dtype=F32, device=dev)
     pre_err = torch.empty((B, I, Kp), dtype=F32, device=dev) if pre_trainable else None
     lib = _lib.load()
-    ws_bytes = lib.bats_lif_backward_workspace_size(B, I, J, K)
+    ws_bytes = lib.bats_lif_backward_workspace_size(B, I, Kp, J, K)
     ws = _workspace(ws_bytes, dev)
     _lib.check(lib.bats_lif_backward(
         _stream(), B, I, Kp, J, K, _ptr(n, I32), _ptr(t, F32), _ptr(a, F32), _ptr(x, F32),
@@ -151,16 +151,43 @@ def ttfs_loss(t: torch.Tensor, labels=None, tau: float = 1.0, want_loss=True,
 
 # ---------------------------------------------------------------- optimizers
 def adam_step(grad, m, v, lr, beta_1, omb1, beta_2, omb2, eps, bias1, bias2, nadam=False,
-              weights=None, want_delta=True):
+              weights=None, want_delta=True, step_dev: Optional[torch.Tensor] = None):
+    """Adam / NAdam deltas (bats/Optimizers/AdamOptimizer.py:35-55); with ``step_dev`` (an
+    int64 device scalar holding t) the bias corrections are computed on the device."""
     _require_gpu()
     delta = torch.empty_like(grad) if want_delta else None
     lib = _lib.load()
+    w = _ptr(weights, F32) if weights is not None else None
+    if step_dev is not None:
+        _lib.check(lib.bats_adam_step_dev(_stream(), grad.numel(), _ptr(grad, F32), _ptr(m, F32),
+                                          _ptr(v, F32), _ptr(delta), w, float(lr), float(beta_1),
+                                          float(omb1), float(beta_2), float(omb2), float(eps),
+                                          _ptr(step_dev, I64), 1 if nadam else 0))
+        return delta
     fn = lib.bats_nadam_step if nadam else lib.bats_adam_step
     _lib.check(fn(_stream(), grad.numel(), _ptr(grad, F32), _ptr(m, F32), _ptr(v, F32),
-                  _ptr(delta), _ptr(weights, F32) if weights is not None else None, float(lr),
-                  float(beta_1), float(omb1), float(beta_2), float(omb2), float(eps),
-                  float(bias1), float(bias2)))
+                  _ptr(delta), w, float(lr), float(beta_1), float(omb1), float(beta_2),
+                  float(omb2), float(eps), float(bias1), float(bias2)))
     return delta
+
+
+def timing(enable: Optional[bool] = None, reset: bool = False):
+    """Library-side HIP-event timing of the main kernels (see bats_timing_* in bats_amd.h)."""
+    lib = _lib.load()
+    if reset:
+        lib.bats_timing_reset()
+    if enable is not None:
+        lib.bats_timing_enable(1 if enable else 0)
+
+
+def timing_query(kernel: str):
+    """(total milliseconds, launches) recorded for ``kernel`` since the last reset."""
+    import ctypes
+    lib = _lib.load()
+    ms = ctypes.c_double(0.0)
+    cnt = ctypes.c_longlong(0)
+    lib.bats_timing_query(kernel.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+    return ms.value, cnt.value
 
 
 def sgd_step(grad, lr, weights=None, want_delta=True):

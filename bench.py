@@ -1,0 +1,318 @@
+#!/usr/bin/env python3
+"""Training-step throughput of the 784-800-10 multi-spike LIF network on one MI355X.
+
+Workload (one step = one batch through the hot path): latency-coded inputs (x ~ U[0,1],
+input active iff x > 0.1, t = 1 - x, one spike per input), 784 -> 800 LIF (theta = delta_theta
+= 1.0, tau_s = 0.130, 30-spike buffers) -> 10 LIF (theta = delta_theta = 1.3, 30-spike
+buffers), batch 256, spike-count class loss (targets 15 / 3), BATS backward, Adam, weight
+update; max_simulation = 1.0. Weights are drawn from N(0, 1) ("active" variant: the literal
+N(0, 1/sqrt(fan_in)) initialisation leaves the hidden layer silent, see --variant literal).
+
+The timed step is one replay of a HIP graph holding the whole step (forward of both layers,
+loss, backward of both layers, Adam with an on-device step counter, weight update); inputs,
+labels and weights are resident in HBM. Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "train samples/s (784-800-10, 30 spk/neuron) vs plain form; p50/p99 serve latency"
+HBM_PEAK_GBS = 8000.0
+
+# network of the workload
+N_IN, N_HID, N_OUT = 784, 800, 10
+TAU_S = 0.130
+THETA_H, THETA_O = 1.0, 1.3
+K_HID, K_OUT = 30, 30
+T_MAX = 1.0
+TARGET_TRUE, TARGET_FALSE = 15, 3
+LR = 0.003
+
+
+def make_inputs(seed, batch, variant):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.0, size=(batch, N_IN))
+    t = np.where(x > 0.1, 1.0 - x, np.inf).astype(np.float32)[:, :, None]
+    n = np.isfinite(t[:, :, 0]).astype(np.int32)
+    labels = rng.integers(0, N_OUT, size=batch).astype(np.int64)
+    if variant == "active":
+        w1 = rng.normal(0.0, 1.0, size=(N_HID, N_IN))
+        w2 = rng.normal(0.0, 1.0, size=(N_OUT, N_HID))
+    else:
+        w1 = rng.normal(0.0, 1.0 / np.sqrt(N_IN), size=(N_HID, N_IN))
+        w2 = rng.normal(0.0, 1.0 / np.sqrt(N_HID), size=(N_OUT, N_HID))
+    return t, n, labels, w1.astype(np.float32), w2.astype(np.float32)
+
+
+# ---------------------------------------------------------------- GPU path
+def build_network(w1, w2):
+    from bats_amd import Network
+    from bats_amd.Layers import InputLayer, LIFLayer
+    net = Network()
+    inp = InputLayer(n_neurons=N_IN, name="input")
+    net.add_layer(inp, input=True)
+    hid = LIFLayer(previous_layer=inp, n_neurons=N_HID, tau_s=TAU_S, theta=THETA_H,
+                   delta_theta=THETA_H, weight_initializer=lambda a, b: w1, max_n_spike=K_HID,
+                   name="hidden")
+    net.add_layer(hid)
+    out = LIFLayer(previous_layer=hid, n_neurons=N_OUT, tau_s=TAU_S, theta=THETA_O,
+                   delta_theta=THETA_O, weight_initializer=lambda a, b: w2, max_n_spike=K_OUT,
+                   name="output")
+    net.add_layer(out)
+    return net, hid, out
+
+
+def gpu_bench(args):
+    import torch
+    from bats_amd import ops
+    from bats_amd.Losses import SpikeCountClassLoss
+    from bats_amd.Optimizers import AdamOptimizer
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    t, n, labels, w1, w2 = make_inputs(args.seed, args.batch, args.variant)
+    net, hid, out = build_network(w1, w2)
+    t_d = torch.as_tensor(t, device=dev)
+    n_d = torch.as_tensor(n, device=dev)
+    lab_d = torch.as_tensor(labels, device=dev)
+    loss_fct = SpikeCountClassLoss(target_true=TARGET_TRUE, target_false=TARGET_FALSE)
+    opt = AdamOptimizer(learning_rate=LR, capturable=True)
+
+    def step():
+        net.reset()
+        net.forward(t_d, n_d, max_simulation=T_MAX, training=True)
+        out_t, out_n = net.output_spike_trains
+        loss, errors = loss_fct.compute_loss_and_errors(out_t, out_n, lab_d)
+        grads = net.backward(errors)
+        # the layers return the batch mean as [1, J, I]; dropping the unit batch axis is the
+        # reference caller's mean over axis 0
+        avg = [None if g is None else g[0] for g in grads]
+        deltas = opt.step(avg)
+        net.apply_deltas(deltas)
+        return loss
+
+    # eager warm-up (lazy library init, Adam state allocation)
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    hid_spikes = float(hid.spike_trains[1].sum().item()) / args.batch
+    out_spikes = float(out.spike_trains[1].sum().item()) / args.batch
+
+    # eager timing, with library-side HIP-event timing of the main kernels
+    ops.timing(enable=True, reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.eager_steps):
+        step()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.eager_steps
+    ops.timing(enable=False)
+    kern = {}
+    for name in ("lif_solve_seq_kernel", "lif_solve_wave_kernel", "sort_events_kernel",
+                 "wgrad_kernel", "pre_err_kernel"):
+        ms, cnt = ops.timing_query(name)
+        kern[name] = (ms / cnt if cnt else None, cnt)
+    ops.timing(reset=True)
+
+    # graph capture of the whole step
+    s = torch.cuda.Stream(device=dev)
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - wall0) * 1e3
+    graph_ms = e0.elapsed_time(e1) / args.steps
+    return dict(graph_ms=graph_ms, wall_ms=wall_ms, eager_ms=eager_ms, kernels=kern,
+                hid_spikes=hid_spikes, out_spikes=out_spikes)
+
+
+# ---------------------------------------------------------------- plain form (baseline)
+def plain_bench(args):
+    """The plain FP32 straight-translation form of the same step, eager (its structure has a
+    device-to-host read per layer, so it cannot be captured), on the same inputs/weights."""
+    import torch
+    from bats_amd import Network
+    from bats_amd.Layers import InputLayer
+    from bats_amd.plain import PlainAdam, PlainLIFLayer, PlainSpikeCountClassLoss
+
+    dev = torch.device("cuda", 0)
+    t, n, labels, w1, w2 = make_inputs(args.seed, args.batch, args.variant)
+    net = Network()
+    inp = InputLayer(n_neurons=N_IN, name="input")
+    net.add_layer(inp, input=True)
+    hid = PlainLIFLayer(inp, tau_s=TAU_S, theta=THETA_H, delta_theta=THETA_H, n_neurons=N_HID,
+                        weight_initializer=lambda a, b: w1, max_n_spike=K_HID, name="hidden")
+    net.add_layer(hid)
+    out = PlainLIFLayer(hid, tau_s=TAU_S, theta=THETA_O, delta_theta=THETA_O, n_neurons=N_OUT,
+                        weight_initializer=lambda a, b: w2, max_n_spike=K_OUT, name="output")
+    net.add_layer(out)
+    t_d = torch.as_tensor(t, device=dev)
+    n_d = torch.as_tensor(n, device=dev)
+    lab_d = torch.as_tensor(labels, device=dev)
+    loss_fct = PlainSpikeCountClassLoss(TARGET_TRUE, TARGET_FALSE)
+    opt = PlainAdam(LR)
+
+    def step():
+        net.reset()
+        net.forward(t_d, n_d, max_simulation=T_MAX, training=True)
+        out_t, out_n = net.output_spike_trains
+        loss, errors = loss_fct.compute_loss_and_errors(out_t, out_n, lab_d)
+        grads = net.backward(errors)
+        avg = [None if g is None else g.mean(dim=0) for g in grads]
+        deltas = opt.step(avg)
+        net.apply_deltas(deltas)
+
+    net.forward(t_d, n_d, max_simulation=T_MAX)
+    counts = (hid.spike_trains[1].cpu().numpy(), out.spike_trains[1].cpu().numpy())
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+    return dict(ms=ms, counts=counts)
+
+
+def first_forward_counts(args):
+    """Spike counts of the optimised path's first forward on the bench inputs."""
+    import torch
+    t, n, labels, w1, w2 = make_inputs(args.seed, args.batch, args.variant)
+    net, hid, out = build_network(w1, w2)
+    net.forward(torch.as_tensor(t, device="cuda"), torch.as_tensor(n, device="cuda"),
+                max_simulation=T_MAX)
+    return hid.spike_trains[1].cpu().numpy(), out.spike_trains[1].cpu().numpy()
+
+
+# ---------------------------------------------------------------- CPU baseline (oracle)
+def cpu_baseline(args):
+    """The FP64 NumPy oracle's training step on a bounded sample of the same workload."""
+    from threadpoolctl import threadpool_limits
+    from oracle import lif as olif
+    from oracle import losses as oloss
+    from oracle import optim as ooptim
+
+    t, n, labels, w1, w2 = make_inputs(args.seed, args.batch, args.variant)
+    S = args.cpu_sample
+    t, labels = t[:S], labels[:S]
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            h = olif.lif_forward(t, w1, TAU_S, THETA_H, THETA_H, K_HID, tmax=T_MAX)
+            o = olif.lif_forward(h.t, w2, TAU_S, THETA_O, THETA_O, K_OUT, tmax=T_MAX)
+            tg = oloss.count_targets(o.n, labels, TARGET_TRUE, TARGET_FALSE)
+            _, err = oloss.count_loss_and_errors(o.n, tg, K_OUT)
+            _, g2, pe = olif.lif_backward(o, err, pre_trainable=True)
+            _, g1, _ = olif.lif_backward(h, pe, pre_trainable=False)
+            opt = ooptim.Adam(LR)
+            opt.step([olif.batch_mean(g1), olif.batch_mean(g2)])
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_min_seconds or reps >= 3:
+                break
+    return dict(value=S * reps / el, seconds=el, reps=reps)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--eager-steps", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--variant", choices=("active", "literal"), default="active")
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-min-seconds", type=float, default=8.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--skip-plain", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the batch would shard over GPUs with one gradient all-reduce per step, "
+              "see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    r = gpu_bench(args)
+    B = args.batch
+    ms = r["graph_ms"]
+    value = B / (ms * 1e-3)
+
+    # roofline of the dominant kernel: the hidden layer's lane-per-neuron solver
+    solve_ms, solve_cnt = r["kernels"]["lif_solve_seq_kernel"]
+    alg_bytes = 4 * N_IN * N_HID + 4 * (B * N_HID * K_HID + B * N_HID)
+    roof = None
+    if solve_ms:
+        achieved = alg_bytes / (solve_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "lif_solve_seq_kernel (784->800 layer)",
+                "kernel_avg_us": round(solve_ms * 1e3, 2), "algorithmic_bytes": alg_bytes}
+
+    plain = None
+    if not args.skip_plain:
+        pr = plain_bench(args)
+        opt_counts = first_forward_counts(args)
+        plain_value = B / (pr["ms"] * 1e-3)
+        plain = {"value": round(plain_value, 1), "unit": "samples/s",
+                 "ms_per_step": round(pr["ms"], 3), "speedup": round(value / plain_value, 2),
+                 "hidden_count_agreement": float(np.mean(opt_counts[0] == pr["counts"][0])),
+                 "output_count_agreement": float(np.mean(opt_counts[1] == pr["counts"][1]))}
+
+    cpu = None
+    if not args.skip_cpu:
+        c = cpu_baseline(args)
+        cpu = {"value": round(c["value"], 3), "unit": "samples/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_sample} samples of the same batch, full training step of the "
+                         f"FP64 NumPy oracle, {c['reps']} rep(s), {c['seconds']:.1f} s, "
+                         f"BLAS limited to 1 thread"}
+
+    step_bytes = 176e6 * B / 256.0
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "784-800-10 LIF train step, K=30/30, count loss 15/3, Adam, "
+                               f"t_max=1.0, W~{'N(0,1) active' if args.variant == 'active' else 'N(0,1/sqrt(fan_in)) literal'} variant",
+                   "global_batch": B, "seed": args.seed, "graph": True},
+        "roofline": roof, "cpu_baseline": cpu, "plain_form": plain,
+        "eager_ms_per_step": round(r["eager_ms"], 4),
+        "step_hbm_fraction": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "spikes_per_sample": {"hidden": round(r["hid_spikes"], 1), "output": round(r["out_spikes"], 2)},
+        "kernel_avg_us": {k: (None if v[0] is None else round(v[0] * 1e3, 2))
+                          for k, v in r["kernels"].items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

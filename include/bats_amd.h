@@ -60,9 +60,17 @@ int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
                      float *pre_exp_tau_s, float *pre_exp_tau,
                      void *workspace, size_t workspace_bytes);
 
+/*
+ * Solver form used by bats_lif_forward: 0 = automatic (default: one lane per (sample, post
+ * neuron) when there are at least 32768 such pairs, else one wave per pair), 1 = always one
+ * lane per pair, 2 = always one wave per pair. Both forms give the same spike counts; the
+ * setting exists for testing and tuning. No counterpart in the reference.
+ */
+int bats_set_solver_mode(int mode);
+
 /* ------------------------------------------------------------------ dense LIF backward */
 
-size_t bats_lif_backward_workspace_size(int B, int I, int J, int K_post);
+size_t bats_lif_backward_workspace_size(int B, int I, int K_pre, int J, int K_post);
 
 /*
  * Dense LIF layer BATS backward.
@@ -131,6 +139,17 @@ int bats_nadam_step(void *stream, int64_t n, const float *grad, float *m, float 
                     float one_minus_beta_1, float beta_2, float one_minus_beta_2,
                     float epsilon, double bias1, double bias2);
 
+/*
+ * Adam (nadam = 0) or NAdam (nadam = 1) with the step counter t in device memory (*step,
+ * incremented by the caller before the call): the bias corrections are computed on the
+ * device, so a captured graph of the training step stays correct on every replay.
+ * Same arithmetic as bats_adam_step / bats_nadam_step otherwise.
+ */
+int bats_adam_step_dev(void *stream, int64_t n, const float *grad, float *m, float *v,
+                       float *delta, float *weights, float lr, float beta_1,
+                       float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                       float epsilon, const int64_t *step, int nadam);
+
 /* Gradient-descent deltas. Replaces bats/Optimizers/GradientDescentOptimizer.py:8-9. */
 int bats_sgd_step(void *stream, int64_t n, const float *grad, float *delta,
                   float *weights, float lr);
@@ -150,6 +169,19 @@ int bats_add_deltas(void *stream, int64_t n, float *weights, const float *delta)
 int bats_events_to_per_neuron(void *stream, int B, int P, int N, int K,
                               const int *indices, const float *times,
                               float *out_times, int *out_counts);
+
+/* ------------------------------------------------------------------ observability */
+
+/*
+ * Per-kernel HIP-event timing of the library's main launches (sort_events_kernel,
+ * lif_solve_seq_kernel, lif_solve_wave_kernel, wgrad_kernel). No counterpart in the
+ * reference (it has no profiler). Off by default; enabling it adds two event records per
+ * timed launch and must not be used while a stream is being captured.
+ */
+void bats_timing_enable(int on);
+void bats_timing_reset(void);
+/* Waits for the recorded launches and returns the total milliseconds and launch count. */
+int bats_timing_query(const char *kernel_name, double *total_ms, long long *count);
 
 #ifdef __cplusplus
 }

@@ -170,3 +170,26 @@ def test_backward_hidden_and_output_layers():
     o = olif.lif_forward(h.t, W2, 0.130, 1.3, 1.3, 30, tmax=1.0)
     check_backward(o, rng.normal(size=o.t.shape).astype(F32), "output layer")
     check_backward(h, rng.normal(size=h.t.shape).astype(F32), "hidden layer")
+
+
+@pytest.fixture
+def solver_mode():
+    from bats_amd import _lib
+    lib = _lib.load()
+    yield lambda m: _lib.check(lib.bats_set_solver_mode(m))
+    _lib.check(lib.bats_set_solver_mode(0))
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_forward_both_solver_forms(solver_mode, mode):
+    # the lane-per-neuron and wave-per-neuron solvers on the same hidden and output shapes
+    solver_mode(mode)
+    rng = np.random.default_rng(11)
+    t, _ = latency_inputs(rng, 16, 784)
+    W1 = (2.0 * rng.normal(0, 1.0, size=(800, 784))).astype(F32)
+    h, _ = check_forward(t, W1, 0.130, 1.0, 1.0, 30, 1.0, f"hidden mode {mode}")
+    W2 = rng.normal(0, 1.0, size=(10, 800)).astype(F32)
+    check_forward(h.t, W2, 0.130, 1.3, 1.3, 30, 1.0, f"output mode {mode}")
+    tb, _ = multi_spike_inputs(rng, 8, 100, 5, t_max=1.0)
+    W3 = np.abs(rng.normal(0, 3.0, size=(64, 100))).astype(F32)
+    check_forward(tb, W3, 0.05, 0.2, 0.2, 3, 0.6, f"buffer full mode {mode}")

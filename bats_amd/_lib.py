@@ -28,6 +28,15 @@ SIGNATURES = {
     "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i, _i]),
     "bats_lif_backward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp, _sz]),
+    "bats_conv_lif_forward_workspace_size": (_sz, [_i, _vp, _i, _vp]),
+    "bats_conv_lif_forward": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _f, _f, _f, _f, _f,
+                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "bats_conv_lif_backward_workspace_size": (_sz, [_i, _vp, _vp, _i]),
+    "bats_conv_lif_backward": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp,
+                                    _sz]),
+    "bats_pool2x2_forward": (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "bats_pool2x2_backward": (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
     "bats_count_loss": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _vp, _vp, _vp]),
     "bats_count_loss_targets": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "bats_ttfs_loss": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _vp, _vp, _vp]),

@@ -71,6 +71,17 @@ struct alignas(16) Event {
 };
 static_assert(sizeof(Event) == 64, "Event layout");
 
+// Per-sample time-ordered event lists built by sort_events_kernel (lif_forward.hip).
+struct EventLists {
+  Event *events;  // [B, Pcap]
+  int *counts;    // [B]
+  int Pcap;
+};
+size_t event_lists_bytes(int B, int S);
+int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,
+                      float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
+                      void *ws, EventLists *out);
+
 // Fill kernel (used instead of hipMemsetAsync so that every call stays graph-capturable).
 template <typename T>
 __global__ void fill_kernel(T *__restrict__ p, int64_t n, T v) {

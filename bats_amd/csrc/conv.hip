@@ -1,0 +1,523 @@
+// Convolutional LIF layer (valid, stride 1) and 2x2 spike pooling for gfx950.
+//
+// Forward: the per-sample event lists of the dense layer (sort_events_kernel) are shared;
+// conv_solve_kernel gives every lane one (position, output channel) of a spatial tile of the
+// output. The block compacts the sample's events that fall inside the tile's joint receptive
+// field into LDS (order preserved by a block scan), and each lane walks them, using only the
+// events inside its own receptive field. The crossing check of an interval is made when the
+// lane reaches its next receptive-field event (or at the end, against t_max): an event outside
+// the field has a zero weight and leaves a and b unchanged, so this is the same decision
+// sequence as the reference's kernel, which checks after every event of the sample
+// (compute_spike_times_conv.cu:155-173).
+//
+// Backward: post_prep_kernel of the dense layer prepares the per-spike suffix sums; then
+// conv_wgrad_kernel (one block per filter tap and sample, lanes over output positions, block
+// reduction) and conv_pre_err_kernel (lanes over input neurons, loop over the output neurons
+// whose field contains them) -- the gradient regrouping of lif_backward.hip applied to the
+// receptive-field structure of compute_weights_gradient_conv.cu and
+// propagate_errors_to_pre_spikes_conv.cu.
+//
+// Pooling: pool_forward_kernel merges the four children's ascending lists per output neuron
+// (aggregate_spikes_conv.cu + the argsort of its wrapper), recording the source slot of every
+// merged slot; pool_backward_kernel scatters errors back (bats/Layers/PoolingLayer.py:55-66).
+#include "solver.hpp"
+
+#include <algorithm>
+
+namespace bats {
+
+struct ConvGeom {
+  int X, Y, Ci;     // input
+  int Xo, Yo, Co;   // output
+  int fx, fy;       // filter
+};
+
+// [Co, fx, fy, Ci] -> [fx, fy, Ci, Co]: consecutive output channels (consecutive lanes)
+// read consecutive weights.
+__global__ void conv_filter_transpose_kernel(const float *__restrict__ W, float *__restrict__ WT,
+                                             int Co, int taps) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Co * taps) return;
+  const int c = e / taps, r = e - c * taps;
+  WT[(size_t)r * Co + c] = W[e];
+}
+
+struct ConvEv {
+  Event ev;
+  int px, py, pc, pad;
+};
+
+constexpr int kConvMaxThreads = 1024;
+
+__global__ __launch_bounds__(kConvMaxThreads) void conv_solve_kernel(
+    const Event *__restrict__ events, const int *__restrict__ counts, int Pcap, ConvGeom g,
+    int TX, int TY, const float *__restrict__ WT, int K, double c, double dtt, double tau,
+    float tmax, int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
+    float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
+  extern __shared__ __align__(16) char smem[];
+  ConvEv *sev = reinterpret_cast<ConvEv *>(smem);
+  int *s_scan = reinterpret_cast<int *>(sev + blockDim.x);
+  const int tilesY = (g.Yo + TY - 1) / TY;
+  const int x0 = (blockIdx.x / tilesY) * TX;
+  const int y0 = (blockIdx.x % tilesY) * TY;
+  const int b = blockIdx.y;
+  const int l = threadIdx.x;
+  const int cc = l % g.Co;
+  const int pos = l / g.Co;
+  const int x = x0 + pos / TY;
+  const int y = y0 + pos % TY;
+  const bool active = pos < TX * TY && x < g.Xo && y < g.Yo;
+  const int j = active ? (x * g.Yo + y) * g.Co + cc : 0;
+  const size_t row = ((size_t)b * g.Xo * g.Yo * g.Co + j) * K;
+  // joint receptive field of the tile
+  const int rx0 = x0, rx1 = min(g.X, x0 + TX + g.fx - 1);
+  const int ry0 = y0, ry1 = min(g.Y, y0 + TY + g.fy - 1);
+  const int P = counts[b];
+  const Event *ev = events + (size_t)b * Pcap;
+  const double c4 = 4.0 * c, inv2c = 0.5 / c;
+  const double zmax = exp((double)tmax / tau);
+  double a = 0.0, bb = 0.0, zq_last = 0.0;
+  float t_last = -kInfF;
+  bool seen = false, live = active;
+  int n = 0;
+  for (int base = 0; base < P; base += blockDim.x) {
+    const int q = base + l;
+    int flag = 0;
+    Event e{};
+    int px = 0, py = 0, pc = 0;
+    if (q < P) {
+      e = ev[q];
+      px = e.idx / (g.Y * g.Ci);
+      const int r = e.idx - px * (g.Y * g.Ci);
+      py = r / g.Ci;
+      pc = r - py * g.Ci;
+      flag = (px >= rx0 && px < rx1 && py >= ry0 && py < ry1) ? 1 : 0;
+    }
+    // block-wide exclusive scan of the flags (Hillis-Steele in LDS)
+    __syncthreads();  // the previous chunk has been consumed
+    s_scan[l] = flag;
+    __syncthreads();
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+      const int v = (l >= d) ? s_scan[l - d] : 0;
+      __syncthreads();
+      s_scan[l] += v;
+      __syncthreads();
+    }
+    const int total = s_scan[blockDim.x - 1];
+    if (flag) {
+      ConvEv ce;
+      ce.ev = e;
+      ce.px = px;
+      ce.py = py;
+      ce.pc = pc;
+      ce.pad = 0;
+      sev[s_scan[l] - 1] = ce;
+    }
+    __syncthreads();
+    for (int u = 0; u < total; ++u) {
+      const int dx = sev[u].px - x, dy = sev[u].py - y;
+      if (live && dx >= 0 && dx < g.fx && dy >= 0 && dy < g.fy) {
+        const float tu = sev[u].ev.t;
+        if (seen) {
+          // interval (t_last, min(tu, t_max)] with the state after the previous field event
+          const float tl = fminf(tu, tmax);
+          const double zl = (tu <= tmax) ? sev[u].ev.zq : zmax;
+          if (always_exact || crossing_candidate3(a, bb, zq_last, zl, c * zl * zl, c4, inv2c)) {
+            const SolveState r = solve_crossings_ool(a, bb, n, (double)t_last, (double)tl, c,
+                                                     dtt, tau, K, row, out_t, out_a, out_x,
+                                                     out_z);
+            bb = r.b;
+            n = r.n;
+            live = r.live != 0;
+          }
+        }
+        if (live) {
+          const double w = (double)WT[((size_t)(dx * g.fy + dy) * g.Ci + sev[u].pc) * g.Co + cc];
+          a = fma(w, sev[u].ev.es, a);
+          bb = fma(w, sev[u].ev.e, bb);
+          t_last = tu;
+          zq_last = sev[u].ev.zq;
+          seen = true;
+        }
+      }
+    }
+  }
+  if (live && seen) {
+    const double zl = zmax;
+    if (always_exact || crossing_candidate3(a, bb, zq_last, zl, c * zl * zl, c4, inv2c)) {
+      const SolveState r = solve_crossings_ool(a, bb, n, (double)t_last, (double)tmax, c, dtt,
+                                               tau, K, row, out_t, out_a, out_x, out_z);
+      n = r.n;
+    }
+  }
+  if (active) {
+    out_n[(size_t)b * g.Xo * g.Yo * g.Co + j] = n;
+    for (int k = n; k < K; ++k) out_t[row + k] = kInfF;
+  }
+}
+
+// ---------------------------------------------------------------- pooling
+__global__ void pool_forward_kernel(int B, int X, int Y, int C, int K,
+                                    const int *__restrict__ n, const float *__restrict__ t,
+                                    int *__restrict__ out_n, float *__restrict__ out_t,
+                                    int *__restrict__ out_src) {
+  const int Xo = X / 2, Yo = Y / 2;
+  const int No = Xo * Yo * C;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= B * No) return;
+  const int b = gid / No, jo = gid - b * No;
+  const int x = jo / (Yo * C), r = jo - x * (Yo * C), y = r / C, c = r - y * C;
+  const int N = X * Y * C;
+  size_t base[4];
+  int cnt = 0;
+  for (int d = 0; d < 4; ++d) {
+    const int i = ((2 * x + (d >> 1)) * Y + (2 * y + (d & 1))) * C + c;
+    base[d] = ((size_t)b * N + i) * K;
+    cnt += n[(size_t)b * N + i];
+  }
+  out_n[gid] = cnt;
+  int p[4] = {0, 0, 0, 0};
+  const size_t obase = (size_t)gid * 4 * K;
+  for (int s = 0; s < 4 * K; ++s) {
+    int best = -1;
+    float bv = 0.f;
+    for (int d = 0; d < 4; ++d) {
+      if (p[d] >= K) continue;
+      const float v = t[base[d] + p[d]];
+      // ascending with +inf last; ties keep the children's order
+      if (best < 0 || v < bv) {
+        best = d;
+        bv = v;
+      }
+    }
+    out_t[obase + s] = bv;
+    out_src[obase + s] = (int)(base[best] + p[best]);
+    ++p[best];
+  }
+}
+
+__global__ void pool_backward_kernel(int64_t total, const float *__restrict__ err,
+                                     const int *__restrict__ src, float *__restrict__ pre_err) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) pre_err[src[e]] = err[e];
+}
+
+// ---------------------------------------------------------------- backward
+__device__ __forceinline__ void lookup_row(const float *__restrict__ tr,
+                                           const double *__restrict__ s1,
+                                           const double *__restrict__ s2, int nj, float tp,
+                                           double &v1, double &v2) {
+  v1 = 0.0;
+  v2 = 0.0;
+  for (int k = nj - 1; k >= 0; --k) {
+    if (tr[k] > tp) {
+      v1 = s1[k];
+      v2 = s2[k];
+    }
+  }
+}
+
+// One block per (filter tap, sample); lanes over output positions.
+__global__ void conv_wgrad_kernel(ConvGeom g, int K, int K_pre, const float *__restrict__ pre_t,
+                                  const float *__restrict__ pre_es,
+                                  const float *__restrict__ pre_e,
+                                  const float *__restrict__ post_t, const int *__restrict__ n_eff,
+                                  const double *__restrict__ S1, const double *__restrict__ S2,
+                                  double *__restrict__ per_sample) {
+  __shared__ double s_red[16];
+  const int taps = g.fx * g.fy * g.Ci;
+  const int tap = blockIdx.x;  // over Co * taps, layout [Co, fx, fy, Ci]
+  const int b = blockIdx.y;
+  const int cc = tap / taps;
+  const int r = tap - cc * taps;
+  const int dx = r / (g.fy * g.Ci);
+  const int r2 = r - dx * (g.fy * g.Ci);
+  const int dy = r2 / g.Ci;
+  const int ci = r2 - dy * g.Ci;
+  const int Npost = g.Xo * g.Yo * g.Co;
+  const int Npre = g.X * g.Y * g.Ci;
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < g.Xo * g.Yo; p += blockDim.x) {
+    const int x = p / g.Yo, y = p - (p / g.Yo) * g.Yo;
+    const int j = (x * g.Yo + y) * g.Co + cc;
+    const size_t prow = (size_t)b * Npost + j;
+    const int nj = n_eff[prow];
+    if (nj == 0) continue;
+    const int i = ((x + dx) * g.Y + (y + dy)) * g.Ci + ci;
+    const size_t pbase = ((size_t)b * Npre + i) * K_pre;
+    for (int m = 0; m < K_pre; ++m) {
+      const float tp = pre_t[pbase + m];
+      if (!isfinite(tp)) break;
+      double v1, v2;
+      lookup_row(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
+      acc = fma((double)pre_es[pbase + m], v1, acc);
+      acc = fma(-(double)pre_e[pbase + m], v2, acc);
+    }
+  }
+  // block reduction in a fixed order
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) s_red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) s += s_red[w];
+    per_sample[(size_t)b * gridDim.x + tap] = s;
+  }
+}
+
+__global__ void conv_wgrad_finish_kernel(int B, int taps_all, int mode,
+                                         const double *__restrict__ per_sample,
+                                         float *__restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 1) {
+    if (e < B * taps_all) grad[e] = (float)per_sample[e];
+    return;
+  }
+  if (e >= taps_all) return;
+  double s = 0.0;
+  for (int b = 0; b < B; ++b) s += per_sample[(size_t)b * taps_all + e];
+  grad[e] = (float)(mode == 2 ? s : s / (double)B);
+}
+
+// Lanes over the input neurons of one sample; every output neuron whose field holds the
+// lane's neuron contributes W * (V1 es/tau_s - V2 e/tau) to each of its spikes.
+__global__ void conv_pre_err_kernel(ConvGeom g, int K, int K_pre, const float *__restrict__ pre_t,
+                                    const float *__restrict__ pre_es,
+                                    const float *__restrict__ pre_e,
+                                    const float *__restrict__ W, const float *__restrict__ post_t,
+                                    const int *__restrict__ n_eff, const double *__restrict__ S1,
+                                    const double *__restrict__ S2, double tau_s, double tau,
+                                    float *__restrict__ pre_err) {
+  const int Npre = g.X * g.Y * g.Ci;
+  const int Npost = g.Xo * g.Yo * g.Co;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= Npre) return;
+  const int px = i / (g.Y * g.Ci), r = i - px * (g.Y * g.Ci), py = r / g.Ci, pc = r - py * g.Ci;
+  const size_t pbase = ((size_t)b * Npre + i) * K_pre;
+  for (int m = 0; m < K_pre; ++m) {
+    const float tp = pre_t[pbase + m];
+    if (!isfinite(tp)) {
+      pre_err[pbase + m] = 0.f;
+      continue;
+    }
+    const double es = (double)pre_es[pbase + m], e = (double)pre_e[pbase + m];
+    double acc = 0.0;
+    for (int dx = 0; dx < g.fx; ++dx) {
+      const int x = px - dx;
+      if (x < 0 || x >= g.Xo) continue;
+      for (int dy = 0; dy < g.fy; ++dy) {
+        const int y = py - dy;
+        if (y < 0 || y >= g.Yo) continue;
+        for (int cc = 0; cc < g.Co; ++cc) {
+          const int j = (x * g.Yo + y) * g.Co + cc;
+          const size_t prow = (size_t)b * Npost + j;
+          const int nj = n_eff[prow];
+          if (nj == 0) continue;
+          double v1, v2;
+          lookup_row(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
+          const double w = (double)W[(((size_t)cc * g.fx + dx) * g.fy + dy) * g.Ci + pc];
+          acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
+        }
+      }
+    }
+    pre_err[pbase + m] = (float)acc;
+  }
+}
+
+// defined in lif_backward.hip
+int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const float *a,
+              const float *x, const float *z, float *errors, float c, float tau, float dtt,
+              int *n_eff, double *S1, double *S2);
+
+static void conv_tile(const ConvGeom &g, int &TX, int &TY) {
+  // as many positions as fit 512 lanes with all output channels, square-ish
+  int per = std::max(1, 512 / g.Co);
+  TX = 1;
+  TY = 1;
+  while (TX * TY * 2 <= per && (TX < g.Xo || TY < g.Yo)) {
+    if (TY <= TX && TY < g.Yo)
+      TY *= 2;
+    else if (TX < g.Xo)
+      TX *= 2;
+    else
+      TY *= 2;
+  }
+}
+
+}  // namespace bats
+
+using namespace bats;
+
+static bool conv_geom(const int *pre_shape, const int *filters_shape, ConvGeom &g) {
+  g.X = pre_shape[0];
+  g.Y = pre_shape[1];
+  g.Ci = pre_shape[2];
+  g.fx = filters_shape[0];
+  g.fy = filters_shape[1];
+  g.Co = filters_shape[2];
+  g.Xo = g.X - g.fx + 1;
+  g.Yo = g.Y - g.fy + 1;
+  return g.X > 0 && g.Y > 0 && g.Ci > 0 && g.fx > 0 && g.fy > 0 && g.Co > 0 && g.Xo > 0 &&
+         g.Yo > 0 && g.Co <= 1024;
+}
+
+extern "C" size_t bats_conv_lif_forward_workspace_size(int B, const int *pre_shape, int K_pre,
+                                                       const int *filters_shape) {
+  ConvGeom g;
+  if (B < 0 || !pre_shape || !filters_shape || !conv_geom(pre_shape, filters_shape, g)) return 0;
+  const int S = g.X * g.Y * g.Ci * K_pre;
+  return align_up(sizeof(float) * (size_t)g.Co * g.fx * g.fy * g.Ci, 256) +
+         event_lists_bytes(B, S);
+}
+
+extern "C" int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, int K_pre,
+                                     const int *filters_shape, int K_post,
+                                     const float *pre_times, const float *weights, float tau_s,
+                                     float tau, float c, float delta_theta_tau,
+                                     float max_simulation, int *out_n, float *out_t,
+                                     float *out_a, float *out_x, float *out_z,
+                                     float *pre_exp_tau_s, float *pre_exp_tau, void *workspace,
+                                     size_t workspace_bytes) {
+  ConvGeom g;
+  BATS_REQUIRE(pre_shape && filters_shape && conv_geom(pre_shape, filters_shape, g),
+               "bad conv shapes (pre_shape = {X, Y, C_in}, filters_shape = {fx, fy, C_out})");
+  BATS_REQUIRE(B >= 0 && K_pre >= 0 && K_post >= 1, "bad sizes");
+  BATS_REQUIRE(B <= 65535, "batch above 65535");
+  if (B == 0) return 0;
+  BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
+  const size_t need = bats_conv_lif_forward_workspace_size(B, pre_shape, K_pre, filters_shape);
+  BATS_REQUIRE(workspace && workspace_bytes >= need, "workspace too small");
+  hipStream_t st = as_stream(stream);
+  const int taps = g.fx * g.fy * g.Ci;
+  char *ws = static_cast<char *>(workspace);
+  float *WT = reinterpret_cast<float *>(ws);
+  void *ev_ws = ws + align_up(sizeof(float) * (size_t)g.Co * taps, 256);
+  const int S = g.X * g.Y * g.Ci * K_pre;
+  if (S > 0) BATS_REQUIRE(pre_times && weights && pre_exp_tau_s && pre_exp_tau, "null input");
+  hipLaunchKernelGGL(conv_filter_transpose_kernel, dim3((g.Co * taps + 255) / 256), dim3(256), 0,
+                     st, weights, WT, g.Co, taps);
+  if (int r = check_launch("conv_filter_transpose_kernel")) return r;
+  EventLists el;
+  if (int r = build_event_lists(st, B, S, K_pre, pre_times, tau_s, tau, c, max_simulation,
+                                pre_exp_tau_s, pre_exp_tau, ev_ws, &el))
+    return r;
+  int TX, TY;
+  conv_tile(g, TX, TY);
+  const int lanes = TX * TY * g.Co;
+  const int threads = std::min(kConvMaxThreads, (lanes + 63) / 64 * 64);
+  const int tiles = ((g.Xo + TX - 1) / TX) * ((g.Yo + TY - 1) / TY);
+  const size_t lds = sizeof(ConvEv) * threads + sizeof(int) * threads;
+  BATS_TIMED("conv_solve_kernel", st,
+             hipLaunchKernelGGL(conv_solve_kernel, dim3(tiles, B), dim3(threads), lds, st,
+                                el.events, el.counts, el.Pcap, g, TX, TY, WT, K_post, (double)c,
+                                (double)delta_theta_tau, (double)tau, max_simulation,
+                                (c > 0.f) ? 0 : 1, out_n, out_t, out_a, out_x, out_z));
+  return check_launch("conv_solve_kernel");
+}
+
+extern "C" size_t bats_conv_lif_backward_workspace_size(int B, const int *pre_shape,
+                                                        const int *filters_shape, int K_post) {
+  ConvGeom g;
+  if (B < 0 || !pre_shape || !filters_shape || !conv_geom(pre_shape, filters_shape, g)) return 0;
+  const size_t BJ = (size_t)B * g.Xo * g.Yo * g.Co;
+  size_t off = 0;
+  off = align_up(off + sizeof(int) * BJ, 256);
+  off = align_up(off + sizeof(double) * BJ * K_post, 256);
+  off = align_up(off + sizeof(double) * BJ * K_post, 256);
+  off = align_up(off + sizeof(double) * (size_t)B * g.Co * g.fx * g.fy * g.Ci, 256);
+  return off;
+}
+
+extern "C" int bats_conv_lif_backward(void *stream, int B, const int *pre_shape, int K_pre,
+                                      const int *filters_shape, int K_post, const int *n,
+                                      const float *t, const float *a, const float *x,
+                                      const float *z, float *errors, const float *pre_times,
+                                      const float *pre_exp_tau_s, const float *pre_exp_tau,
+                                      const float *weights, float tau_s, float tau, float c,
+                                      float delta_theta_tau, float *grad, int grad_mode,
+                                      float *pre_errors, void *workspace,
+                                      size_t workspace_bytes) {
+  ConvGeom g;
+  BATS_REQUIRE(pre_shape && filters_shape && conv_geom(pre_shape, filters_shape, g),
+               "bad conv shapes");
+  BATS_REQUIRE(B >= 0 && K_pre >= 0 && K_post >= 1, "bad sizes");
+  BATS_REQUIRE(grad_mode >= 0 && grad_mode <= 2, "grad_mode must be 0, 1 or 2");
+  BATS_REQUIRE(B <= 65535, "batch above 65535");
+  if (B == 0) return 0;
+  const size_t need = bats_conv_lif_backward_workspace_size(B, pre_shape, filters_shape, K_post);
+  BATS_REQUIRE(workspace && workspace_bytes >= need, "workspace too small");
+  BATS_REQUIRE(n && t && a && x && z && errors && grad, "null pointer");
+  hipStream_t st = as_stream(stream);
+  const size_t BJ = (size_t)B * g.Xo * g.Yo * g.Co;
+  char *ws = static_cast<char *>(workspace);
+  size_t off = 0;
+  int *n_eff = reinterpret_cast<int *>(ws + off);
+  off = align_up(off + sizeof(int) * BJ, 256);
+  double *S1 = reinterpret_cast<double *>(ws + off);
+  off = align_up(off + sizeof(double) * BJ * K_post, 256);
+  double *S2 = reinterpret_cast<double *>(ws + off);
+  off = align_up(off + sizeof(double) * BJ * K_post, 256);
+  double *per_sample = reinterpret_cast<double *>(ws + off);
+  if (int r = post_prep(st, (int)BJ, K_post, n, t, a, x, z, errors, c, tau, delta_theta_tau,
+                        n_eff, S1, S2))
+    return r;
+  const int taps_all = g.Co * g.fx * g.fy * g.Ci;
+  if (K_pre > 0) BATS_REQUIRE(pre_times && pre_exp_tau_s && pre_exp_tau, "null pre pointer");
+  const int threads = std::min(1024, std::max(64, (g.Xo * g.Yo + 63) / 64 * 64));
+  BATS_TIMED("conv_wgrad_kernel", st,
+             hipLaunchKernelGGL(conv_wgrad_kernel, dim3(taps_all, B), dim3(threads), 0, st, g,
+                                K_post, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff,
+                                S1, S2, per_sample));
+  if (int r = check_launch("conv_wgrad_kernel")) return r;
+  const int fin = grad_mode == 1 ? B * taps_all : taps_all;
+  hipLaunchKernelGGL(conv_wgrad_finish_kernel, dim3((fin + 255) / 256), dim3(256), 0, st, B,
+                     taps_all, grad_mode, per_sample, grad);
+  if (int r = check_launch("conv_wgrad_finish_kernel")) return r;
+  if (pre_errors && K_pre > 0) {
+    BATS_REQUIRE(weights, "null weights");
+    const int Npre = g.X * g.Y * g.Ci;
+    BATS_TIMED("conv_pre_err_kernel", st,
+               hipLaunchKernelGGL(conv_pre_err_kernel, dim3((Npre + 255) / 256, B), dim3(256), 0,
+                                  st, g, K_post, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau,
+                                  weights, t, n_eff, S1, S2, (double)tau_s, (double)tau,
+                                  pre_errors));
+    if (int r = check_launch("conv_pre_err_kernel")) return r;
+  }
+  return 0;
+}
+
+extern "C" int bats_pool2x2_forward(void *stream, int B, const int *pre_shape, int K,
+                                    const int *n, const float *t, int *out_n, float *out_t,
+                                    int *out_src) {
+  BATS_REQUIRE(pre_shape && B >= 0 && K >= 0, "bad sizes");
+  const int X = pre_shape[0], Y = pre_shape[1], C = pre_shape[2];
+  BATS_REQUIRE(X >= 2 && Y >= 2 && C >= 1, "pooling needs at least 2x2 positions");
+  const long long total = (long long)B * (X / 2) * (Y / 2) * C;
+  BATS_REQUIRE((double)B * X * Y * C * K < 2147483647.0, "pooling input above 2^31 slots");
+  if (total == 0 || K == 0) return 0;
+  BATS_REQUIRE(n && t && out_n && out_t && out_src, "null pointer");
+  hipLaunchKernelGGL(pool_forward_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), B, X, Y, C, K, n, t, out_n, out_t, out_src);
+  return check_launch("pool_forward_kernel");
+}
+
+extern "C" int bats_pool2x2_backward(void *stream, int B, const int *pre_shape, int K,
+                                     const float *errors, const int *src, float *pre_errors) {
+  BATS_REQUIRE(pre_shape && B >= 0 && K >= 0, "bad sizes");
+  const int X = pre_shape[0], Y = pre_shape[1], C = pre_shape[2];
+  const int64_t npre = (int64_t)B * X * Y * C * K;
+  const int64_t npost = (int64_t)B * (X / 2) * (Y / 2) * C * 4 * K;
+  if (npre == 0) return 0;
+  BATS_REQUIRE(errors && src && pre_errors, "null pointer");
+  hipStream_t st = as_stream(stream);
+  // slots outside every pooling window (odd input sizes) get zero
+  hipLaunchKernelGGL(fill_kernel<float>, dim3((unsigned)std::min<int64_t>((npre + 255) / 256, 65535)),
+                     dim3(256), 0, st, pre_errors, npre, 0.f);
+  if (int r = check_launch("fill pre errors")) return r;
+  if (npost == 0) return 0;
+  hipLaunchKernelGGL(pool_backward_kernel, dim3((unsigned)((npost + 255) / 256)), dim3(256), 0, st,
+                     npost, errors, src, pre_errors);
+  return check_launch("pool_backward_kernel");
+}

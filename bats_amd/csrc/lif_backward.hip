@@ -288,6 +288,16 @@ static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K) {
   return L;
 }
 
+int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const float *a,
+              const float *x, const float *z, float *errors, float c, float tau, float dtt,
+              int *n_eff, double *S1, double *S2) {
+  if (BJ <= 0) return 0;
+  hipLaunchKernelGGL(post_prep_kernel, dim3((BJ + kBwdThreads - 1) / kBwdThreads),
+                     dim3(kBwdThreads), 0, st, BJ, K, n, t, a, x, z, errors, (double)c,
+                     (double)tau, (double)dtt, n_eff, S1, S2);
+  return check_launch("post_prep_kernel");
+}
+
 }  // namespace bats
 
 using namespace bats;
@@ -309,11 +319,12 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                                  float *pre_errors, void *workspace, size_t workspace_bytes) {
   BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0 && K_post >= 1, "bad sizes");
   BATS_REQUIRE(K_post <= kMaxBackwardK, "max_n_spike above 384 is not supported by the backward");
-  BATS_REQUIRE(grad_mode == 0 || grad_mode == 1, "grad_mode must be 0 (mean) or 1 (per sample)");
+  BATS_REQUIRE(grad_mode >= 0 && grad_mode <= 2,
+               "grad_mode must be 0 (batch mean), 1 (per sample) or 2 (batch sum)");
   hipStream_t st = as_stream(stream);
   const int64_t JI = (int64_t)J * I;
   if (B == 0) {
-    if (grad_mode == 0 && JI > 0) {
+    if (grad_mode != 1 && JI > 0) {
       BATS_REQUIRE(grad, "null grad");
       hipLaunchKernelGGL(fill_kernel<float>, dim3((unsigned)std::min<int64_t>((JI + 255) / 256, 65535)),
                          dim3(256), 0, st, grad, JI, 0.f);
@@ -358,7 +369,8 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                                     t, n_eff, S1, S2, 0, grad, partial));
       if (int r = check_launch("wgrad_kernel")) return r;
       hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
-                         dim3(kBwdThreads), 0, st, L.slices, JI, 1.0 / (double)B, partial, grad);
+                         dim3(kBwdThreads), 0, st, L.slices, JI,
+                         grad_mode == 2 ? 1.0 : 1.0 / (double)B, partial, grad);
       if (int r = check_launch("wgrad_reduce_kernel")) return r;
     }
   }

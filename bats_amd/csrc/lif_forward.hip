@@ -22,7 +22,7 @@
 // Reference: bats/Layers/LIFLayer.py:63-91, bats/CudaKernels/CuFiles/Inference/
 // compute_pre_exps.cu, get_spike_weights.cu, compute_spike_times.cu:34-138,
 // bats/CudaKernels/Wrappers/Inference/get_sorted_spikes_indices.py:7-16.
-#include "common.hpp"
+#include "solver.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -133,74 +133,6 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     pre_exp[(size_t)b * S + s] = e32;
   }
   if (threadIdx.x == 0) counts[b] = P;
-}
-
-// ------------------------------------------------------------------ crossing filter
-// Necessary condition (with a relative margin) for the reference's smaller root
-// z = 2a/(b + sqrt(b^2 - 4ac)) to be accepted in (t_q, min(t_next, t_max)]:
-// a > 0 and b > 0, and either g(zl) = c zl^2 - b zl + a <= 0 (one root inside) or the
-// vertex b/(2c) of g lies in [zq, zl] with a non-negative discriminant (two roots inside).
-constexpr double kFilterEps = 1e-10;
-
-__device__ __forceinline__ bool crossing_candidate(double a, double b, const Event &e,
-                                                   double c4, double inv2c) {
-  if (!(a > 0.0) || !(b > 0.0)) return false;
-  const double bb_hi = b * b * (1.0 + kFilterEps);
-  if (isinf(e.zl)) return bb_hi >= c4 * a;
-  const double g = (a + e.czl2) - b * e.zl * (1.0 + kFilterEps);
-  if (g <= 0.0) return true;
-  const double v = b * inv2c;
-  if (v < e.zq * (1.0 - kFilterEps) || v > e.zl * (1.0 + kFilterEps)) return false;
-  return bb_hi >= c4 * a;
-}
-
-// The reference's compute_spikes loop (compute_spike_times.cu:53-83) in float64.
-// Returns false when the neuron's buffer became full (its event loop ends).
-__device__ __forceinline__ bool solve_crossings(double a, double &b, int &n, double last,
-                                                double tl, double c, double dtt, double tau,
-                                                int K, size_t row, float *__restrict__ out_t,
-                                                float *__restrict__ out_a,
-                                                float *__restrict__ out_x,
-                                                float *__restrict__ out_z, bool writer = true) {
-  while (true) {
-    const double disc = b * b - (4.0 * a) * c;
-    if (disc < 0.0) return true;
-    const double x = sqrt(disc);
-    const double den = x + b;
-    if (den == 0.0) return true;
-    const double z = (2.0 * a) / den;
-    if (z < 0.0) return true;
-    const double t = tau * log(z);
-    if (t <= last || t > tl) return true;
-    if (writer) {
-      out_a[row + n] = (float)a;
-      out_x[row + n] = (float)x;
-      out_t[row + n] = (float)t;
-      out_z[row + n] = (float)z;
-    }
-    last = t;
-    b = b - dtt * z;
-    ++n;
-    if (n >= K) return false;
-  }
-}
-
-// Out-of-line copy of the closed-form loop for the lane-per-neuron solver, whose event loop
-// is fully unrolled per chunk (the loop is entered rarely: only when the filter passes).
-struct SolveState {
-  double b;
-  int n;
-  int live;
-};
-
-__device__ __attribute__((noinline)) SolveState solve_crossings_ool(
-    double a, double b, int n, double last, double tl, double c, double dtt, double tau, int K,
-    size_t row, float *out_t, float *out_a, float *out_x, float *out_z) {
-  SolveState r;
-  r.live = solve_crossings(a, b, n, last, tl, c, dtt, tau, K, row, out_t, out_a, out_x, out_z);
-  r.b = b;
-  r.n = n;
-  return r;
 }
 
 // ------------------------------------------------------------------ solver, lane per neuron
@@ -403,6 +335,48 @@ static FwdLayout fwd_layout(int B, int I, int K_pre, int J, int lds_cap) {
   off = align_up(off + sizeof(uint64_t) * (size_t)B * L.gcap, 256);
   L.total = off;
   return L;
+}
+
+size_t event_lists_bytes(int B, int S) {
+  const int cap = lds_capacity_entries();
+  int N = 1;
+  while (N < S) N <<= 1;
+  const int gcap = (N > cap) ? N : 0;
+  size_t off = 0;
+  off = align_up(off + sizeof(Event) * (size_t)B * std::max(S, 1), 256);
+  off = align_up(off + sizeof(int) * (size_t)B, 256);
+  off = align_up(off + sizeof(uint64_t) * (size_t)B * gcap, 256);
+  return off;
+}
+
+int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,
+                      float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
+                      void *ws, EventLists *out) {
+  const int cap = lds_capacity_entries();
+  int N = 1;
+  while (N < S) N <<= 1;
+  const int gcap = (N > cap) ? N : 0;
+  const int Pcap = std::max(S, 1);
+  char *base = static_cast<char *>(ws);
+  size_t off = 0;
+  out->events = reinterpret_cast<Event *>(base + off);
+  off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  out->counts = reinterpret_cast<int *>(base + off);
+  off = align_up(off + sizeof(int) * (size_t)B, 256);
+  uint64_t *gkeys = reinterpret_cast<uint64_t *>(base + off);
+  out->Pcap = Pcap;
+  if (S > 0) {
+    const int lds_entries = std::min(N, cap);
+    BATS_TIMED("sort_events_kernel", st,
+               hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
+                                  sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,
+                                  tau, c, tmax, Pcap, out->events, out->counts, pre_es, pre_e,
+                                  gkeys, gcap, lds_entries));
+    return check_launch("sort_events_kernel");
+  }
+  hipLaunchKernelGGL(fill_kernel<int>, dim3((B + 255) / 256), dim3(256), 0, st, out->counts,
+                     (int64_t)B, 0);
+  return check_launch("zero_counts");
 }
 
 }  // namespace bats

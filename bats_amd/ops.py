@@ -86,15 +86,17 @@ def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float3
 
 
 def lif_backward(n, t, a, x, z, errors, pre_times, pre_es, pre_e, weights, tau_s, tau, c,
-                 delta_theta_tau, per_sample: bool = False, pre_trainable: bool = True):
+                 delta_theta_tau, per_sample: bool = False, pre_trainable: bool = True,
+                 batch_sum: bool = False):
     """Dense LIF BATS backward (bats/Layers/LIFLayer.py:93-110). ``errors`` is updated in
     place with the reset-chain errors. Returns (grad, pre_errors | None); grad is the batch
-    mean [J,I] unless ``per_sample`` ([B,J,I])."""
+    mean [J,I] (the batch sum with ``batch_sum``) unless ``per_sample`` ([B,J,I])."""
     _require_gpu()
     B, J, K = t.shape
     _, I, Kp = pre_times.shape
     dev = t.device
     grad = torch.empty((B, J, I) if per_sample else (J, I), dtype=F32, device=dev)
+    mode = 1 if per_sample else (2 if batch_sum else 0)
     pre_err = torch.empty((B, I, Kp), dtype=F32, device=dev) if pre_trainable else None
     lib = _lib.load()
     ws_bytes = lib.bats_lif_backward_workspace_size(B, I, Kp, J, K)
@@ -103,7 +105,7 @@ def lif_backward(n, t, a, x, z, errors, pre_times, pre_es, pre_e, weights, tau_s
         _stream(), B, I, Kp, J, K, _ptr(n, I32), _ptr(t, F32), _ptr(a, F32), _ptr(x, F32),
         _ptr(z, F32), _ptr(errors, F32), _ptr(pre_times, F32), _ptr(pre_es, F32),
         _ptr(pre_e, F32), _ptr(weights, F32), float(tau_s), float(tau), float(c),
-        float(delta_theta_tau), _ptr(grad), 1 if per_sample else 0, _ptr(pre_err), _ptr(ws),
+        float(delta_theta_tau), _ptr(grad), mode, _ptr(pre_err), _ptr(ws),
         ws_bytes))
     return grad, pre_err
 
@@ -236,3 +238,95 @@ def events_to_per_neuron(indices, times, n_neurons: int, max_per_neuron: Optiona
     _lib.check(lib.bats_events_to_per_neuron(_stream(), B, P, N, K, _ptr(idx), _ptr(tim),
                                              _ptr(out_t), _ptr(out_n)))
     return out_t, out_n
+
+
+# ---------------------------------------------------------------- conv LIF and pooling
+def _shape3(v):
+    import ctypes
+    arr = (ctypes.c_int * 3)(*[int(u) for u in v])
+    return arr
+
+
+def conv_lif_forward(pre_times, pre_shape, weights, tau_s, tau, c, delta_theta_tau,
+                     max_simulation, max_n_post):
+    """Conv LIF forward (bats/Layers/ConvLIFLayer.py:77-107). weights [C_out, fx, fy, C_in];
+    pre_shape (X, Y, C_in). Returns (n, t, a, x, z, pre_exp_tau_s, pre_exp_tau)."""
+    _require_gpu()
+    B, Npre, Kp = pre_times.shape
+    Co, fx, fy, Ci = weights.shape
+    X, Y, Cin = (int(v) for v in pre_shape)
+    if X * Y * Cin != Npre or Cin != Ci:
+        raise ValueError("pre_shape / weights do not match the input spike trains")
+    Xo, Yo = X - fx + 1, Y - fy + 1
+    J = Xo * Yo * Co
+    K = int(max_n_post)
+    dev = pre_times.device
+    n = torch.empty((B, J), dtype=I32, device=dev)
+    t = torch.empty((B, J, K), dtype=F32, device=dev)
+    a = torch.empty((B, J, K), dtype=F32, device=dev)
+    x = torch.empty((B, J, K), dtype=F32, device=dev)
+    z = torch.empty((B, J, K), dtype=F32, device=dev)
+    es = torch.empty((B, Npre, Kp), dtype=F32, device=dev)
+    e = torch.empty((B, Npre, Kp), dtype=F32, device=dev)
+    lib = _lib.load()
+    ps, fs = _shape3((X, Y, Cin)), _shape3((fx, fy, Co))
+    ws_bytes = lib.bats_conv_lif_forward_workspace_size(B, ps, Kp, fs)
+    ws = _workspace(ws_bytes, dev)
+    _lib.check(lib.bats_conv_lif_forward(
+        _stream(), B, ps, Kp, fs, K, _ptr(pre_times, F32), _ptr(weights, F32), float(tau_s),
+        float(tau), float(c), float(delta_theta_tau), float(np.float32(max_simulation)),
+        _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), _ptr(es), _ptr(e), _ptr(ws), ws_bytes))
+    return n, t, a, x, z, es, e
+
+
+def conv_lif_backward(n, t, a, x, z, errors, pre_times, pre_shape, pre_es, pre_e, weights,
+                      tau_s, tau, c, delta_theta_tau, per_sample=False, pre_trainable=True,
+                      batch_sum=False):
+    """Conv LIF backward (bats/Layers/ConvLIFLayer.py:109-134); ``errors`` updated in place.
+    Returns (grad [C_out,fx,fy,C_in] or [B,...], pre_errors | None)."""
+    _require_gpu()
+    B, J, K = t.shape
+    _, Npre, Kp = pre_times.shape
+    Co, fx, fy, Ci = weights.shape
+    dev = t.device
+    grad = torch.empty(((B,) if per_sample else ()) + (Co, fx, fy, Ci), dtype=F32, device=dev)
+    pre_err = torch.empty((B, Npre, Kp), dtype=F32, device=dev) if pre_trainable else None
+    mode = 1 if per_sample else (2 if batch_sum else 0)
+    lib = _lib.load()
+    ps, fs = _shape3(pre_shape), _shape3((fx, fy, Co))
+    ws_bytes = lib.bats_conv_lif_backward_workspace_size(B, ps, fs, K)
+    ws = _workspace(ws_bytes, dev)
+    _lib.check(lib.bats_conv_lif_backward(
+        _stream(), B, ps, Kp, fs, K, _ptr(n, I32), _ptr(t, F32), _ptr(a, F32), _ptr(x, F32),
+        _ptr(z, F32), _ptr(errors, F32), _ptr(pre_times, F32), _ptr(pre_es, F32),
+        _ptr(pre_e, F32), _ptr(weights, F32), float(tau_s), float(tau), float(c),
+        float(delta_theta_tau), _ptr(grad), mode, _ptr(pre_err), _ptr(ws), ws_bytes))
+    return grad, pre_err
+
+
+def pool_forward(n, t, pre_shape):
+    """2x2 spike pooling (bats/Layers/PoolingLayer.py:49-53). Returns (n, t [B,N,4K], src)."""
+    _require_gpu()
+    B, Npre, K = t.shape
+    X, Y, C = (int(v) for v in pre_shape)
+    No = (X // 2) * (Y // 2) * C
+    dev = t.device
+    out_n = torch.empty((B, No), dtype=I32, device=dev)
+    out_t = torch.empty((B, No, 4 * K), dtype=F32, device=dev)
+    src = torch.empty((B, No, 4 * K), dtype=I32, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.bats_pool2x2_forward(_stream(), B, _shape3((X, Y, C)), K, _ptr(n, I32),
+                                        _ptr(t, F32), _ptr(out_n), _ptr(out_t), _ptr(src)))
+    return out_n, out_t, src
+
+
+def pool_backward(errors, src, pre_shape, K):
+    """Scatter pooled errors back to their source slots (bats/Layers/PoolingLayer.py:55-66)."""
+    _require_gpu()
+    B = errors.shape[0]
+    X, Y, C = (int(v) for v in pre_shape)
+    out = torch.empty((B, X * Y * C, int(K)), dtype=F32, device=errors.device)
+    lib = _lib.load()
+    _lib.check(lib.bats_pool2x2_backward(_stream(), B, _shape3((X, Y, C)), int(K),
+                                         _ptr(errors, F32), _ptr(src, I32), _ptr(out)))
+    return out

@@ -84,6 +84,7 @@ size_t bats_lif_backward_workspace_size(int B, int I, int K_pre, int J, int K_po
  * propagate_recurrent_errors does (only slots holding a spike change).
  * grad_mode 0: grad is [J,I] = batch mean of the per-sample gradients.
  * grad_mode 1: grad is [B,J,I], the per-sample gradients of the reference.
+ * grad_mode 2: grad is [J,I] = batch sum (the gradient of a summed loss, for autograd).
  * pre_errors [B,I,K_pre] may be NULL (previous layer not trainable).
  */
 int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, int K_post,
@@ -94,6 +95,63 @@ int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, int K_post,
                       float tau_s, float tau, float c, float delta_theta_tau,
                       float *grad, int grad_mode, float *pre_errors,
                       void *workspace, size_t workspace_bytes);
+
+/* ------------------------------------------------------------------ conv LIF and pooling */
+
+/*
+ * Shapes: pre_shape = {X, Y, C_in} of the input layer, filters_shape = {fx, fy, C_out}
+ * (the reference's ConvLIFLayer(filters_shape=[fx, fy, C_out])); weights [C_out, fx, fy, C_in];
+ * neuron index (x*Y + y)*C + c; valid, stride 1, output {X-fx+1, Y-fy+1, C_out}.
+ */
+size_t bats_conv_lif_forward_workspace_size(int B, const int *pre_shape, int K_pre,
+                                            const int *filters_shape);
+
+/*
+ * Conv LIF forward. Replaces bats/Layers/ConvLIFLayer.py:77-107 with compute_pre_exps,
+ * get_sorted_spikes_indices and compute_spike_times_conv
+ * (bats/CudaKernels/Wrappers/Inference/compute_spike_times_conv.py:11-35). Outputs as
+ * bats_lif_forward, with J = (X-fx+1)*(Y-fy+1)*C_out.
+ */
+int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, int K_pre,
+                          const int *filters_shape, int K_post, const float *pre_times,
+                          const float *weights, float tau_s, float tau, float c,
+                          float delta_theta_tau, float max_simulation, int *out_n,
+                          float *out_t, float *out_a, float *out_x, float *out_z,
+                          float *pre_exp_tau_s, float *pre_exp_tau, void *workspace,
+                          size_t workspace_bytes);
+
+size_t bats_conv_lif_backward_workspace_size(int B, const int *pre_shape,
+                                             const int *filters_shape, int K_post);
+
+/*
+ * Conv LIF backward. Replaces bats/Layers/ConvLIFLayer.py:109-134 with
+ * propagate_recurrent_errors, compute_factors, compute_weights_gradient_conv
+ * (bats/CudaKernels/Wrappers/Backpropagation/compute_weights_gradient_conv.py:11-32) and
+ * propagate_errors_to_pre_spikes_conv (.../propagate_errors_to_pre_spikes_conv.py:11-32).
+ * grad: grad_mode 0 [C_out,fx,fy,C_in] batch mean, 1 [B,C_out,fx,fy,C_in] per sample,
+ * 2 batch sum. errors updated in place; pre_errors may be NULL.
+ */
+int bats_conv_lif_backward(void *stream, int B, const int *pre_shape, int K_pre,
+                           const int *filters_shape, int K_post, const int *n, const float *t,
+                           const float *a, const float *x, const float *z, float *errors,
+                           const float *pre_times, const float *pre_exp_tau_s,
+                           const float *pre_exp_tau, const float *weights, float tau_s,
+                           float tau, float c, float delta_theta_tau, float *grad,
+                           int grad_mode, float *pre_errors, void *workspace,
+                           size_t workspace_bytes);
+
+/*
+ * 2x2 spike pooling. Replaces bats/Layers/PoolingLayer.py:49-53 with aggregate_spikes_conv
+ * (bats/CudaKernels/Wrappers/Inference/aggregate_spikes_conv.py:10-28): out_n [B,N_post],
+ * out_t [B,N_post,4K] ascending (+inf last), out_src [B,N_post,4K] int32 flat index of each
+ * slot's source in t [B,N_pre,K]. N_post = (X/2)*(Y/2)*C.
+ */
+int bats_pool2x2_forward(void *stream, int B, const int *pre_shape, int K, const int *n,
+                         const float *t, int *out_n, float *out_t, int *out_src);
+
+/* Pooling backward (bats/Layers/PoolingLayer.py:55-66): pre_errors[src] = errors. */
+int bats_pool2x2_backward(void *stream, int B, const int *pre_shape, int K, const float *errors,
+                          const int *src, float *pre_errors);
 
 /* ------------------------------------------------------------------ losses */
 

@@ -1,0 +1,85 @@
+"""torch.autograd integration: LIFFunction gradients against the FP64 oracle (batch sums,
+1e-4 relative + 1e-6 max-relative) and against central finite differences of the GPU forward
+(h = 1e-4 relative to |w| + 0.1 on 64 random weights; the forward returns float32 times, so the
+finite differences carry ~1e-3 relative noise and are held to 2e-2)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lif as olif
+from tests.helpers import assert_close_rel
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _inputs(seed):
+    rng = np.random.default_rng(seed)
+    B, I, J, K = 4, 60, 20, 6
+    n = rng.integers(0, 3, size=(B, I))
+    t = np.sort(rng.uniform(0, 0.5, size=(B, I, 2)), axis=2).astype(F32)
+    t[np.arange(2)[None, None, :] >= n[:, :, None]] = np.inf
+    W = rng.normal(0.4, 1.0, size=(J, I)).astype(F32)
+    e = rng.normal(size=(B, J, K)).astype(F32)
+    return t, W, e
+
+
+PARAMS = dict(tau_s=0.1, theta=0.4, delta_theta=0.2, max_n_spike=6, t_max=0.6)
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_autograd_matches_oracle(recompute):
+    from bats_amd.autograd import lif
+    t, W, e = _inputs(0)
+    pt = torch.as_tensor(t, device="cuda").requires_grad_(True)
+    Wd = torch.as_tensor(W, device="cuda").requires_grad_(True)
+    ed = torch.as_tensor(e, device="cuda")
+    out_t, out_n = lif(pt, Wd, recompute=recompute, **PARAMS)
+    L = torch.where(torch.isfinite(out_t), out_t * ed, torch.zeros_like(out_t)).sum()
+    L.backward()
+    st = olif.lif_forward(t, W, PARAMS["tau_s"], PARAMS["theta"], PARAMS["delta_theta"],
+                          PARAMS["max_n_spike"], tmax=PARAMS["t_max"])
+    assert np.array_equal(out_n.cpu().numpy(), st.n)
+    assert st.n.sum() > 20
+    err = np.where(np.isfinite(st.t), e, 0.0).astype(F32)
+    _, g, p = olif.lif_backward(st, err, pre_trainable=True)
+    assert_close_rel(Wd.grad.cpu().numpy(), g.astype(np.float64).sum(axis=0), 1e-4, 1e-6, "dL/dW")
+    gp = pt.grad.cpu().numpy()
+    fin = np.isfinite(t)
+    assert_close_rel(gp[fin], p[fin], 1e-4, 1e-6, "dL/dt_pre")
+
+
+def test_autograd_finite_differences():
+    from bats_amd.autograd import lif
+    t, W, e = _inputs(1)
+    pt = torch.as_tensor(t, device="cuda")
+    ed = torch.as_tensor(e, device="cuda")
+
+    def L(Wnp):
+        ot, on = lif(pt, torch.as_tensor(Wnp, device="cuda"), **PARAMS)
+        val = torch.where(torch.isfinite(ot), ot.double() * ed.double(),
+                          torch.zeros_like(ot, dtype=torch.float64)).sum()
+        return float(val.item()), on.cpu().numpy()
+
+    Wd = torch.as_tensor(W, device="cuda").requires_grad_(True)
+    ot, on = lif(pt, Wd, **PARAMS)
+    torch.where(torch.isfinite(ot), ot * ed, torch.zeros_like(ot)).sum().backward()
+    g = Wd.grad.cpu().numpy()
+    n0 = on.cpu().numpy()
+    rng = np.random.default_rng(2)
+    picks = [(rng.integers(W.shape[0]), rng.integers(W.shape[1])) for _ in range(64)]
+    checked = 0
+    scale = np.abs(g).max()
+    for j, i in picks:
+        h = np.float32(1e-4 * (abs(W[j, i]) + 0.1))
+        Wp, Wm = W.copy(), W.copy()
+        Wp[j, i] += h
+        Wm[j, i] -= h
+        lp, np_ = L(Wp)
+        lm, nm = L(Wm)
+        if not (np.array_equal(np_, n0) and np.array_equal(nm, n0)):
+            continue
+        fd = (lp - lm) / (float(Wp[j, i]) - float(Wm[j, i]))
+        assert abs(fd - g[j, i]) <= 2e-2 * max(abs(g[j, i]), 0.05 * scale), (j, i, fd, g[j, i])
+        checked += 1
+    assert checked >= 40

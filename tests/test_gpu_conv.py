@@ -1,0 +1,141 @@
+"""GPU parity of the conv LIF layer and 2x2 pooling against the FP64 oracle, at the dense
+layer's tolerances: identical spike counts, spike times / a / x / z within 1e-5 relative,
+gradients and errors within 1e-4 |ref| + 1e-6 max|ref|; pooling exact."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import conv as oconv
+from oracle import lif as olif
+from tests.helpers import assert_close_rel, assert_forward_match, latency_inputs
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _inputs(rng, B, shape, K, tmax=0.6, p=0.5):
+    X, Y, C = shape
+    N = X * Y * C
+    n = (rng.uniform(size=(B, N)) < p) * rng.integers(1, K + 1, size=(B, N))
+    t = np.sort(rng.uniform(0, tmax, size=(B, N, K)), axis=2).astype(F32)
+    t[np.arange(K)[None, None, :] >= n[:, :, None]] = np.inf
+    return t
+
+
+def _T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def _gpu_conv(t, W, pre_shape, params, K, tmax):
+    from bats_amd import ops
+    from bats_amd._constants import lif_constants
+    consts = lif_constants(*params)
+    out = ops.conv_lif_forward(_T(t), pre_shape, _T(W), *consts, tmax, K)
+    return out, consts
+
+
+@pytest.mark.parametrize("case", ["small", "c4_conv1"])
+def test_conv_forward_backward(case):
+    rng = np.random.default_rng(0)
+    if case == "small":
+        pre_shape, W = (12, 10, 2), rng.normal(0.5, 1.0, size=(4, 5, 3, 2)).astype(F32)
+        t = _inputs(rng, 3, pre_shape, 3)
+        params, K, tmax = (0.05, 0.15, 0.1), 5, 0.6
+    else:
+        # C4's first layer: 28x28x1 latency input, 5x5x15 filters, 30 spikes
+        pre_shape = (28, 28, 1)
+        t, _ = latency_inputs(rng, 4, 784)
+        W = rng.normal(0.0, 1.0, size=(15, 5, 5, 1)).astype(F32) * 3.0
+        params, K, tmax = (0.130, 1.0, 1.0), 30, 1.0
+    print(f"[conv] {case}", flush=True)
+    ref = oconv.conv_forward(t, W, pre_shape, *params, K, tmax=tmax)
+    (n, tt, a, x, z, es, e), consts = _gpu_conv(t, W, pre_shape, params, K, tmax)
+    torch.cuda.synchronize()
+    assert_forward_match(n.cpu().numpy(), tt.cpu().numpy(), a.cpu().numpy(), x.cpu().numpy(),
+                         z.cpu().numpy(), ref)
+    assert ref.n.sum() > 10
+    from bats_amd import ops
+    err = rng.normal(size=ref.t.shape).astype(F32)
+    e_ref, g_ref, p_ref = oconv.conv_backward(ref, err, pre_trainable=True)
+    err_d = _T(err)
+    g, p = ops.conv_lif_backward(n, tt, a, x, z, err_d, _T(t), pre_shape, es, e, _T(W), *consts,
+                                 per_sample=True, pre_trainable=True)
+    torch.cuda.synchronize()
+    assert_close_rel(err_d.cpu().numpy(), e_ref, 1e-4, 1e-6, "conv reset-chain errors")
+    assert_close_rel(g.cpu().numpy(), g_ref, 1e-4, 1e-6, "conv per-sample gradient")
+    assert_close_rel(p.cpu().numpy(), p_ref, 1e-4, 1e-6, "conv pre-spike errors")
+    err_d = _T(err)
+    gm, _ = ops.conv_lif_backward(n, tt, a, x, z, err_d, _T(t), pre_shape, es, e, _T(W), *consts,
+                                  per_sample=False, pre_trainable=False)
+    assert_close_rel(gm.cpu().numpy(), g_ref.astype(np.float64).mean(axis=0), 1e-4, 1e-6,
+                     "conv mean gradient")
+
+
+def test_pooling_forward_backward():
+    from bats_amd import ops
+    rng = np.random.default_rng(1)
+    pre_shape, K = (8, 6, 3), 4
+    t = _inputs(rng, 3, pre_shape, K)
+    n = np.isfinite(t).sum(axis=2).astype(np.int32)
+    n_ref, t_ref, src_ref, _ = oconv.pool_forward(n, t, pre_shape)
+    no, to, src = ops.pool_forward(_T(n), _T(t), pre_shape)
+    torch.cuda.synchronize()
+    assert np.array_equal(no.cpu().numpy(), n_ref)
+    assert np.array_equal(to.cpu().numpy(), t_ref)
+    # the source of every finite slot is the same (ties keep the children's order)
+    fin = np.isfinite(t_ref)
+    assert np.array_equal(src.cpu().numpy()[fin], src_ref[fin])
+    err = rng.normal(size=t_ref.shape).astype(F32)
+    back = ops.pool_backward(_T(err), src, pre_shape, K).cpu().numpy()
+    ref_back = oconv.pool_backward(err, src.cpu().numpy().astype(np.int64), t.shape[1], K)
+    assert np.array_equal(back, ref_back)
+
+
+def test_conv_network_forward_chain():
+    """C4's architecture at B = 2: conv 5x5x15 -> pool -> conv 5x5x40 -> pool -> 300 -> 10,
+    layer by layer against the oracle fed the same inputs."""
+    from bats_amd import Network
+    from bats_amd.Layers import ConvInputLayer, ConvLIFLayer, LIFLayer, PoolingLayer
+    rng = np.random.default_rng(2)
+    t, n_in = latency_inputs(rng, 2, 784)
+    W1 = (3.0 * rng.normal(size=(15, 5, 5, 1))).astype(F32)
+    W2 = (1.5 * rng.normal(size=(40, 5, 5, 15))).astype(F32)
+    W3 = rng.normal(size=(300, 640)).astype(F32)
+    W4 = rng.normal(size=(10, 300)).astype(F32)
+    net = Network()
+    inp = ConvInputLayer(neurons_shape=np.array([28, 28, 1]), name="in")
+    net.add_layer(inp, input=True)
+    c1 = ConvLIFLayer(inp, filters_shape=np.array([5, 5, 15]), tau_s=0.13, theta=1.0,
+                      delta_theta=1.0, weight_initializer=lambda *s: W1, max_n_spike=30, name="c1")
+    net.add_layer(c1)
+    p1 = PoolingLayer(c1, name="p1")
+    net.add_layer(p1)
+    c2 = ConvLIFLayer(p1, filters_shape=np.array([5, 5, 40]), tau_s=0.13, theta=1.0,
+                      delta_theta=1.0, weight_initializer=lambda *s: W2, max_n_spike=10, name="c2")
+    net.add_layer(c2)
+    p2 = PoolingLayer(c2, name="p2")
+    net.add_layer(p2)
+    d3 = LIFLayer(p2, tau_s=0.13, theta=1.0, delta_theta=1.0, n_neurons=300,
+                  weight_initializer=lambda *s: W3, max_n_spike=30, name="d3")
+    net.add_layer(d3)
+    d4 = LIFLayer(d3, tau_s=0.13, theta=1.3, delta_theta=1.3, n_neurons=10,
+                  weight_initializer=lambda *s: W4, max_n_spike=30, name="d4")
+    net.add_layer(d4)
+    net.forward(t.reshape(2, 784, 1), n_in, max_simulation=1.0)
+    torch.cuda.synchronize()
+    # layer-by-layer parity, each oracle layer fed the GPU layer's input
+    r1 = oconv.conv_forward(t, W1, (28, 28, 1), 0.13, 1.0, 1.0, 30, tmax=1.0)
+    assert np.array_equal(c1.spike_trains[1].cpu().numpy(), r1.n)
+    n1 = c1.spike_trains[1].cpu().numpy()
+    t1 = c1.spike_trains[0].cpu().numpy()
+    pn, pt, _, shp = oconv.pool_forward(n1, t1, (24, 24, 15))
+    assert np.array_equal(p1.spike_trains[0].cpu().numpy(), pt)
+    r2 = oconv.conv_forward(pt, W2, shp, 0.13, 1.0, 1.0, 10, tmax=1.0)
+    assert np.array_equal(c2.spike_trains[1].cpu().numpy(), r2.n)
+    t2 = c2.spike_trains[0].cpu().numpy()
+    _, pt2, _, _ = oconv.pool_forward(c2.spike_trains[1].cpu().numpy(), t2, (8, 8, 40))
+    r3 = olif.lif_forward(pt2, W3, 0.13, 1.0, 1.0, 30, tmax=1.0)
+    assert np.array_equal(d3.spike_trains[1].cpu().numpy(), r3.n)
+    r4 = olif.lif_forward(d3.spike_trains[0].cpu().numpy(), W4, 0.13, 1.3, 1.3, 30, tmax=1.0)
+    assert np.array_equal(d4.spike_trains[1].cpu().numpy(), r4.n)
+    assert r1.n.sum() > 0 and r2.n.sum() > 0

@@ -55,27 +55,40 @@ constexpr float kInfF = __builtin_huge_valf();
 constexpr double kInfD = __builtin_huge_val();
 
 // One time-sorted input event of one sample, read uniformly by every lane of a wave.
-// es/e are the float32 exps exactly as stored for the backward, widened to float64; zq/zl
-// are float64 exp(t/tau) and exp(min(t_next, t_max)/tau); czl2 = c*zl*zl;
-// tl = min(t_next, t_max).
+// es/e are the float32 exps exactly as stored for the backward, widened to float64. The
+// crossing filter's bounds for the interval after the event are precomputed in float64 with
+// zq = exp(t/tau) and zl = exp(min(t_next, t_max)/tau) (capped at 1e300):
+//   czl2 = c zl^2, zl1 = zl (1 + eps), blo = 2c zq (1 - eps), bhi = 2c zl (1 + eps).
+// tl = min(t_next, t_max). Padding events (after the sample's last event) have es = e = 0
+// and bounds that no state passes.
 struct alignas(16) Event {
   double es;
   double e;
-  double zq;
-  double zl;
   double czl2;
+  double zl1;
+  double blo;
+  double bhi;
+  double zq;  // exp(t / tau)
+  double zl;  // exp(tl / tau), at most 1e300
   int idx;
   float t;
   float tl;
   float pad;
 };
-static_assert(sizeof(Event) == 64, "Event layout");
+static_assert(sizeof(Event) == 80, "Event layout");
+
+// Relative margin of the crossing filter (far above float64 rounding of the quantities it
+// compares, far below the spacing of distinct float32 spike times).
+constexpr double kFilterEps = 1e-10;
+// Events are padded to a multiple of this many per sample.
+constexpr int kEventPad = 8;
 
 // Per-sample time-ordered event lists built by sort_events_kernel (lif_forward.hip).
 struct EventLists {
   Event *events;  // [B, Pcap]
-  int *counts;    // [B]
-  int Pcap;
+  int *idx;       // [B, Pcap] pre-neuron index of each event (also in Event), for wide loads
+  int *counts;    // [B] real events per sample (padding excluded)
+  int Pcap;       // multiple of kEventPad
 };
 size_t event_lists_bytes(int B, int S);
 int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,

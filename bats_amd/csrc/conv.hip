@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kConvMaxThreads) void conv_solve_kernel(
   const int ry0 = y0, ry1 = min(g.Y, y0 + TY + g.fy - 1);
   const int P = counts[b];
   const Event *ev = events + (size_t)b * Pcap;
-  const double c4 = 4.0 * c, inv2c = 0.5 / c;
+  const double c4 = 4.0 * c;
   const double zmax = exp((double)tmax / tau);
   double a = 0.0, bb = 0.0, zq_last = 0.0;
   float t_last = -kInfF;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kConvMaxThreads) void conv_solve_kernel(
           // interval (t_last, min(tu, t_max)] with the state after the previous field event
           const float tl = fminf(tu, tmax);
           const double zl = (tu <= tmax) ? sev[u].ev.zq : zmax;
-          if (always_exact || crossing_candidate3(a, bb, zq_last, zl, c * zl * zl, c4, inv2c)) {
+          if (always_exact || crossing_filter3(a, bb, zq_last, zl, c, c4)) {
             const SolveState r = solve_crossings_ool(a, bb, n, (double)t_last, (double)tl, c,
                                                      dtt, tau, K, row, out_t, out_a, out_x,
                                                      out_z);
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kConvMaxThreads) void conv_solve_kernel(
   }
   if (live && seen) {
     const double zl = zmax;
-    if (always_exact || crossing_candidate3(a, bb, zq_last, zl, c * zl * zl, c4, inv2c)) {
+    if (always_exact || crossing_filter3(a, bb, zq_last, zl, c, c4)) {
       const SolveState r = solve_crossings_ool(a, bb, n, (double)t_last, (double)tmax, c, dtt,
                                                tau, K, row, out_t, out_a, out_x, out_z);
       n = r.n;
@@ -409,6 +409,13 @@ extern "C" int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, 
   const int threads = std::min(kConvMaxThreads, (lanes + 63) / 64 * 64);
   const int tiles = ((g.Xo + TX - 1) / TX) * ((g.Yo + TY - 1) / TY);
   const size_t lds = sizeof(ConvEv) * threads + sizeof(int) * threads;
+  if (lds > 65536) {
+    // more than 655 output channels: one position per block needs up to 100 KB of LDS
+    BATS_REQUIRE(hipFuncSetAttribute((const void *)conv_solve_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds) == hipSuccess,
+                 "conv_solve_kernel: LDS request above the device limit");
+  }
   BATS_TIMED("conv_solve_kernel", st,
              hipLaunchKernelGGL(conv_solve_kernel, dim3(tiles, B), dim3(threads), lds, st,
                                 el.events, el.counts, el.Pcap, g, TX, TY, WT, K_post, (double)c,

@@ -14,10 +14,12 @@
 //      and the float64 suffix sums S1[k] = sum_{k'>=k} e f1, S2[k] = sum_{k'>=k} e f2.
 //      V_j(t) is then S[first k with t_jk > t].
 //   2. wgrad_kernel      lanes over pre neurons i, a tile of post neurons j, a slice of the
-//      batch; the post neuron's spike list is wave-uniform (scalar loads), the lookup is a
-//      select chain over its at most K spikes; float64 accumulation; a deterministic
-//      fixed-order reduction over batch slices (wgrad_reduce_kernel) finishes the mean.
-//   3. pre_err_kernel    lanes over pre neurons i of one sample, loop over post neurons j.
+//      batch; the post rows of a few samples at a time are staged in LDS (wave-uniform
+//      reads), the lookup counts the post spikes at or before t_m; float64 accumulation;
+//      a deterministic fixed-order reduction over batch slices (wgrad_reduce_kernel)
+//      finishes the mean.
+//   3. pre_err_kernel    lanes over pre neurons i of one sample, post rows staged in LDS
+//      a tile at a time.
 #include "common.hpp"
 
 #include <algorithm>
@@ -62,95 +64,96 @@ __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
   n_eff[lane] = ne;
 }
 
-// V(t) lookup: S[first k with tpost[k] > tp] (0 when none), post times ascending.
-__device__ __forceinline__ void lookup(const float *__restrict__ tpost,
-                                       const double *__restrict__ s1,
-                                       const double *__restrict__ s2, int nj, float tp,
-                                       double &v1, double &v2) {
-  v1 = 0.0;
-  v2 = 0.0;
-  for (int k = nj - 1; k >= 0; --k) {
-    if (tpost[k] > tp) {
-      v1 = s1[k];
-      v2 = s2[k];
-    }
-  }
-}
-
-// Post-neuron rows (b, j0 .. j0+jn-1) staged in LDS: spike counts, times and suffix sums of
-// the valid slots only.
+// Post-neuron rows staged in LDS for SB samples at a time: spike counts, times and suffix
+// sums of rows (b, j0 .. j0+jn-1), b = b0 .. b0+sb-1. Rows are copied whole (slots past a
+// row's spike count are never read), so one copy and two barriers serve SB samples.
 struct RowStage {
-  int *ne;
-  float *t;
-  double *s1;
-  double *s2;
+  int *ne;     // [SB][JT]
+  float *t;    // [SB][JT][K]
+  double *s1;  // [SB][JT][K]
+  double *s2;  // [SB][JT][K]
 };
 
-__device__ __forceinline__ RowStage row_stage(char *smem, int JT, int K) {
+__device__ __forceinline__ RowStage row_stage(char *smem, int SB, int JT, int K) {
   RowStage r;
+  const size_t n = (size_t)SB * JT * K;
   r.s1 = reinterpret_cast<double *>(smem);
-  r.s2 = r.s1 + (size_t)JT * K;
-  r.t = reinterpret_cast<float *>(r.s2 + (size_t)JT * K);
-  r.ne = reinterpret_cast<int *>(r.t + (size_t)JT * K);
+  r.s2 = r.s1 + n;
+  r.t = reinterpret_cast<float *>(r.s2 + n);
+  r.ne = reinterpret_cast<int *>(r.t + n);
   return r;
 }
 
-__host__ __device__ constexpr size_t row_stage_bytes(int JT, int K) {
-  return (size_t)JT * K * (2 * sizeof(double) + sizeof(float)) + (size_t)JT * sizeof(int);
+__host__ __device__ constexpr size_t row_stage_bytes(int SB, int JT, int K) {
+  return (size_t)SB * JT * K * (2 * sizeof(double) + sizeof(float)) + (size_t)SB * JT * sizeof(int);
 }
 
-// Loads rows (b, j0 ..) into the stage. Contains barriers: call from every thread.
-__device__ __forceinline__ void load_rows(const RowStage &st, int b, int J, int j0, int jn, int K,
+// Copies rows of samples b0 .. b0+sb-1 into the stage. Contains barriers: call from every
+// thread of the block.
+__device__ __forceinline__ void load_rows(const RowStage &st, int b0, int sb, int J, int j0,
+                                          int jn, int JT, int K,
                                           const float *__restrict__ post_t,
                                           const int *__restrict__ n_eff,
                                           const double *__restrict__ S1,
                                           const double *__restrict__ S2) {
   const int tid = threadIdx.x, nth = blockDim.x;
   __syncthreads();  // previous contents consumed
-  if (tid < jn) st.ne[tid] = n_eff[(size_t)b * J + j0 + tid];
-  __syncthreads();
-  const size_t base = ((size_t)b * J + j0) * K;
-  for (int e = tid; e < jn * K; e += nth) {
-    const int r = e / K;
-    const int k = e - r * K;
-    if (k < st.ne[r]) {
-      st.t[e] = post_t[base + e];
-      st.s1[e] = S1[base + e];
-      st.s2[e] = S2[base + e];
-    }
+  const int per = jn * K;
+  for (int e = tid; e < sb * per; e += nth) {
+    const int s = e / per;
+    const int r = e - s * per;
+    const size_t src = ((size_t)(b0 + s) * J + j0) * K + r;
+    const int dst = s * JT * K + r;
+    st.t[dst] = post_t[src];
+    st.s1[dst] = S1[src];
+    st.s2[dst] = S2[src];
+  }
+  for (int e = tid; e < sb * jn; e += nth) {
+    const int s = e / jn;
+    const int r = e - s * jn;
+    st.ne[s * JT + r] = n_eff[(size_t)(b0 + s) * J + j0 + r];
   }
   __syncthreads();
 }
 
-__device__ __forceinline__ void lookup_lds(const RowStage &st, int r, int K, float tp,
-                                           double &v1, double &v2) {
-  v1 = 0.0;
-  v2 = 0.0;
-  const int nj = st.ne[r];
-  const float *tr = st.t + r * K;
-  const double *s1 = st.s1 + r * K;
-  const double *s2 = st.s2 + r * K;
-  for (int k = nj - 1; k >= 0; --k) {
-    if (tr[k] > tp) {
-      v1 = s1[k];
-      v2 = s2[k];
-    }
+// V(t): S[first k with tpost[k] > tp], 0 when there is none; the post times of a row are
+// ascending, so the index is the count of times <= tp.
+__device__ __forceinline__ void lookup_count(const float *__restrict__ tr,
+                                             const double *__restrict__ s1,
+                                             const double *__restrict__ s2, int nj, float tp,
+                                             double &v1, double &v2) {
+  int kk = 0;
+  for (int k = 0; k < nj; ++k) kk += (tr[k] <= tp) ? 1 : 0;
+  v1 = (kk < nj) ? s1[kk] : 0.0;
+  v2 = (kk < nj) ? s2[kk] : 0.0;
+}
+
+// Number of leading finite slots of a spike row (independent loads, no dependent chain).
+__device__ __forceinline__ int leading_finite(const float *__restrict__ row, int K) {
+  int n = 0;
+  bool run = true;
+#pragma unroll 8
+  for (int m = 0; m < K; ++m) {
+    run = run && isfinite(row[m]);
+    n += run ? 1 : 0;
   }
+  return n;
 }
 
 // Lanes are pre neurons (a block covers up to 1024 of them), a block owns kWJT post neurons
-// and a slice of the batch. Per sample the block stages its post rows in LDS; every lane
-// accumulates es*V1 - e*V2 over its pre spikes in float64.
+// and a slice of the batch. Per group of samples the block stages its post rows in LDS;
+// every lane accumulates es*V1 - e*V2 over its pre spikes in float64.
 constexpr int kWJT = 16;
 
 __global__ __launch_bounds__(1024) void wgrad_kernel(
-    int B, int I, int K_pre, int J, int K, int b_per_slice, const float *__restrict__ pre_t,
-    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
-    const float *__restrict__ post_t, const int *__restrict__ n_eff,
-    const double *__restrict__ S1, const double *__restrict__ S2, int per_sample,
-    float *__restrict__ grad, float *__restrict__ partial) {
+    int B, int I, int K_pre, int J, int K, int b_per_slice, int SB,
+    const float *__restrict__ pre_t, const float *__restrict__ pre_es,
+    const float *__restrict__ pre_e, const float *__restrict__ post_t,
+    const int *__restrict__ n_eff, const double *__restrict__ S1,
+    const double *__restrict__ S2, int per_sample, float *__restrict__ grad,
+    float *__restrict__ partial) {
   extern __shared__ __align__(16) char smem[];
-  const RowStage st = row_stage(smem, kWJT, K);
+  const RowStage st = row_stage(smem, SB, kWJT, K);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < I;
   const int ic = active ? i : 0;
@@ -162,29 +165,35 @@ __global__ __launch_bounds__(1024) void wgrad_kernel(
   double acc[kWJT];
 #pragma unroll
   for (int u = 0; u < kWJT; ++u) acc[u] = 0.0;
-  for (int b = b0; b < b1; ++b) {
-    load_rows(st, b, J, j0, jn, K, post_t, n_eff, S1, S2);
-    const size_t pbase = ((size_t)b * I + ic) * K_pre;
-    for (int m = 0; m < K_pre; ++m) {
-      const float tp = active ? pre_t[pbase + m] : kInfF;
-      if (!isfinite(tp)) break;
-      const double es = (double)pre_es[pbase + m];
-      const double e = (double)pre_e[pbase + m];
+  for (int bg = b0; bg < b1; bg += SB) {
+    const int sb = min(SB, b1 - bg);
+    load_rows(st, bg, sb, J, j0, jn, kWJT, K, post_t, n_eff, S1, S2);
+    for (int s = 0; s < sb; ++s) {
+      const int b = bg + s;
+      const size_t pbase = ((size_t)b * I + ic) * K_pre;
+      const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
+      for (int m = 0; m < np; ++m) {
+        const float tp = pre_t[pbase + m];
+        const double es = (double)pre_es[pbase + m];
+        const double e = (double)pre_e[pbase + m];
 #pragma unroll
-      for (int u = 0; u < kWJT; ++u) {
-        if (u < jn && st.ne[u] > 0) {
-          double v1, v2;
-          lookup_lds(st, u, K, tp, v1, v2);
-          acc[u] = fma(es, v1, acc[u]);
-          acc[u] = fma(-e, v2, acc[u]);
+        for (int u = 0; u < kWJT; ++u) {
+          const int nj = (u < jn) ? st.ne[s * kWJT + u] : 0;
+          if (nj > 0) {
+            const int off = (s * kWJT + u) * K;
+            double v1, v2;
+            lookup_count(st.t + off, st.s1 + off, st.s2 + off, nj, tp, v1, v2);
+            acc[u] = fma(es, v1, acc[u]);
+            acc[u] = fma(-e, v2, acc[u]);
+          }
         }
       }
-    }
-    if (per_sample) {
+      if (per_sample) {
 #pragma unroll
-      for (int u = 0; u < kWJT; ++u) {
-        if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
-        acc[u] = 0.0;
+        for (int u = 0; u < kWJT; ++u) {
+          if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
+          acc[u] = 0.0;
+        }
       }
     }
   }
@@ -200,9 +209,16 @@ __global__ __launch_bounds__(kBwdThreads) void wgrad_reduce_kernel(
     float *__restrict__ grad) {
   const int64_t e = (int64_t)blockIdx.x * kBwdThreads + threadIdx.x;
   if (e >= JI) return;
-  double s = 0.0;
-  for (int z = 0; z < slices; ++z) s += (double)partial[(size_t)z * JI + e];
-  grad[e] = (float)(s * inv_b);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int z = 0;
+  for (; z + 4 <= slices; z += 4) {
+    s0 += (double)partial[(size_t)z * JI + e];
+    s1 += (double)partial[(size_t)(z + 1) * JI + e];
+    s2 += (double)partial[(size_t)(z + 2) * JI + e];
+    s3 += (double)partial[(size_t)(z + 3) * JI + e];
+  }
+  for (; z < slices; ++z) s0 += (double)partial[(size_t)z * JI + e];
+  grad[e] = (float)(((s0 + s1) + (s2 + s3)) * inv_b);
 }
 
 // Lanes are pre neurons of one sample; the post rows are staged kWJT at a time and every
@@ -216,29 +232,27 @@ __global__ __launch_bounds__(1024) void pre_err_kernel(
     const double *__restrict__ S2, double tau_s, double tau, double *__restrict__ acc_scratch,
     float *__restrict__ pre_errors) {
   extern __shared__ __align__(16) char smem[];
-  const RowStage st = row_stage(smem, kWJT, K);
+  const RowStage st = row_stage(smem, 1, kWJT, K);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   const bool active = i < I;
   const int ic = active ? i : 0;
   const size_t pbase = ((size_t)b * I + ic) * K_pre;
-  // number of leading finite pre spikes of this lane
-  int np = 0;
-  if (active)
-    while (np < K_pre && isfinite(pre_t[pbase + np])) ++np;
+  const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
   const bool single_chunk = J <= kWJT;
   for (int j0 = 0; j0 < J; j0 += kWJT) {
     const int jn = min(kWJT, J - j0);
-    load_rows(st, b, J, j0, jn, K, post_t, n_eff, S1, S2);
+    load_rows(st, b, 1, J, j0, jn, kWJT, K, post_t, n_eff, S1, S2);
     for (int m = 0; m < np; ++m) {
       const float tp = pre_t[pbase + m];
       const double es = (double)pre_es[pbase + m];
       const double e = (double)pre_e[pbase + m];
       double acc = (j0 == 0) ? 0.0 : acc_scratch[pbase + m];
       for (int u = 0; u < jn; ++u) {
-        if (st.ne[u] == 0) continue;
+        const int nj = st.ne[u];
+        if (nj == 0) continue;
         double v1, v2;
-        lookup_lds(st, u, K, tp, v1, v2);
+        lookup_count(st.t + u * K, st.s1 + u * K, st.s2 + u * K, nj, tp, v1, v2);
         const double w = (double)W[(size_t)(j0 + u) * I + i];
         acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
       }
@@ -248,14 +262,8 @@ __global__ __launch_bounds__(1024) void pre_err_kernel(
         acc_scratch[pbase + m] = acc;
     }
   }
-  if (active) {
-    // slots without a spike (the reference's pre_errors are zero there)
-    for (int m = np; m < K_pre; ++m) {
-      const float tp = pre_t[pbase + m];
-      pre_errors[pbase + m] = 0.f;
-      (void)tp;
-    }
-  }
+  if (active)
+    for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
 }
 
 struct BwdLayout {
@@ -343,7 +351,10 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   double *S2 = reinterpret_cast<double *>(ws + L.s2);
   float *partial = reinterpret_cast<float *>(ws + L.partial);
   double *scratch = reinterpret_cast<double *>(ws + L.scratch);
-  const size_t lds = row_stage_bytes(kWJT, K_post);
+  // samples staged per round: as many as fit 48 KB of LDS (at least one)
+  const int SB = (int)std::max<size_t>(1, std::min<size_t>(4, (48 * 1024) / row_stage_bytes(1, kWJT, K_post)));
+  const size_t lds_w = row_stage_bytes(SB, kWJT, K_post);
+  const size_t lds_p = row_stage_bytes(1, kWJT, K_post);
 
   const int BJ = B * J;
   if (BJ > 0) {
@@ -357,16 +368,16 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
     const int gj = (J + kWJT - 1) / kWJT;
     if (grad_mode == 1) {
       dim3 g(L.gi, gj, B);  // one sample per block
-      hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds, st, B, I, K_pre, J, K_post, 1,
-                         pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1, S2, 1, grad,
+      hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds_w, st, B, I, K_pre, J, K_post, 1,
+                         SB, pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1, S2, 1, grad,
                          partial);
       if (int r = check_launch("wgrad_kernel")) return r;
     } else {
       dim3 g(L.gi, gj, L.slices);
       BATS_TIMED("wgrad_kernel", st,
-                 hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds, st, B, I, K_pre, J,
-                                    K_post, L.b_per_slice, pre_times, pre_exp_tau_s, pre_exp_tau,
-                                    t, n_eff, S1, S2, 0, grad, partial));
+                 hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds_w, st, B, I, K_pre, J,
+                                    K_post, L.b_per_slice, SB, pre_times, pre_exp_tau_s,
+                                    pre_exp_tau, t, n_eff, S1, S2, 0, grad, partial));
       if (int r = check_launch("wgrad_kernel")) return r;
       hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
                          dim3(kBwdThreads), 0, st, L.slices, JI,
@@ -378,7 +389,7 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
     BATS_REQUIRE(weights, "null weights");
     dim3 g(L.gi, B);
     BATS_TIMED("pre_err_kernel", st,
-               hipLaunchKernelGGL(pre_err_kernel, g, dim3(L.lanes), lds, st, I, K_pre, J, K_post,
+               hipLaunchKernelGGL(pre_err_kernel, g, dim3(L.lanes), lds_p, st, I, K_pre, J, K_post,
                                   pre_times, pre_exp_tau_s, pre_exp_tau, weights, t, n_eff, S1,
                                   S2, (double)tau_s, (double)tau, scratch, pre_errors));
     if (int r = check_launch("pre_err_kernel")) return r;

@@ -69,11 +69,14 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *keys, int N, bool global_
   }
 }
 
+// One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
+// sample's Event list padded to a multiple of kEventPad, the event indices, the count, and
+// the per-slot float32 exps the backward reads.
 __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
-    float tmax, int Pcap, Event *__restrict__ events, int *__restrict__ counts,
-    float *__restrict__ pre_exp_s, float *__restrict__ pre_exp, uint64_t *__restrict__ gkeys,
-    int gcap, int lds_cap) {
+    float tmax, int Pcap, Event *__restrict__ events, int *__restrict__ evidx,
+    int *__restrict__ counts, float *__restrict__ pre_exp_s, float *__restrict__ pre_exp,
+    uint64_t *__restrict__ gkeys, int gcap, int lds_cap) {
   extern __shared__ uint64_t lkeys[];
   __shared__ int s_count;
   __shared__ int s_fill;
@@ -107,99 +110,127 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
   if (P > 1) bitonic_sort(keys, N, use_global);
 
   Event *ev = events + (size_t)b * Pcap;
+  int *ix = evidx + (size_t)b * Pcap;
   const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
-  for (int q = threadIdx.x; q < P; q += blockDim.x) {
-    const uint64_t key = keys[q];
-    const float t = key_float((uint32_t)(key >> 32));
-    const int s = (int)(uint32_t)key;
-    const float tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
-    const float tl = fminf(tn, tmax);
-    const double es64 = exp((double)t / dtau_s);
-    const double zq = exp((double)t / dtau);
-    const double zl = exp((double)tl / dtau);
+  const int Ppad = (P + kEventPad - 1) / kEventPad * kEventPad;
+  for (int q = threadIdx.x; q < Ppad; q += blockDim.x) {
     Event e;
-    const float es32 = (float)es64, e32 = (float)zq;
-    e.es = (double)es32;
-    e.e = (double)e32;
-    e.idx = s / K_pre;
-    e.t = t;
-    e.zq = zq;
-    e.zl = zl;
-    e.czl2 = dc * zl * zl;
-    e.tl = tl;
+    if (q < P) {
+      const uint64_t key = keys[q];
+      const float t = key_float((uint32_t)(key >> 32));
+      const int s = (int)(uint32_t)key;
+      const float tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
+      const float tl = fminf(tn, tmax);
+      const double es64 = exp((double)t / dtau_s);
+      const double zq = exp((double)t / dtau);
+      const double zl = fmin(exp((double)tl / dtau), 1e300);
+      const float es32 = (float)es64, e32 = (float)zq;
+      e.es = (double)es32;
+      e.e = (double)e32;
+      e.czl2 = dc * zl * zl;
+      e.zl1 = zl * (1.0 + kFilterEps);
+      e.blo = 2.0 * dc * zq * (1.0 - kFilterEps);
+      e.bhi = 2.0 * dc * zl * (1.0 + kFilterEps);
+      e.zq = zq;
+      e.zl = zl;
+      e.idx = s / K_pre;
+      e.t = t;
+      e.tl = tl;
+      pre_exp_s[(size_t)b * S + s] = es32;
+      pre_exp[(size_t)b * S + s] = e32;
+    } else {
+      // padding: no contribution, never a candidate
+      e.es = 0.0;
+      e.e = 0.0;
+      e.czl2 = kInfD;
+      e.zl1 = 0.0;
+      e.blo = kInfD;
+      e.bhi = -kInfD;
+      e.zq = kInfD;
+      e.zl = kInfD;
+      e.idx = 0;
+      e.t = kInfF;
+      e.tl = kInfF;
+    }
     e.pad = 0.f;
     ev[q] = e;
-    pre_exp_s[(size_t)b * S + s] = es32;
-    pre_exp[(size_t)b * S + s] = e32;
+    ix[q] = e.idx;
   }
   if (threadIdx.x == 0) counts[b] = P;
 }
 
 // ------------------------------------------------------------------ solver, lane per neuron
-// A workgroup is one sample and 256 consecutive post neurons (one lane each). Events are
-// staged through LDS kChunk at a time (one coalesced copy per chunk, read back as wave-wide
-// broadcasts); the weights of a whole chunk are requested before the chunk is processed, so
-// kChunk loads per lane are in flight at once.
-constexpr int kSolveThreads = 256;
-constexpr int kChunk = 32;
+// A workgroup is one sample and up to 1024 consecutive post neurons (one lane each; all of
+// them when J <= 1024, so that every wave of the sample shares one CU's caches). Event data
+// is wave-uniform and read with scalar loads; the weights of the next kGroup events are
+// requested while the current group is processed. Per event: a and b accumulate in float64,
+// the branch-free crossing filter runs, and only lanes whose filter passes take the
+// out-of-line closed-form loop; the spike times of a lane are computed after its walk.
+constexpr int kSolveMaxThreads = 1024;
+constexpr int kGroup = kEventPad;
 
-__global__ __launch_bounds__(kSolveThreads) void lif_solve_seq_kernel(
-    const Event *__restrict__ events, const int *__restrict__ counts, int Pcap,
-    const float *__restrict__ WT, int J, int K, double c, double dtt, double tau,
-    int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
-    float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
-  __shared__ Event sev[kChunk];
-  __shared__ int s_n[kSolveThreads];
+__global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
+    const Event *__restrict__ events, const int *__restrict__ evidx,
+    const int *__restrict__ counts, int Pcap, const float *__restrict__ WT, int J, int K,
+    double c, double dtt, double tau, int always_exact, int *__restrict__ out_n,
+    float *__restrict__ out_t, float *__restrict__ out_a, float *__restrict__ out_x,
+    float *__restrict__ out_z) {
+  __shared__ int s_n[kSolveMaxThreads];
   const int b = blockIdx.y;
-  const int j0 = blockIdx.x * kSolveThreads;
+  const int j0 = blockIdx.x * blockDim.x;
   const int j = j0 + threadIdx.x;
   const bool active = j < J;
   const int jc = active ? j : 0;
   const int P = counts[b];
+  const int Ppad = (P + kGroup - 1) / kGroup * kGroup;
   const Event *ev = events + (size_t)b * Pcap;
+  const int *ix = evidx + (size_t)b * Pcap;
   const float *wcol = WT + jc;
-  const double c4 = 4.0 * c, inv2c = 0.5 / c;
+  const double c4 = 4.0 * c;
   const size_t row = ((size_t)b * J + jc) * K;
   double a = 0.0, bb = 0.0;
   int n = 0;
   bool live = active;
-  for (int q0 = 0; q0 < P; q0 += kChunk) {
-    const int cnt = min(kChunk, P - q0);
-    __syncthreads();  // the previous chunk has been consumed by every wave
-    {
-      const uint4 *src = reinterpret_cast<const uint4 *>(ev + q0);
-      uint4 *dst = reinterpret_cast<uint4 *>(sev);
-      constexpr int kVec = sizeof(Event) / sizeof(uint4);
-      for (int v = threadIdx.x; v < cnt * kVec; v += kSolveThreads) dst[v] = src[v];
+  float wn[kGroup];
+  if (Ppad > 0) {
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[u] * J];
+  }
+  for (int q0 = 0; q0 < Ppad; q0 += kGroup) {
+    float w[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) w[u] = wn[u];
+    if (q0 + kGroup < Ppad) {
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[q0 + kGroup + u] * J];
     }
-    __syncthreads();
-    float w[kChunk];
+    // event data one event ahead, so that its scalar loads are in flight during the
+    // previous event's arithmetic
+    Event cur = ev[q0];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
-      w[u] = (u < cnt) ? wcol[(size_t)sev[u].idx * J] : 0.f;
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      if (u < cnt) {
-        const double w64 = (double)w[u];
-        a = fma(w64, sev[u].es, a);
-        bb = fma(w64, sev[u].e, bb);
-        if (live && (always_exact || crossing_candidate(a, bb, sev[u], c4, inv2c))) {
-          const SolveState r = solve_crossings_ool(a, bb, n, (double)sev[u].t,
-                                                   (double)sev[u].tl, c, dtt, tau, K, row,
-                                                   out_t, out_a, out_x, out_z);
-          bb = r.b;
-          n = r.n;
-          live = r.live != 0;
-        }
+    for (int u = 0; u < kGroup; ++u) {
+      Event nxt;
+      if (u + 1 < kGroup) nxt = ev[q0 + u + 1];
+      const double w64 = (double)w[u];
+      a = fma(w64, cur.es, a);
+      bb = fma(w64, cur.e, bb);
+      if (live & (always_exact | crossing_filter(a, bb, cur, c4))) {
+        const SolveStateZ r = solve_crossings_deferred(a, bb, n, ev + q0 + u, c, dtt, tau, K,
+                                                       row, out_t, out_a, out_x, out_z);
+        bb = r.b;
+        n = r.n;
+        live = r.live != 0;
       }
+      if (u + 1 < kGroup) cur = nxt;
     }
   }
+  if (active) finish_spike_times(row, n, tau, out_t, out_z);
   s_n[threadIdx.x] = active ? n : 0;
   if (active) out_n[(size_t)b * J + j] = n;
   __syncthreads();
-  const int jcount = min(kSolveThreads, J - j0);
+  const int jcount = min((int)blockDim.x, J - j0);
   float *tb = out_t + ((size_t)b * J + j0) * K;
-  for (int e = threadIdx.x; e < jcount * K; e += kSolveThreads) {
+  for (int e = threadIdx.x; e < jcount * K; e += blockDim.x) {
     const int jj = e / K;
     if (e - jj * K >= s_n[jj]) tb[e] = kInfF;
   }
@@ -237,7 +268,7 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
   const Event *ev = events + (size_t)b * Pcap;
   const float *wrow = W + (size_t)j * I;
   const size_t row = ((size_t)b * J + j) * K;
-  const double c4 = 4.0 * c, inv2c = 0.5 / c;
+  const double c4 = 4.0 * c;
   double carry_a = 0.0, carry_b = 0.0;
   int n = 0;
   bool live = true;
@@ -254,7 +285,7 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
     }
     double A = wave_scan_incl(pa, lane) + carry_a;
     double Bv = wave_scan_incl(pb, lane) + carry_b;
-    bool cand = valid && (always_exact || crossing_candidate(A, Bv, e, c4, inv2c));
+    bool cand = valid & (always_exact | crossing_filter(A, Bv, e, c4));
     unsigned long long mask = __ballot(cand);
     while (mask) {
       const int f = __ffsll(mask) - 1;
@@ -271,7 +302,7 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
         const double delta = bin - bf;
         if (lane > f) Bv -= delta;
         if (lane == f) Bv = bf;
-        cand = valid && lane > f && (always_exact || crossing_candidate(A, Bv, e, c4, inv2c));
+        cand = valid & (lane > f) & (always_exact | crossing_filter(A, Bv, e, c4));
         mask = __ballot(cand);
       } else {
         mask &= ~(1ull << f);
@@ -291,6 +322,7 @@ static int g_solver_mode = 0;  // 0 auto, 1 lane per neuron, 2 wave per neuron
 // Lane-per-neuron keeps the event walk sequential in each lane and needs many (sample,
 // neuron) pairs to fill 256 CUs; below this many pairs the wave-per-neuron form is used.
 constexpr long long kSeqMinLanes = 32768;
+
 static int lds_capacity_entries() {
   static int cached[64];
   static std::once_flag once[64];
@@ -312,29 +344,8 @@ static int lds_capacity_entries() {
   return cached[dev];
 }
 
-struct FwdLayout {
-  size_t wt, events, counts, gkeys, total;
-  int Pcap, gcap;
-};
-
-static FwdLayout fwd_layout(int B, int I, int K_pre, int J, int lds_cap) {
-  FwdLayout L{};
-  const int S = I * K_pre;
-  L.Pcap = std::max(S, 1);
-  int N = 1;
-  while (N < S) N <<= 1;
-  L.gcap = (N > lds_cap) ? N : 0;
-  size_t off = 0;
-  L.wt = off;
-  off = align_up(off + sizeof(float) * (size_t)I * J, 256);
-  L.events = off;
-  off = align_up(off + sizeof(Event) * (size_t)B * L.Pcap, 256);
-  L.counts = off;
-  off = align_up(off + sizeof(int) * (size_t)B, 256);
-  L.gkeys = off;
-  off = align_up(off + sizeof(uint64_t) * (size_t)B * L.gcap, 256);
-  L.total = off;
-  return L;
+static int event_capacity(int S) {
+  return std::max(kEventPad, (S + kEventPad - 1) / kEventPad * kEventPad);
 }
 
 size_t event_lists_bytes(int B, int S) {
@@ -342,8 +353,10 @@ size_t event_lists_bytes(int B, int S) {
   int N = 1;
   while (N < S) N <<= 1;
   const int gcap = (N > cap) ? N : 0;
+  const int Pcap = event_capacity(S);
   size_t off = 0;
-  off = align_up(off + sizeof(Event) * (size_t)B * std::max(S, 1), 256);
+  off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
   off = align_up(off + sizeof(int) * (size_t)B, 256);
   off = align_up(off + sizeof(uint64_t) * (size_t)B * gcap, 256);
   return off;
@@ -356,11 +369,13 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   int N = 1;
   while (N < S) N <<= 1;
   const int gcap = (N > cap) ? N : 0;
-  const int Pcap = std::max(S, 1);
+  const int Pcap = event_capacity(S);
   char *base = static_cast<char *>(ws);
   size_t off = 0;
   out->events = reinterpret_cast<Event *>(base + off);
   off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  out->idx = reinterpret_cast<int *>(base + off);
+  off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
   out->counts = reinterpret_cast<int *>(base + off);
   off = align_up(off + sizeof(int) * (size_t)B, 256);
   uint64_t *gkeys = reinterpret_cast<uint64_t *>(base + off);
@@ -370,13 +385,17 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
     BATS_TIMED("sort_events_kernel", st,
                hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
                                   sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,
-                                  tau, c, tmax, Pcap, out->events, out->counts, pre_es, pre_e,
-                                  gkeys, gcap, lds_entries));
+                                  tau, c, tmax, Pcap, out->events, out->idx, out->counts, pre_es,
+                                  pre_e, gkeys, gcap, lds_entries));
     return check_launch("sort_events_kernel");
   }
   hipLaunchKernelGGL(fill_kernel<int>, dim3((B + 255) / 256), dim3(256), 0, st, out->counts,
                      (int64_t)B, 0);
   return check_launch("zero_counts");
+}
+
+static size_t fwd_bytes(int B, int I, int K_pre, int J) {
+  return align_up(sizeof(float) * (size_t)I * J, 256) + event_lists_bytes(B, I * K_pre);
 }
 
 }  // namespace bats
@@ -385,7 +404,7 @@ using namespace bats;
 
 extern "C" size_t bats_lif_forward_workspace_size(int B, int I, int K_pre, int J) {
   if (B < 0 || I < 0 || K_pre < 0 || J < 0) return 0;
-  return fwd_layout(B, I, K_pre, J, lds_capacity_entries()).total;
+  return fwd_bytes(B, I, K_pre, J);
 }
 
 extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
@@ -397,41 +416,28 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0, "negative size");
   BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
   BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
+  BATS_REQUIRE(B <= 65535, "batch above 65535");
   if (B == 0 || J == 0) return 0;
   BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
   BATS_REQUIRE(tau_s > 0.f && tau > 0.f, "tau_s and tau must be positive");
   const int S = I * K_pre;
   hipStream_t st = as_stream(stream);
-  const int cap = lds_capacity_entries();
-  const FwdLayout L = fwd_layout(B, I, K_pre, J, cap);
-  BATS_REQUIRE(workspace != nullptr || L.total == 0, "null workspace");
-  BATS_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+  const size_t need = fwd_bytes(B, I, K_pre, J);
+  BATS_REQUIRE(workspace != nullptr, "null workspace");
+  BATS_REQUIRE(workspace_bytes >= need, "workspace too small");
   char *ws = static_cast<char *>(workspace);
-  float *WT = reinterpret_cast<float *>(ws + L.wt);
-  Event *events = reinterpret_cast<Event *>(ws + L.events);
-  int *counts = reinterpret_cast<int *>(ws + L.counts);
-  uint64_t *gkeys = reinterpret_cast<uint64_t *>(ws + L.gkeys);
-
+  float *WT = reinterpret_cast<float *>(ws);
+  void *ev_ws = ws + align_up(sizeof(float) * (size_t)I * J, 256);
   if (S > 0) {
     BATS_REQUIRE(pre_times && weights && pre_exp_tau_s && pre_exp_tau, "null input pointer");
     dim3 tg((I + 31) / 32, (J + 31) / 32);
     hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I);
     if (int r = check_launch("transpose_kernel")) return r;
-    int N = 1;
-    while (N < S) N <<= 1;
-    const int lds_entries = std::min(N, cap);
-    BATS_TIMED("sort_events_kernel", st,
-               hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
-                                  sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,
-                                  tau, c, max_simulation, L.Pcap, events, counts, pre_exp_tau_s,
-                                  pre_exp_tau, gkeys, L.gcap, lds_entries));
-    if (int r = check_launch("sort_events_kernel")) return r;
-  } else {
-    // no input neurons: every sample has zero events
-    hipLaunchKernelGGL(fill_kernel<int>, dim3((B + 255) / 256), dim3(256), 0, st, counts,
-                       (int64_t)B, 0);
-    if (int r = check_launch("zero_counts")) return r;
   }
+  EventLists el;
+  if (int r = build_event_lists(st, B, S, K_pre, pre_times, tau_s, tau, c, max_simulation,
+                                pre_exp_tau_s, pre_exp_tau, ev_ws, &el))
+    return r;
   const int always_exact = (c > 0.f) ? 0 : 1;
   const long long BJ = (long long)B * J;
   const bool wave_mode = g_solver_mode == 2 || (g_solver_mode == 0 && BJ < kSeqMinLanes);
@@ -440,15 +446,19 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
     const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));
     BATS_TIMED("lif_solve_wave_kernel", st,
                hipLaunchKernelGGL(lif_solve_wave_kernel, dim3(blocks), dim3(kWaveThreads), 0, st,
-                                  events, counts, L.Pcap, weights, I, J, (int)BJ, K_post,
+                                  el.events, el.counts, el.Pcap, weights, I, J, (int)BJ, K_post,
                                   (double)c, (double)delta_theta_tau, (double)tau, always_exact,
                                   out_n, out_t, out_a, out_x, out_z));
     return check_launch("lif_solve_wave_kernel");
   }
-  dim3 sg((J + kSolveThreads - 1) / kSolveThreads, B);
+  // blocks per sample, and lanes per block rounded up to whole waves
+  const int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
+  const int per = (J + nblk - 1) / nblk;
+  const int threads = (per + 63) / 64 * 64;
+  dim3 sg((J + threads - 1) / threads, B);
   BATS_TIMED("lif_solve_seq_kernel", st,
-             hipLaunchKernelGGL(lif_solve_seq_kernel, sg, dim3(kSolveThreads), 0, st, events,
-                                counts, L.Pcap, WT, J, K_post, (double)c,
+             hipLaunchKernelGGL(lif_solve_seq_kernel, sg, dim3(threads), 0, st, el.events,
+                                el.idx, el.counts, el.Pcap, WT, J, K_post, (double)c,
                                 (double)delta_theta_tau, (double)tau, always_exact, out_n, out_t,
                                 out_a, out_x, out_z));
   return check_launch("lif_solve_seq_kernel");

@@ -21,29 +21,43 @@ __global__ __launch_bounds__(kThreads) void count_loss_kernel(
     int B, int O, int K, const int *__restrict__ n, const int64_t *__restrict__ labels,
     const float *__restrict__ targets, float target_true, float target_false,
     float *__restrict__ loss, float *__restrict__ errors, int64_t *__restrict__ pred) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const int64_t lab = labels ? labels[b] : -1;
-  double sq = 0.0;
-  int best = 0;
-  int bestn = 0;
-  for (int j = 0; j < O; ++j) {
-    const int nj = n[(size_t)b * O + j];
-    const float tg = targets ? targets[(size_t)b * O + j] : (j == lab ? target_true : target_false);
-    const double d = (double)tg - (double)(float)nj;
-    sq += d * d;
-    if (errors) {
-      const float d32 = (float)d;
-      float *er = errors + ((size_t)b * O + j) * K;
-      for (int k = 0; k < K; ++k) er[k] = d32;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  // per-sample loss and prediction
+  if (gid < B && (loss || pred)) {
+    const int b = gid;
+    const int64_t lab = labels ? labels[b] : -1;
+    double sq = 0.0;
+    int best = 0, bestn = 0;
+    for (int j = 0; j < O; ++j) {
+      const int nj = n[(size_t)b * O + j];
+      const float tg = targets ? targets[(size_t)b * O + j] : (j == lab ? target_true : target_false);
+      const double d = (double)tg - (double)(float)nj;
+      sq += d * d;
+      if (j == 0 || nj > bestn) {
+        bestn = nj;
+        best = j;
+      }
     }
-    if (j == 0 || nj > bestn) {
-      bestn = nj;
-      best = j;
+    if (loss) loss[b] = (float)(sq / 2.0);
+    if (pred) pred[b] = best;
+  }
+  // errors, one coalesced element per thread
+  if (errors) {
+    const int64_t total = (int64_t)B * O * K;
+    for (int64_t e = gid; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t bj = e / K;
+      const int b = (int)(bj / O), j = (int)(bj - (int64_t)b * O);
+      const int nj = n[bj];
+      float tg;
+      if (targets) {
+        tg = targets[bj];
+      } else {
+        const int64_t lab = labels[b];
+        tg = (j == lab) ? target_true : target_false;
+      }
+      errors[e] = (float)((double)tg - (double)(float)nj);
     }
   }
-  if (loss) loss[b] = (float)(sq / 2.0);
-  if (pred) pred[b] = best;
 }
 
 // ---------------------------------------------------------------- TTFS softmax CE
@@ -190,9 +204,10 @@ extern "C" int bats_count_loss(void *stream, int B, int O, int K, const int *n,
   BATS_REQUIRE(B >= 0 && O >= 1 && K >= 0, "bad sizes");
   if (B == 0) return 0;
   BATS_REQUIRE(n && labels, "null n or labels");
-  hipLaunchKernelGGL(count_loss_kernel, dim3(grid_for(B)), dim3(kThreads), 0, as_stream(stream),
-                     B, O, K, n, labels, (const float *)nullptr, target_true, target_false, loss,
-                     errors, pred);
+  const int64_t work = std::max<int64_t>(B, errors ? (int64_t)B * O * K : 0);
+  hipLaunchKernelGGL(count_loss_kernel, dim3(std::min<unsigned>(grid_for(work), 2048)),
+                     dim3(kThreads), 0, as_stream(stream), B, O, K, n, labels,
+                     (const float *)nullptr, target_true, target_false, loss, errors, pred);
   return check_launch("count_loss_kernel");
 }
 
@@ -202,8 +217,10 @@ extern "C" int bats_count_loss_targets(void *stream, int B, int O, int K, const 
   BATS_REQUIRE(B >= 0 && O >= 1 && K >= 0, "bad sizes");
   if (B == 0) return 0;
   BATS_REQUIRE(n && targets, "null n or targets");
-  hipLaunchKernelGGL(count_loss_kernel, dim3(grid_for(B)), dim3(kThreads), 0, as_stream(stream),
-                     B, O, K, n, (const int64_t *)nullptr, targets, 0.f, 0.f, loss, errors, pred);
+  const int64_t work = std::max<int64_t>(B, errors ? (int64_t)B * O * K : 0);
+  hipLaunchKernelGGL(count_loss_kernel, dim3(std::min<unsigned>(grid_for(work), 2048)),
+                     dim3(kThreads), 0, as_stream(stream), B, O, K, n, (const int64_t *)nullptr,
+                     targets, 0.f, 0.f, loss, errors, pred);
   return check_launch("count_loss_kernel");
 }
 

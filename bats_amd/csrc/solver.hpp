@@ -10,18 +10,11 @@ namespace bats {
 // z = 2a/(b + sqrt(b^2 - 4ac)) to be accepted in (t_q, min(t_next, t_max)]:
 // a > 0 and b > 0, and either g(zl) = c zl^2 - b zl + a <= 0 (one root inside) or the
 // vertex b/(2c) of g lies in [zq, zl] with a non-negative discriminant (two roots inside).
-constexpr double kFilterEps = 1e-10;
-
-__device__ __forceinline__ bool crossing_candidate(double a, double b, const Event &e,
-                                                   double c4, double inv2c) {
-  if (!(a > 0.0) || !(b > 0.0)) return false;
-  const double bb_hi = b * b * (1.0 + kFilterEps);
-  if (isinf(e.zl)) return bb_hi >= c4 * a;
-  const double g = (a + e.czl2) - b * e.zl * (1.0 + kFilterEps);
-  if (g <= 0.0) return true;
-  const double v = b * inv2c;
-  if (v < e.zq * (1.0 - kFilterEps) || v > e.zl * (1.0 + kFilterEps)) return false;
-  return bb_hi >= c4 * a;
+// Evaluated without branches (one divergent branch per event is left to the caller).
+__device__ __forceinline__ bool crossing_filter(double a, double b, const Event &e, double c4) {
+  const double g = fma(-b, e.zl1, a + e.czl2);
+  const double dd = fma(b, b * (1.0 + kFilterEps), -(c4 * a));
+  return (a > 0.0) & (b > 0.0) & ((g <= 0.0) | ((b >= e.blo) & (b <= e.bhi) & (dd >= 0.0)));
 }
 
 // The reference's compute_spikes loop (compute_spike_times.cu:53-83) in float64.
@@ -74,18 +67,121 @@ static __device__ __attribute__((noinline)) SolveState solve_crossings_ool(
 }
 
 
-// The filter with the interval given explicitly: lower end zq = exp(t_last/tau), upper end
-// zl = exp(min(t_next, t_max)/tau) and czl2 = c*zl*zl.
-__device__ __forceinline__ bool crossing_candidate3(double a, double b, double zq, double zl,
-                                                    double czl2, double c4, double inv2c) {
-  if (!(a > 0.0) || !(b > 0.0)) return false;
-  const double bb_hi = b * b * (1.0 + kFilterEps);
-  if (isinf(zl)) return bb_hi >= c4 * a;
-  const double g = (a + czl2) - b * zl * (1.0 + kFilterEps);
-  if (g <= 0.0) return true;
-  const double v = b * inv2c;
-  if (v < zq * (1.0 - kFilterEps) || v > zl * (1.0 + kFilterEps)) return false;
-  return bb_hi >= c4 * a;
+// ------------------------------------------------------------------ deferred-log form
+// The closed-form loop of solve_crossings with the same accept/reject decisions, made in
+// z = exp(t/tau) space: t = tau*log(z) is monotone in z, so a root z clearly above the lower
+// end z_lo (clearly below the upper end z_hi) is above last (below tl) in t as well. "Clearly"
+// is a relative band kZBand around the end; a root inside the band is decided with the
+// float64 logarithm exactly as solve_crossings does (the band, 1e-12, is far above the few
+// ulps by which tau*log(z) and the precomputed ends can round for any t < 700 tau).
+//
+// An accepted spike's a and x are stored at once; its time is left for
+// finish_spike_times: the float64 z is kept across the float32 slots of t and z
+// (high word in t, low word in z), so the logarithm of all spikes of a lane runs once,
+// after its event walk, instead of once per candidate interval inside it.
+constexpr double kZBand = 1e-12;
+
+struct SolveStateZ {
+  double b;
+  int n;
+  int live;
+};
+
+__device__ __forceinline__ void stash_z(size_t slot, double z, float *__restrict__ out_t,
+                                        float *__restrict__ out_z) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(z);
+  out_t[slot] = __uint_as_float((unsigned)(bits >> 32));
+  out_z[slot] = __uint_as_float((unsigned)bits);
+}
+
+__device__ __forceinline__ double tau_log(double tau, double z) { return tau * log(z); }
+
+// The interval is (t_lo, t_hi] with z_lo = exp(t_lo/tau), z_hi = exp(t_hi/tau); t_lo is an
+// input event time, t_hi the next event time or t_max. The ends are read from the event
+// record here, so that the caller's event loop keeps only the filter's fields live.
+// Inlined: an out-of-line call would wait for every vector load in flight (the caller's
+// weight prefetch) at its entry and return.
+__device__ __forceinline__ SolveStateZ solve_crossings_deferred(
+    double a, double b, int n, const Event *ev, double c, double dtt, double tau, int K,
+    size_t row, float *out_t, float *out_a, float *out_x, float *out_z) {
+  const float t_lo = ev->t, t_hi = ev->tl;
+  const double z_lo = ev->zq, z_hi = ev->zl;
+  SolveStateZ r;
+  r.live = 1;
+  double zlast = z_lo;
+  bool last_is_event = true;
+  const bool hi_finite = isfinite(t_hi);
+  while (true) {
+    const double disc = b * b - (4.0 * a) * c;
+    if (disc < 0.0) break;
+    const double x = sqrt(disc);
+    const double den = x + b;
+    if (den == 0.0) break;
+    const double z = (2.0 * a) / den;
+    if (z < 0.0) break;
+    // t <= last  -> reject
+    bool above;
+    if (z > zlast * (1.0 + kZBand)) {
+      above = true;
+    } else if (z < zlast * (1.0 - kZBand)) {
+      above = false;
+    } else {
+      const double last = last_is_event ? (double)t_lo : tau_log(tau, zlast);
+      above = tau_log(tau, z) > last;
+    }
+    if (!above) break;
+    // t > tl  -> reject
+    bool below;
+    if (z < z_hi * (1.0 - kZBand)) {
+      below = true;
+    } else if (hi_finite && z > z_hi * (1.0 + kZBand)) {
+      below = false;
+    } else {
+      below = !(tau_log(tau, z) > (double)t_hi);
+    }
+    if (!below) break;
+    const size_t slot = row + n;
+    out_a[slot] = (float)a;
+    out_x[slot] = (float)x;
+    stash_z(slot, z, out_t, out_z);
+    zlast = z;
+    last_is_event = false;
+    b = b - dtt * z;
+    ++n;
+    if (n >= K) {
+      r.live = 0;
+      break;
+    }
+  }
+  r.b = b;
+  r.n = n;
+  return r;
+}
+
+// Spike times of one lane's n stashed spikes: t = tau*log(z) and z rounded to float32, as
+// solve_crossings stores them.
+__device__ __forceinline__ void finish_spike_times(size_t row, int n, double tau,
+                                                   float *__restrict__ out_t,
+                                                   float *__restrict__ out_z) {
+  for (int k = 0; k < n; ++k) {
+    const unsigned hi = __float_as_uint(out_t[row + k]);
+    const unsigned lo = __float_as_uint(out_z[row + k]);
+    const double z = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    out_t[row + k] = (float)(tau * log(z));
+    out_z[row + k] = (float)z;
+  }
+}
+
+// The same filter with the interval given explicitly: lower end zq = exp(t_last/tau), upper
+// end zl = exp(min(t_next, t_max)/tau) (any value above 1e300 stands for "no upper end").
+__device__ __forceinline__ bool crossing_filter3(double a, double b, double zq, double zl,
+                                                 double c, double c4) {
+  zl = fmin(zl, 1e300);
+  const double zl1 = zl * (1.0 + kFilterEps);
+  const double g = fma(-b, zl1, a + c * zl * zl);
+  const double dd = fma(b, b * (1.0 + kFilterEps), -(c4 * a));
+  const double blo = 2.0 * c * zq * (1.0 - kFilterEps), bhi = 2.0 * c * zl1;
+  return (a > 0.0) & (b > 0.0) & ((g <= 0.0) | ((b >= blo) & (b <= bhi) & (dd >= 0.0)));
 }
 
 }  // namespace bats

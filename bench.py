@@ -198,6 +198,59 @@ def plain_bench(args):
     return dict(ms=ms, counts=counts)
 
 
+def serve_bench(args):
+    """Inference serving of the same network (weights as trained-from, inputs of the same
+    distribution): one request = one latency-coded sample as an event list. Measures the
+    saturated throughput (all requests queued at once), then p50/p99 latency (arrival ->
+    result on the host) under Poisson arrivals at 25%, 50% and 90% of it."""
+    import torch
+    from bats_amd.serving import InferenceServer, LayerSpec, Request, run_poisson
+
+    _, _, _, w1, w2 = make_inputs(args.seed, 8, args.variant)
+    layers = [LayerSpec(w1, TAU_S, THETA_H, THETA_H, K_HID),
+              LayerSpec(w2, TAU_S, THETA_O, THETA_O, K_OUT)]
+    srv = InferenceServer(layers, N_IN, max_batch=args.serve_batch, max_simulation=T_MAX)
+    rng = np.random.default_rng(args.seed + 1)
+
+    def requests(count):
+        out = []
+        for i in range(count):
+            x = rng.uniform(0.0, 1.0, size=N_IN)
+            idx = np.nonzero(x > 0.1)[0].astype(np.int32)
+            tm = (1.0 - x[idx]).astype(np.float32)
+            order = np.argsort(tm, kind="stable")
+            out.append(Request(rid=i, indices=idx[order], times=tm[order], arrival=0.0))
+        return out
+
+    Bb = 8
+    shapes = []
+    while Bb <= args.serve_batch:
+        shapes.append((Bb, 1024, 1))
+        Bb *= 2
+    srv.warmup(shapes)
+    # saturated throughput
+    reqs = requests(args.serve_requests)
+    srv.completed = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        r.arrival = t0
+        srv.submit(r)
+    srv.drain()
+    torch.cuda.synchronize()
+    sat = len(srv.completed) / (time.perf_counter() - t0)
+    loads = []
+    for frac in (0.25, 0.5, 0.9):
+        rate = frac * sat
+        n_req = int(min(args.serve_requests, max(200, rate * args.serve_seconds)))
+        res = run_poisson(srv, requests(n_req), rate, seed=args.seed)
+        loads.append({"load": frac, "offered_per_s": round(rate, 1),
+                      "served_per_s": round(res["samples_per_s"], 1),
+                      "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+                      "requests": res["n"]})
+    return {"saturated_samples_per_s": round(sat, 1), "max_batch": args.serve_batch,
+            "loads": loads}
+
+
 def first_forward_counts(args):
     """Spike counts of the optimised path's first forward on the bench inputs."""
     import torch
@@ -253,6 +306,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=20000)
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -287,6 +344,8 @@ def main():
                  "hidden_count_agreement": float(np.mean(opt_counts[0] == pr["counts"][0])),
                  "output_count_agreement": float(np.mean(opt_counts[1] == pr["counts"][1]))}
 
+    serve = None if args.skip_serve else serve_bench(args)
+
     cpu = None
     if not args.skip_cpu:
         c = cpu_baseline(args)
@@ -304,7 +363,7 @@ def main():
         "config": {"workload": "784-800-10 LIF train step, K=30/30, count loss 15/3, Adam, "
                                f"t_max=1.0, W~{'N(0,1) active' if args.variant == 'active' else 'N(0,1/sqrt(fan_in)) literal'} variant",
                    "global_batch": B, "seed": args.seed, "graph": True},
-        "roofline": roof, "cpu_baseline": cpu, "plain_form": plain,
+        "roofline": roof, "cpu_baseline": cpu, "plain_form": plain, "serve": serve,
         "eager_ms_per_step": round(r["eager_ms"], 4),
         "step_hbm_fraction": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
         "spikes_per_sample": {"hidden": round(r["hid_spikes"], 1), "output": round(r["out_spikes"], 2)},

@@ -1,7 +1,13 @@
 """torch.autograd integration: LIFFunction gradients against the FP64 oracle (batch sums,
 1e-4 relative + 1e-6 max-relative) and against central finite differences of the GPU forward
-(h = 1e-4 relative to |w| + 0.1 on 64 random weights; the forward returns float32 times, so the
-finite differences carry ~1e-3 relative noise and are held to 2e-2)."""
+(h = 1e-4 (|w| + 0.1) on 64 random weights).
+
+The forward returns float32 spike times, so a central difference carries the rounding of the
+perturbed row's times divided by 2h: the test allows 2e-2 |g| plus that rounding bound,
+sum_k |dL/dt_k| ulp(t_k) / 2h (on the oracle the observed error stays below 0.4 of it, and
+with float64 storage the same differences agree with the analytic gradient to 1e-7). Stencils
+whose spikes move across a pre-spike time are skipped: the spike time is only piecewise smooth
+in the weights, and there the central difference averages two one-sided slopes."""
 import numpy as np
 import pytest
 import torch
@@ -49,6 +55,24 @@ def test_autograd_matches_oracle(recompute):
     assert_close_rel(gp[fin], p[fin], 1e-4, 1e-6, "dL/dt_pre")
 
 
+def _straddles_breakpoint(pre_t, row0, rowp, rowm, eps=1e-6):
+    """A post spike time is a smooth function of the weights only between pre-spike times:
+    when it moves across one (or sits on one), the slope of the membrane jumps there and
+    the central difference averages the two one-sided derivatives. True when any spike of
+    the perturbed row moves over an interval that holds a pre-spike time of its sample
+    (widened by eps)."""
+    for b in range(pre_t.shape[0]):
+        pre = pre_t[b][np.isfinite(pre_t[b])]
+        for k in range(row0.shape[1]):
+            if not np.isfinite(row0[b, k]):
+                continue
+            v = (row0[b, k], rowp[b, k], rowm[b, k])
+            lo, hi = min(v) - eps, max(v) + eps
+            if np.any((pre >= lo) & (pre <= hi)):
+                return True
+    return False
+
+
 def test_autograd_finite_differences():
     from bats_amd.autograd import lif
     t, W, e = _inputs(1)
@@ -59,27 +83,37 @@ def test_autograd_finite_differences():
         ot, on = lif(pt, torch.as_tensor(Wnp, device="cuda"), **PARAMS)
         val = torch.where(torch.isfinite(ot), ot.double() * ed.double(),
                           torch.zeros_like(ot, dtype=torch.float64)).sum()
-        return float(val.item()), on.cpu().numpy()
+        return float(val.item()), on.cpu().numpy(), ot.cpu().numpy()
 
     Wd = torch.as_tensor(W, device="cuda").requires_grad_(True)
     ot, on = lif(pt, Wd, **PARAMS)
+    t0 = ot.detach().cpu().numpy()
     torch.where(torch.isfinite(ot), ot * ed, torch.zeros_like(ot)).sum().backward()
     g = Wd.grad.cpu().numpy()
     n0 = on.cpu().numpy()
     rng = np.random.default_rng(2)
     picks = [(rng.integers(W.shape[0]), rng.integers(W.shape[1])) for _ in range(64)]
     checked = 0
-    scale = np.abs(g).max()
+    skipped_kinks = 0
     for j, i in picks:
         h = np.float32(1e-4 * (abs(W[j, i]) + 0.1))
         Wp, Wm = W.copy(), W.copy()
         Wp[j, i] += h
         Wm[j, i] -= h
-        lp, np_ = L(Wp)
-        lm, nm = L(Wm)
+        lp, np_, tp_ = L(Wp)
+        lm, nm, tm_ = L(Wm)
         if not (np.array_equal(np_, n0) and np.array_equal(nm, n0)):
             continue
-        fd = (lp - lm) / (float(Wp[j, i]) - float(Wm[j, i]))
-        assert abs(fd - g[j, i]) <= 2e-2 * max(abs(g[j, i]), 0.05 * scale), (j, i, fd, g[j, i])
+        if _straddles_breakpoint(t, t0[:, j], tp_[:, j], tm_[:, j]):
+            skipped_kinks += 1
+            continue
+        dw = float(Wp[j, i]) - float(Wm[j, i])
+        fd = (lp - lm) / dw
+        row, ej = t0[:, j], e[:, j]
+        fin = np.isfinite(row)
+        noise = float(np.sum(np.abs(ej[fin]).astype(np.float64) *
+                             np.spacing(np.abs(row[fin])).astype(np.float64))) / dw
+        assert abs(fd - g[j, i]) <= 2e-2 * abs(g[j, i]) + noise, (j, i, fd, g[j, i], noise)
         checked += 1
+    print(f"[fd] checked {checked}, skipped at breakpoints {skipped_kinks}")
     assert checked >= 40

@@ -84,8 +84,19 @@ constexpr double kFilterEps = 1e-10;
 constexpr int kEventPad = 8;
 
 // Per-sample time-ordered event lists built by sort_events_kernel (lif_forward.hip).
+// The fields of an event the lane-per-neuron solver reads for every event: the two exps and
+// the interval ends, widened by kFilterEps (the exact ends are in the Event record).
+struct alignas(32) FastEvent {
+  double es;
+  double e;
+  double zq_lo;  // exp(t/tau) * (1 - kFilterEps)
+  double zl_hi;  // exp(tl/tau) * (1 + kFilterEps)
+};
+static_assert(sizeof(FastEvent) == 32, "FastEvent layout");
+
 struct EventLists {
   Event *events;  // [B, Pcap]
+  FastEvent *fast;  // [B, Pcap]
   int *idx;       // [B, Pcap] pre-neuron index of each event (also in Event), for wide loads
   int *counts;    // [B] real events per sample (padding excluded)
   int Pcap;       // multiple of kEventPad

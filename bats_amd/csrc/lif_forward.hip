@@ -74,7 +74,8 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *keys, int N, bool global_
 // the per-slot float32 exps the backward reads.
 __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
-    float tmax, int Pcap, Event *__restrict__ events, int *__restrict__ evidx,
+    float tmax, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
+    int *__restrict__ evidx,
     int *__restrict__ counts, float *__restrict__ pre_exp_s, float *__restrict__ pre_exp,
     uint64_t *__restrict__ gkeys, int gcap, int lds_cap) {
   extern __shared__ uint64_t lkeys[];
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
   if (P > 1) bitonic_sort(keys, N, use_global);
 
   Event *ev = events + (size_t)b * Pcap;
+  FastEvent *fv = fast + (size_t)b * Pcap;
   int *ix = evidx + (size_t)b * Pcap;
   const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
   const int Ppad = (P + kEventPad - 1) / kEventPad * kEventPad;
@@ -154,6 +156,12 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     }
     e.pad = 0.f;
     ev[q] = e;
+    FastEvent f;
+    f.es = e.es;
+    f.e = e.e;
+    f.zq_lo = (q < P) ? e.zq * (1.0 - kFilterEps) : kInfD;  // padding: never a candidate
+    f.zl_hi = (q < P) ? e.zl * (1.0 + kFilterEps) : kInfD;
+    fv[q] = f;
     ix[q] = e.idx;
   }
   if (threadIdx.x == 0) counts[b] = P;
@@ -161,18 +169,21 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
 
 // ------------------------------------------------------------------ solver, lane per neuron
 // A workgroup is one sample and up to 1024 consecutive post neurons (one lane each; all of
-// them when J <= 1024, so that every wave of the sample shares one CU's caches). Event data
-// is wave-uniform and read with scalar loads; the weights of the next kGroup events are
-// requested while the current group is processed. Per event: a and b accumulate in float64,
-// the branch-free crossing filter runs, and only lanes whose filter passes take the
-// out-of-line closed-form loop; the spike times of a lane are computed after its walk.
+// them when J <= 1024, so that every wave of the sample shares one CU's caches). The 32-byte
+// FastEvent records are wave-uniform and read with scalar loads; the weights of the next
+// kGroup events are requested while the current group is processed. Per event: a and b
+// accumulate in float64, the clamp filter runs, and only lanes whose filter passes enter the
+// closed-form loop; the spike times of a lane are computed after its walk.
 constexpr int kSolveMaxThreads = 1024;
 constexpr int kGroup = kEventPad;
 
+// kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every interval is solved.
+template <bool kAlwaysExact>
 __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
-    const Event *__restrict__ events, const int *__restrict__ evidx,
+    const Event *__restrict__ events, const FastEvent *__restrict__ fast,
+    const int *__restrict__ evidx,
     const int *__restrict__ counts, int Pcap, const float *__restrict__ WT, int J, int K,
-    double c, double dtt, double tau, int always_exact, int *__restrict__ out_n,
+    double c, double dtt, double tau, int *__restrict__ out_n,
     float *__restrict__ out_t, float *__restrict__ out_a, float *__restrict__ out_x,
     float *__restrict__ out_z) {
   __shared__ int s_n[kSolveMaxThreads];
@@ -184,11 +195,14 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
   const int P = counts[b];
   const int Ppad = (P + kGroup - 1) / kGroup * kGroup;
   const Event *ev = events + (size_t)b * Pcap;
+  const FastEvent *fv = fast + (size_t)b * Pcap;
   const int *ix = evidx + (size_t)b * Pcap;
   const float *wcol = WT + jc;
-  const double c4 = 4.0 * c;
+  const double inv2c = 0.5 / c;
   const size_t row = ((size_t)b * J + jc) * K;
-  double a = 0.0, bb = 0.0;
+  // With the filter, a lane that is inactive or whose buffer is full carries a = -inf (it
+  // stays -inf and fails the filter's a > 0); in the always-exact form a flag does this.
+  double a = (kAlwaysExact || active) ? 0.0 : -kInfD, bb = 0.0;
   int n = 0;
   bool live = active;
   float wn[kGroup];
@@ -206,20 +220,24 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
     }
     // event data one event ahead, so that its scalar loads are in flight during the
     // previous event's arithmetic
-    Event cur = ev[q0];
+    FastEvent cur = fv[q0];
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
-      Event nxt;
-      if (u + 1 < kGroup) nxt = ev[q0 + u + 1];
+      FastEvent nxt;
+      if (u + 1 < kGroup) nxt = fv[q0 + u + 1];
       const double w64 = (double)w[u];
       a = fma(w64, cur.es, a);
       bb = fma(w64, cur.e, bb);
-      if (live & (always_exact | crossing_filter(a, bb, cur, c4))) {
-        const SolveStateZ r = solve_crossings_deferred(a, bb, n, ev + q0 + u, c, dtt, tau, K,
-                                                       row, out_t, out_a, out_x, out_z);
+      if (kAlwaysExact ? live : clamp_filter(a, bb, cur, c, inv2c)) {
+        const SolveStateZ r = solve_crossings_deferred(a, bb, n, cur.zq_lo, cur.zl_hi,
+                                                       ev + q0 + u, c, inv2c, dtt, tau, K, row,
+                                                       out_t, out_a, out_x, out_z);
         bb = r.b;
         n = r.n;
-        live = r.live != 0;
+        if (kAlwaysExact)
+          live = r.live != 0;
+        else if (!r.live)
+          a = -kInfD;
       }
       if (u + 1 < kGroup) cur = nxt;
     }
@@ -356,6 +374,7 @@ size_t event_lists_bytes(int B, int S) {
   const int Pcap = event_capacity(S);
   size_t off = 0;
   off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  off = align_up(off + sizeof(FastEvent) * (size_t)B * Pcap, 256);
   off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
   off = align_up(off + sizeof(int) * (size_t)B, 256);
   off = align_up(off + sizeof(uint64_t) * (size_t)B * gcap, 256);
@@ -374,6 +393,8 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   size_t off = 0;
   out->events = reinterpret_cast<Event *>(base + off);
   off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  out->fast = reinterpret_cast<FastEvent *>(base + off);
+  off = align_up(off + sizeof(FastEvent) * (size_t)B * Pcap, 256);
   out->idx = reinterpret_cast<int *>(base + off);
   off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
   out->counts = reinterpret_cast<int *>(base + off);
@@ -385,7 +406,7 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
     BATS_TIMED("sort_events_kernel", st,
                hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
                                   sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,
-                                  tau, c, tmax, Pcap, out->events, out->idx, out->counts, pre_es,
+                                  tau, c, tmax, Pcap, out->events, out->fast, out->idx, out->counts, pre_es,
                                   pre_e, gkeys, gcap, lds_entries));
     return check_launch("sort_events_kernel");
   }
@@ -456,11 +477,12 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   const int per = (J + nblk - 1) / nblk;
   const int threads = (per + 63) / 64 * 64;
   dim3 sg((J + threads - 1) / threads, B);
+  const auto seq = always_exact ? lif_solve_seq_kernel<true> : lif_solve_seq_kernel<false>;
   BATS_TIMED("lif_solve_seq_kernel", st,
-             hipLaunchKernelGGL(lif_solve_seq_kernel, sg, dim3(threads), 0, st, el.events,
-                                el.idx, el.counts, el.Pcap, WT, J, K_post, (double)c,
-                                (double)delta_theta_tau, (double)tau, always_exact, out_n, out_t,
-                                out_a, out_x, out_z));
+             hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
+                                el.counts, el.Pcap, WT, J, K_post, (double)c,
+                                (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
+                                out_z));
   return check_launch("lif_solve_seq_kernel");
 }
 

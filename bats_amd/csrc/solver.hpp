@@ -17,6 +17,28 @@ __device__ __forceinline__ bool crossing_filter(double a, double b, const Event 
   return (a > 0.0) & (b > 0.0) & ((g <= 0.0) | ((b >= e.blo) & (b <= e.bhi) & (dd >= 0.0)));
 }
 
+// The same necessary condition from the minimum of the convex quadratic: with a > 0 and
+// c > 0 the accepted (smaller) root lies in [zq, zl] only if g(z) = c z^2 - b z + a is <= 0
+// somewhere there, i.e. at the vertex b/(2c) clamped to the interval. The interval is widened
+// by kFilterEps and g may exceed 0 by kFilterEps * a, far more than the rounding of g near a
+// root (a few ulps of 2a there) or of the computed root. A larger root at the interval start
+// also passes; the exact loop rejects it.
+// Clamp of a lane value to wave-uniform bounds held in SGPRs. Written as the two
+// instructions: fmin/fmax would first re-quiet the loaded bounds (two more float64 ops per
+// event) to honour signalling NaNs, which the bounds never are (finite or +inf).
+__device__ __forceinline__ double clamp_uniform(double x, double lo, double hi) {
+  double r;
+  asm("v_max_f64 %0, %1, %2\n\tv_min_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "s"(lo), "s"(hi));
+  return r;
+}
+
+__device__ __forceinline__ bool clamp_filter(double a, double b, const FastEvent &f, double c,
+                                             double inv2c) {
+  const double zv = clamp_uniform(b * inv2c, f.zq_lo, f.zl_hi);
+  const double g = fma(fma(c, zv, -b), zv, a);
+  return (a > 0.0) & (g <= kFilterEps * a);
+}
+
 // The reference's compute_spikes loop (compute_spike_times.cu:53-83) in float64.
 // Returns false when the neuron's buffer became full (its event loop ends).
 __device__ __forceinline__ bool solve_crossings(double a, double &b, int &n, double last,
@@ -97,20 +119,22 @@ __device__ __forceinline__ void stash_z(size_t slot, double z, float *__restrict
 __device__ __forceinline__ double tau_log(double tau, double z) { return tau * log(z); }
 
 // The interval is (t_lo, t_hi] with z_lo = exp(t_lo/tau), z_hi = exp(t_hi/tau); t_lo is an
-// input event time, t_hi the next event time or t_max. The ends are read from the event
-// record here, so that the caller's event loop keeps only the filter's fields live.
-// Inlined: an out-of-line call would wait for every vector load in flight (the caller's
-// weight prefetch) at its entry and return.
+// input event time, t_hi the next event time or t_max. The ends come in widened by
+// kFilterEps (as the filter uses them) and are narrowed back here to within a few ulps,
+// far inside kZBand; the event record is read only for a root inside the band. After an
+// accepted spike the clamp filter is applied to (z, t_hi] with the reset b, so that the
+// closed form runs again only when another root can be accepted.
 __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
-    double a, double b, int n, const Event *ev, double c, double dtt, double tau, int K,
-    size_t row, float *out_t, float *out_a, float *out_x, float *out_z) {
-  const float t_lo = ev->t, t_hi = ev->tl;
-  const double z_lo = ev->zq, z_hi = ev->zl;
+    double a, double b, int n, double zq_lo, double zl_hi, const Event *ev, double c,
+    double inv2c, double dtt, double tau, int K, size_t row, float *out_t, float *out_a,
+    float *out_x, float *out_z) {
+  const double z_lo = zq_lo * (1.0 / (1.0 - kFilterEps));
+  const double z_hi = zl_hi * (1.0 / (1.0 + kFilterEps));
+  const bool hi_finite = z_hi < 1e299;  // an end of +inf is stored as 1e300
   SolveStateZ r;
   r.live = 1;
   double zlast = z_lo;
   bool last_is_event = true;
-  const bool hi_finite = isfinite(t_hi);
   while (true) {
     const double disc = b * b - (4.0 * a) * c;
     if (disc < 0.0) break;
@@ -126,7 +150,7 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
     } else if (z < zlast * (1.0 - kZBand)) {
       above = false;
     } else {
-      const double last = last_is_event ? (double)t_lo : tau_log(tau, zlast);
+      const double last = last_is_event ? (double)ev->t : tau_log(tau, zlast);
       above = tau_log(tau, z) > last;
     }
     if (!above) break;
@@ -137,7 +161,7 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
     } else if (hi_finite && z > z_hi * (1.0 + kZBand)) {
       below = false;
     } else {
-      below = !(tau_log(tau, z) > (double)t_hi);
+      below = !(tau_log(tau, z) > (double)ev->tl);
     }
     if (!below) break;
     const size_t slot = row + n;
@@ -152,6 +176,8 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
       r.live = 0;
       break;
     }
+    const double zv = fmin(fmax(b * inv2c, z * (1.0 - kFilterEps)), zl_hi);
+    if (!(fma(fma(c, zv, -b), zv, a) <= kFilterEps * a)) break;
   }
   r.b = b;
   r.n = n;

@@ -28,18 +28,35 @@ namespace bats {
 
 constexpr int kBwdThreads = 256;
 
+// Compact record of a post row for the weight gradient of wide layers: its first kRK spike
+// times and their suffix sums rounded to float32 (rows with more spikes read their float64
+// sums from S1/S2). Slots past ne hold t = -inf (never above a pre-spike time).
+constexpr int kRK = 8;  // spikes of a row held in its record
+struct alignas(16) RowRec {
+  float t0, s10, s20;  // first spike: time and suffix sums (the one-spike fast path)
+  int ne;
+  float t[kRK];        // spike times, -inf past ne
+  float s1[kRK + 1];   // suffix sums; slot ne holds 0 (when ne <= kRK)
+  float s2[kRK + 1];
+  float pad[2];
+};
+static_assert(sizeof(RowRec) == 128, "RowRec layout");
+
+// apply_reset = 0: the errors already hold the reset-chain result, only the suffix sums are
+// formed. rec (optional) receives the row records.
 __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
     int BJ, int K, const int *__restrict__ n, const float *__restrict__ t,
     const float *__restrict__ a, const float *__restrict__ x, const float *__restrict__ z,
-    float *__restrict__ errors, double c, double tau, double dtt, int *__restrict__ n_eff,
-    double *__restrict__ S1, double *__restrict__ S2) {
+    float *__restrict__ errors, double c, double tau, double dtt, int apply_reset,
+    int *__restrict__ n_eff, double *__restrict__ S1, double *__restrict__ S2,
+    RowRec *__restrict__ rec) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= BJ) return;
   const size_t row = (size_t)lane * K;
   const int nk = min(n[lane], K);
   // reset chain, reverse order (propagate_recurrent_errors.cu:17-22)
   double C = 0.0;
-  for (int k = nk - 1; k >= 0; --k) {
+  for (int k = apply_reset ? nk - 1 : -1; k >= 0; --k) {
     const float zk = z[row + k];
     if (isinf(zk)) continue;
     const double ek = (double)errors[row + k] + C * (double)zk;
@@ -51,6 +68,9 @@ __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
   int ne = 0;
   while (ne < nk && !isinf(t[row + ne])) ++ne;
   double s1 = 0.0, s2 = 0.0;
+  float r1[kRK + 1], r2[kRK + 1];
+#pragma unroll
+  for (int k = 0; k <= kRK; ++k) r1[k] = r2[k] = 0.f;
   for (int k = ne - 1; k >= 0; --k) {
     const double ak = (double)a[row + k], xk = (double)x[row + k], zk = (double)z[row + k];
     const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
@@ -60,8 +80,31 @@ __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
     s2 += ek * (double)f2;
     S1[row + k] = s1;
     S2[row + k] = s2;
+#pragma unroll
+    for (int q = 0; q < kRK; ++q) {
+      if (q == k) {
+        r1[q] = (float)s1;
+        r2[q] = (float)s2;
+      }
+    }
   }
   n_eff[lane] = ne;
+  if (rec) {
+    RowRec r;
+    r.t0 = ne > 0 ? t[row] : -kInfF;
+    r.s10 = r1[0];
+    r.s20 = r2[0];
+    r.ne = ne;
+#pragma unroll
+    for (int q = 0; q < kRK; ++q) r.t[q] = q < ne ? t[row + q] : -kInfF;
+#pragma unroll
+    for (int q = 0; q <= kRK; ++q) {
+      r.s1[q] = r1[q];
+      r.s2[q] = r2[q];
+    }
+    r.pad[0] = r.pad[1] = 0.f;
+    rec[lane] = r;
+  }
 }
 
 // Post-neuron rows staged in LDS for SB samples at a time: spike counts, times and suffix
@@ -134,75 +177,13 @@ __device__ __forceinline__ int leading_finite(const float *__restrict__ row, int
   bool run = true;
 #pragma unroll 8
   for (int m = 0; m < K; ++m) {
-    run = run && isfinite(row[m]);
+    run = run & (bool)isfinite(row[m]);  // no short-circuit: the loads stay independent
     n += run ? 1 : 0;
   }
   return n;
 }
 
-// Lanes are pre neurons (a block covers up to 1024 of them), a block owns kWJT post neurons
-// and a slice of the batch. Per group of samples the block stages its post rows in LDS;
-// every lane accumulates es*V1 - e*V2 over its pre spikes in float64.
-constexpr int kWJT = 16;
-
-__global__ __launch_bounds__(1024) void wgrad_kernel(
-    int B, int I, int K_pre, int J, int K, int b_per_slice, int SB,
-    const float *__restrict__ pre_t, const float *__restrict__ pre_es,
-    const float *__restrict__ pre_e, const float *__restrict__ post_t,
-    const int *__restrict__ n_eff, const double *__restrict__ S1,
-    const double *__restrict__ S2, int per_sample, float *__restrict__ grad,
-    float *__restrict__ partial) {
-  extern __shared__ __align__(16) char smem[];
-  const RowStage st = row_stage(smem, SB, kWJT, K);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < I;
-  const int ic = active ? i : 0;
-  const int j0 = blockIdx.y * kWJT;
-  const int jn = min(kWJT, J - j0);
-  const int slice = blockIdx.z;
-  const int b0 = slice * b_per_slice;
-  const int b1 = min(B, b0 + b_per_slice);
-  double acc[kWJT];
-#pragma unroll
-  for (int u = 0; u < kWJT; ++u) acc[u] = 0.0;
-  for (int bg = b0; bg < b1; bg += SB) {
-    const int sb = min(SB, b1 - bg);
-    load_rows(st, bg, sb, J, j0, jn, kWJT, K, post_t, n_eff, S1, S2);
-    for (int s = 0; s < sb; ++s) {
-      const int b = bg + s;
-      const size_t pbase = ((size_t)b * I + ic) * K_pre;
-      const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
-      for (int m = 0; m < np; ++m) {
-        const float tp = pre_t[pbase + m];
-        const double es = (double)pre_es[pbase + m];
-        const double e = (double)pre_e[pbase + m];
-#pragma unroll
-        for (int u = 0; u < kWJT; ++u) {
-          const int nj = (u < jn) ? st.ne[s * kWJT + u] : 0;
-          if (nj > 0) {
-            const int off = (s * kWJT + u) * K;
-            double v1, v2;
-            lookup_count(st.t + off, st.s1 + off, st.s2 + off, nj, tp, v1, v2);
-            acc[u] = fma(es, v1, acc[u]);
-            acc[u] = fma(-e, v2, acc[u]);
-          }
-        }
-      }
-      if (per_sample) {
-#pragma unroll
-        for (int u = 0; u < kWJT; ++u) {
-          if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
-          acc[u] = 0.0;
-        }
-      }
-    }
-  }
-  if (!per_sample && active) {
-#pragma unroll
-    for (int u = 0; u < kWJT; ++u)
-      if (u < jn) partial[((size_t)slice * J + j0 + u) * I + i] = (float)acc[u];
-  }
-}
+constexpr int kWJT = 16;  // post rows per LDS stage of pre_err_kernel
 
 __global__ __launch_bounds__(kBwdThreads) void wgrad_reduce_kernel(
     int slices, int64_t JI, double inv_b, const float *__restrict__ partial,
@@ -266,28 +247,319 @@ __global__ __launch_bounds__(1024) void pre_err_kernel(
     for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
 }
 
-struct BwdLayout {
-  size_t neff, s1, s2, partial, scratch, total;
-  int slices, b_per_slice, lanes, gi;
+// ------------------------------------------------------------------ wide layers
+// Weight gradient from the row records: lanes are pre neurons, a block owns kRT post
+// neurons and a slice of the batch. The block first copies the records of all its (sample,
+// row) pairs into LDS (one coalesced load, one barrier); the walk over the slice then reads
+// them as wave-uniform LDS broadcasts and touches global memory only for the lanes' own pre
+// spikes (the next sample's slot is loaded while the current one is processed) and for
+// rows with more than kRK spikes. Per row and lane: kk = number of post spikes at or before
+// t_pre, q = es s1[kk] - e s2[kk] in float32 from the float32 record (one compare and a
+// select for one-spike rows), accumulated in float64.
+constexpr int kRT = 16;
+constexpr int kMaxRecSamples = 32;
+
+template <bool kSingle>
+__global__ __launch_bounds__(1024) void wgrad_rec_kernel(
+    int B, int I, int K_pre, int J, int K, int b_per_slice, const float *__restrict__ pre_t,
+    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
+    const RowRec *__restrict__ rec, const float *__restrict__ post_t,
+    const double *__restrict__ S1, const double *__restrict__ S2, int per_sample,
+    float *__restrict__ grad, float *__restrict__ partial) {
+  extern __shared__ __align__(16) char smem[];
+  RowRec *srec = reinterpret_cast<RowRec *>(smem);  // [b_per_slice][kRT]
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int i = blockIdx.x * nth + tid;
+  const bool active = i < I;
+  const int ic = active ? i : 0;
+  const int j0 = blockIdx.y * kRT;
+  const int jn = min(kRT, J - j0);
+  const int b0 = blockIdx.z * b_per_slice;
+  const int b1 = min(B, b0 + b_per_slice);
+  // the next pre slot (one slot per pre neuron): its loads overlap the record copy
+  float ntp = kInfF, nes = 0.f, nee = 0.f;
+  if (kSingle && b0 < b1 && active) {
+    const size_t pb = (size_t)b0 * I + i;
+    ntp = pre_t[pb];
+    nes = pre_es[pb];
+    nee = pre_e[pb];
+  }
+  {
+    constexpr int kQ = sizeof(RowRec) / 16;  // 16-byte quads per record
+    const int quads = (b1 - b0) * kRT * kQ;
+    const float4 *src = reinterpret_cast<const float4 *>(rec);
+    float4 *dst = reinterpret_cast<float4 *>(srec);
+    for (int e = tid; e < quads; e += nth) {
+      const int r = e / kQ, s = r / kRT, u = r - s * kRT;
+      float4 v = make_float4(-kInfF, 0.f, 0.f, 0.f);  // ne = 0 (bit pattern of 0.f)
+      if (u < jn) v = src[((size_t)(b0 + s) * J + j0 + u) * kQ + (e - r * kQ)];
+      dst[e] = v;
+    }
+  }
+  __syncthreads();
+
+  double acc[kRT];
+#pragma unroll
+  for (int u = 0; u < kRT; ++u) acc[u] = 0.0;
+
+  auto rows = [&](int s, size_t grow0, float tp, float es, float e) {
+    const RowRec *rs = srec + s * kRT;
+    unsigned slow = 0;  // rows with more than kRK spikes (wave-uniform mask)
+#pragma unroll
+    for (int u = 0; u < kRT; ++u) {
+      if (u < jn) {
+        const float4 r0 = *reinterpret_cast<const float4 *>(rs + u);
+        const int ne = __builtin_amdgcn_readfirstlane(__float_as_int(r0.w));
+        if (ne == 1) {
+          const float q0 = fmaf(es, r0.y, -(e * r0.z));
+          acc[u] += (double)((tp < r0.x) ? q0 : 0.f);
+        } else if (ne > 1 && ne <= kRK) {
+          int kk = 0;
+#pragma unroll
+          for (int k = 0; k < kRK; ++k)
+            if (k < ne) kk += (rs[u].t[k] <= tp) ? 1 : 0;
+          const float q = fmaf(es, rs[u].s1[kk], -(e * rs[u].s2[kk]));
+          acc[u] += (double)q;
+        } else if (ne > kRK) {
+          slow |= 1u << u;
+        }
+      }
+    }
+    while (slow) {
+      const int u = __builtin_ctz(slow);
+      slow &= slow - 1;
+      const int ne = __builtin_amdgcn_readfirstlane(rs[u].ne);
+      const size_t row = grow0 + (size_t)u * K;
+      int kk = 0;
+      for (int k = 0; k < ne; ++k) kk += (post_t[row + k] <= tp) ? 1 : 0;
+      const double v1 = kk < ne ? S1[row + kk] : 0.0;
+      const double v2 = kk < ne ? S2[row + kk] : 0.0;
+      const double d = fma((double)es, v1, -((double)e * v2));
+#pragma unroll
+      for (int q = 0; q < kRT; ++q)
+        if (q == u) acc[q] += d;
+    }
+  };
+
+  for (int b = b0; b < b1; ++b) {
+    const int s = b - b0;
+    const size_t grow0 = ((size_t)b * J + j0) * K;
+    if (kSingle) {
+      const float tp = ntp, es = nes, e = nee;
+      if (b + 1 < b1 && active) {
+        const size_t pb = (size_t)(b + 1) * I + i;
+        ntp = pre_t[pb];
+        nes = pre_es[pb];
+        nee = pre_e[pb];
+      }
+      // a slot without a spike has no defined exps: its contribution is forced to zero
+      const bool has = isfinite(tp);
+      rows(s, grow0, has ? tp : kInfF, has ? es : 0.f, has ? e : 0.f);
+    } else {
+      const size_t pbase = ((size_t)b * I + ic) * K_pre;
+      const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
+      for (int m = 0; m < np; ++m)
+        rows(s, grow0, pre_t[pbase + m], pre_es[pbase + m], pre_e[pbase + m]);
+    }
+    if (per_sample) {
+#pragma unroll
+      for (int u = 0; u < kRT; ++u) {
+        if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
+        acc[u] = 0.0;
+      }
+    }
+  }
+  if (!per_sample && active) {
+#pragma unroll
+    for (int u = 0; u < kRT; ++u)
+      if (u < jn) partial[((size_t)blockIdx.z * J + j0 + u) * I + i] = (float)acc[u];
+  }
+}
+
+// ------------------------------------------------------------------ narrow layers
+// Layers with at most kST post neurons (an output layer): one block per (pre-neuron block,
+// sample). The sample's raw post rows are copied to LDS, one thread per row runs the reset
+// chain, the factors and the suffix sums (the arithmetic of post_prep_kernel), and every
+// lane then forms, for each of its pre spikes, the weight-gradient terms of all post neurons
+// and -- from the same lookups -- the pre-spike error sum_j W (V1 es/tau_s - V2 e/tau).
+// The per-sample weight-gradient rows are reduced over the batch by wgrad_reduce_kernel.
+constexpr int kST = 16;
+
+struct DenseStage {
+  double *s1, *s2;            // [R][K+1]; slot ne holds 0
+  float *t, *a, *x, *z, *er;  // [R][K]
+  int *nn, *ne;               // [R]
 };
 
-static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K) {
+__host__ __device__ inline size_t dense_stage_bytes(int R, int K) {
+  return (size_t)R * (K + 1) * 16 + (size_t)R * K * 20 + (size_t)R * 8;
+}
+
+__device__ __forceinline__ DenseStage dense_stage(char *smem, int R, int K) {
+  DenseStage d;
+  d.s1 = reinterpret_cast<double *>(smem);
+  d.s2 = d.s1 + (size_t)R * (K + 1);
+  d.t = reinterpret_cast<float *>(d.s2 + (size_t)R * (K + 1));
+  d.a = d.t + (size_t)R * K;
+  d.x = d.a + (size_t)R * K;
+  d.z = d.x + (size_t)R * K;
+  d.er = d.z + (size_t)R * K;
+  d.nn = reinterpret_cast<int *>(d.er + (size_t)R * K);
+  d.ne = d.nn + R;
+  return d;
+}
+
+template <bool kPre>
+__global__ __launch_bounds__(1024) void small_bwd_kernel(
+    int I, int K_pre, int J, int K, const int *__restrict__ n, const float *__restrict__ t,
+    const float *__restrict__ a, const float *__restrict__ x, const float *__restrict__ z,
+    float *__restrict__ errors, int write_errors, double c, double tau, double dtt,
+    double inv_tau_s, double inv_tau, const float *__restrict__ pre_t,
+    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
+    const float *__restrict__ W, float *__restrict__ out, float *__restrict__ pre_errors) {
+  extern __shared__ __align__(16) char smem[];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int i = blockIdx.x * nth + tid;
+  const bool active = i < I;
+  const int ic = active ? i : 0;
+  const int b = blockIdx.y;
+  const DenseStage st = dense_stage(smem, J, K);
+  // the lane's pre spikes: count, and the first one (loads in flight with the row copy)
+  const size_t pbase = ((size_t)b * I + ic) * K_pre;
+  const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
+  float tp0 = kInfF, es0 = 0.f, e0 = 0.f;
+  if (np > 0) {
+    tp0 = pre_t[pbase];
+    es0 = pre_es[pbase];
+    e0 = pre_e[pbase];
+  }
+  float wv[kST];
+#pragma unroll
+  for (int u = 0; u < kST; ++u) wv[u] = (kPre && active && u < J) ? W[(size_t)u * I + i] : 0.f;
+  // 1. the sample's rows
+  const size_t rbase = (size_t)b * J * K;
+  for (int e = tid; e < J * K; e += nth) {
+    st.t[e] = t[rbase + e];
+    st.a[e] = a[rbase + e];
+    st.x[e] = x[rbase + e];
+    st.z[e] = z[rbase + e];
+    st.er[e] = errors[rbase + e];
+  }
+  if (tid < J) st.nn[tid] = n[(size_t)b * J + tid];
+  __syncthreads();
+  // 2. one thread per row
+  if (tid < J) {
+    const int r = tid;
+    const int nk = min(st.nn[r], K);
+    float *er = st.er + r * K;
+    const float *tr = st.t + r * K, *ar = st.a + r * K, *xr = st.x + r * K, *zr = st.z + r * K;
+    double C = 0.0;
+    for (int k = nk - 1; k >= 0; --k) {
+      const float zk = zr[k];
+      if (isinf(zk)) continue;
+      const double ek = (double)er[k] + C * (double)zk;
+      const float ek32 = (float)ek;
+      er[k] = ek32;
+      if (write_errors) errors[rbase + (size_t)r * K + k] = ek32;
+      C = C + (double)ek32 * dtt / (double)xr[k];
+    }
+    int ne = 0;
+    while (ne < nk && !isinf(tr[ne])) ++ne;
+    double *s1 = st.s1 + (size_t)r * (K + 1);
+    double *s2 = st.s2 + (size_t)r * (K + 1);
+    s1[ne] = 0.0;
+    s2[ne] = 0.0;
+    double a1 = 0.0, a2 = 0.0;
+    for (int k = ne - 1; k >= 0; --k) {
+      const double ak = (double)ar[k], xk = (double)xr[k], zk = (double)zr[k];
+      const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
+      const float f2 = (float)(tau / xk);
+      const double ek = (double)er[k];
+      a1 += ek * (double)f1;
+      a2 += ek * (double)f2;
+      s1[k] = a1;
+      s2[k] = a2;
+    }
+    st.ne[r] = ne;
+  }
+  __syncthreads();
+  // 3. lanes
+  double acc[kST];
+#pragma unroll
+  for (int u = 0; u < kST; ++u) acc[u] = 0.0;
+  for (int m = 0; m < np; ++m) {
+    float tp = tp0;
+    double es = (double)es0, e = (double)e0;
+    if (m > 0) {
+      tp = pre_t[pbase + m];
+      es = (double)pre_es[pbase + m];
+      e = (double)pre_e[pbase + m];
+    }
+    double pacc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kST; ++u) {
+      if (u < J) {
+        const int nj = __builtin_amdgcn_readfirstlane(st.ne[u]);
+        if (nj > 0) {
+          const float *tr = st.t + u * K;
+          int kk = 0;
+          for (int k = 0; k < nj; ++k) kk += (tr[k] <= tp) ? 1 : 0;
+          const double v1 = st.s1[(size_t)u * (K + 1) + kk];
+          const double v2 = st.s2[(size_t)u * (K + 1) + kk];
+          acc[u] = fma(es, v1, acc[u]);
+          acc[u] = fma(-e, v2, acc[u]);
+          if (kPre) pacc += (double)wv[u] * (v1 * es * inv_tau_s - v2 * e * inv_tau);
+        }
+      }
+    }
+    if (kPre) pre_errors[pbase + m] = (float)pacc;
+  }
+  if (kPre && active)
+    for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < kST; ++u)
+      if (u < J) out[((size_t)b * J + u) * I + i] = (float)acc[u];
+  }
+}
+
+struct BwdLayout {
+  size_t neff, s1, s2, rec, partial, scratch, total;
+  int slices, b_per_slice, lanes, gi;
+  bool small;
+};
+
+static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_sample = false) {
   BwdLayout L{};
   L.gi = std::max(1, (I + 1023) / 1024);
   const int per_block = (I + L.gi - 1) / L.gi;
   L.lanes = std::max(64, (per_block + 63) / 64 * 64);
-  const int gj = (J + kWJT - 1) / kWJT;
-  const int base = std::max(1, L.gi * gj);
-  int slices = std::max(1, std::min(B, (512 + base - 1) / base));
-  L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
-  L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
+  L.small = J <= kST && dense_stage_bytes(J, K) <= 64 * 1024;
+  if (L.small) {
+    L.slices = std::max(B, 1);
+    L.b_per_slice = 1;
+  } else {
+    const int gj = (J + kRT - 1) / kRT;
+    const int base = std::max(1, L.gi * gj);
+    // about 512 blocks (two per CU), rounded down so that the last round is not a tail; a
+    // block's record stage holds at most kMaxRecSamples samples (64 KB of LDS)
+    int slices = per_sample ? B : std::max(1, std::min(B, 512 / base));
+    slices = std::min(std::max(B, 1), std::max(slices, (B + kMaxRecSamples - 1) / kMaxRecSamples));
+    L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
+    L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
+  }
+  // rows prepared in global memory: the wide path, and the error write-back of wide
+  // narrow-layer launches (more than one block of pre neurons)
+  const bool rows = !L.small || L.gi > 1;
   size_t off = 0;
   L.neff = off;
-  off = align_up(off + sizeof(int) * (size_t)B * J, 256);
+  if (rows) off = align_up(off + sizeof(int) * (size_t)B * J, 256);
   L.s1 = off;
-  off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  if (rows) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
   L.s2 = off;
-  off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  if (rows) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  L.rec = off;
+  if (rows) off = align_up(off + sizeof(RowRec) * (size_t)B * J, 256);
   L.partial = off;
   off = align_up(off + sizeof(float) * (size_t)L.slices * J * I, 256);
   L.scratch = off;
@@ -296,13 +568,22 @@ static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K) {
   return L;
 }
 
+// Dynamic LDS above the default 64 KB needs the kernel's attribute raised first.
+static int allow_lds(const void *kernel, size_t bytes) {
+  if (bytes <= 64 * 1024) return 0;
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+      hipSuccess)
+    return fail("backward: LDS request above the device limit");
+  return 0;
+}
+
 int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const float *a,
               const float *x, const float *z, float *errors, float c, float tau, float dtt,
               int *n_eff, double *S1, double *S2) {
   if (BJ <= 0) return 0;
   hipLaunchKernelGGL(post_prep_kernel, dim3((BJ + kBwdThreads - 1) / kBwdThreads),
                      dim3(kBwdThreads), 0, st, BJ, K, n, t, a, x, z, errors, (double)c,
-                     (double)tau, (double)dtt, n_eff, S1, S2);
+                     (double)tau, (double)dtt, 1, n_eff, S1, S2, (RowRec *)nullptr);
   return check_launch("post_prep_kernel");
 }
 
@@ -329,6 +610,7 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   BATS_REQUIRE(K_post <= kMaxBackwardK, "max_n_spike above 384 is not supported by the backward");
   BATS_REQUIRE(grad_mode >= 0 && grad_mode <= 2,
                "grad_mode must be 0 (batch mean), 1 (per sample) or 2 (batch sum)");
+  BATS_REQUIRE(B <= 65535, "batch above 65535");
   hipStream_t st = as_stream(stream);
   const int64_t JI = (int64_t)J * I;
   if (B == 0) {
@@ -349,49 +631,82 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   int *n_eff = reinterpret_cast<int *>(ws + L.neff);
   double *S1 = reinterpret_cast<double *>(ws + L.s1);
   double *S2 = reinterpret_cast<double *>(ws + L.s2);
+  RowRec *rec = reinterpret_cast<RowRec *>(ws + L.rec);
   float *partial = reinterpret_cast<float *>(ws + L.partial);
   double *scratch = reinterpret_cast<double *>(ws + L.scratch);
-  // samples staged per round: as many as fit 48 KB of LDS (at least one)
-  const int SB = (int)std::max<size_t>(1, std::min<size_t>(4, (48 * 1024) / row_stage_bytes(1, kWJT, K_post)));
-  const size_t lds_w = row_stage_bytes(SB, kWJT, K_post);
-  const size_t lds_p = row_stage_bytes(1, kWJT, K_post);
-
   const int BJ = B * J;
-  if (BJ > 0) {
-    hipLaunchKernelGGL(post_prep_kernel, dim3((BJ + kBwdThreads - 1) / kBwdThreads),
-                       dim3(kBwdThreads), 0, st, BJ, K_post, n, t, a, x, z, errors, (double)c,
-                       (double)tau, (double)delta_theta_tau, n_eff, S1, S2);
-    if (int r = check_launch("post_prep_kernel")) return r;
+  const unsigned prep_blocks = (unsigned)((BJ + kBwdThreads - 1) / kBwdThreads);
+  if (JI == 0) {  // no weights: only the reset chain of the errors
+    if (BJ > 0) {
+      hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st, BJ,
+                         K_post, n, t, a, x, z, errors, (double)c, (double)tau,
+                         (double)delta_theta_tau, 1, n_eff, S1, S2, (RowRec *)nullptr);
+      return check_launch("post_prep_kernel");
+    }
+    return 0;
   }
-  if (JI > 0) {
-    BATS_REQUIRE(K_pre == 0 || (pre_times && pre_exp_tau_s && pre_exp_tau), "null pre pointer");
-    const int gj = (J + kWJT - 1) / kWJT;
-    if (grad_mode == 1) {
-      dim3 g(L.gi, gj, B);  // one sample per block
-      hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds_w, st, B, I, K_pre, J, K_post, 1,
-                         SB, pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff, S1, S2, 1, grad,
-                         partial);
-      if (int r = check_launch("wgrad_kernel")) return r;
-    } else {
-      dim3 g(L.gi, gj, L.slices);
-      BATS_TIMED("wgrad_kernel", st,
-                 hipLaunchKernelGGL(wgrad_kernel, g, dim3(L.lanes), lds_w, st, B, I, K_pre, J,
-                                    K_post, L.b_per_slice, SB, pre_times, pre_exp_tau_s,
-                                    pre_exp_tau, t, n_eff, S1, S2, 0, grad, partial));
-      if (int r = check_launch("wgrad_kernel")) return r;
-      hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
-                         dim3(kBwdThreads), 0, st, L.slices, JI,
-                         grad_mode == 2 ? 1.0 : 1.0 / (double)B, partial, grad);
+  const bool want_pre = pre_errors && K_pre > 0;
+  if (want_pre) BATS_REQUIRE(weights, "null weights");
+  BATS_REQUIRE(K_pre == 0 || (pre_times && pre_exp_tau_s && pre_exp_tau), "null pre pointer");
+  const double inv_b = grad_mode == 2 ? 1.0 : 1.0 / (double)B;
+  const unsigned red_blocks = (unsigned)((JI + kBwdThreads - 1) / kBwdThreads);
+
+  if (L.small) {
+    float *out = grad_mode == 1 ? grad : partial;
+    const size_t lds = dense_stage_bytes(J, K_post);
+    auto kern = want_pre ? small_bwd_kernel<true> : small_bwd_kernel<false>;
+    BATS_TIMED("small_bwd_kernel", st,
+               hipLaunchKernelGGL(kern, dim3(L.gi, B), dim3(L.lanes), lds, st, I, K_pre, J,
+                                  K_post, n, t, a, x, z, errors, L.gi == 1 ? 1 : 0, (double)c,
+                                  (double)tau, (double)delta_theta_tau, 1.0 / (double)tau_s,
+                                  1.0 / (double)tau, pre_times, pre_exp_tau_s, pre_exp_tau,
+                                  weights, out, want_pre ? pre_errors : nullptr));
+    if (int r = check_launch("small_bwd_kernel")) return r;
+    if (grad_mode != 1) {
+      hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(red_blocks), dim3(kBwdThreads), 0, st, B, JI,
+                         inv_b, partial, grad);
+      if (int r = check_launch("wgrad_reduce_kernel")) return r;
+    }
+    if (L.gi > 1) {  // every block read the raw errors; write the reset chain back now
+      hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st, BJ,
+                         K_post, n, t, a, x, z, errors, (double)c, (double)tau,
+                         (double)delta_theta_tau, 1, n_eff, S1, S2, (RowRec *)nullptr);
+      if (int r = check_launch("post_prep_kernel")) return r;
+    }
+    return 0;
+  }
+
+  BATS_TIMED("post_prep_kernel", st,
+             hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st,
+                                BJ, K_post, n, t, a, x, z, errors, (double)c, (double)tau,
+                                (double)delta_theta_tau, 1, n_eff, S1, S2, rec));
+  if (int r = check_launch("post_prep_kernel")) return r;
+  {
+    const BwdLayout F = bwd_layout(B, I, K_pre, J, K_post, grad_mode == 1);
+    const int gj = (J + kRT - 1) / kRT;
+    auto kern = K_pre == 1 ? wgrad_rec_kernel<true> : wgrad_rec_kernel<false>;
+    const size_t lds = sizeof(RowRec) * (size_t)F.b_per_slice * kRT;
+    if (int r = allow_lds((const void *)kern, lds)) return r;
+    BATS_TIMED("wgrad_rec_kernel", st,
+               hipLaunchKernelGGL(kern, dim3(F.gi, gj, F.slices), dim3(F.lanes), lds, st, B, I,
+                                  K_pre, J, K_post, F.b_per_slice, pre_times, pre_exp_tau_s,
+                                  pre_exp_tau, rec, t, S1, S2, grad_mode == 1 ? 1 : 0, grad,
+                                  partial));
+    if (int r = check_launch("wgrad_rec_kernel")) return r;
+    if (grad_mode != 1) {
+      hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(red_blocks), dim3(kBwdThreads), 0, st,
+                         F.slices, JI, inv_b, partial, grad);
       if (int r = check_launch("wgrad_reduce_kernel")) return r;
     }
   }
-  if (pre_errors && I > 0 && K_pre > 0) {
-    BATS_REQUIRE(weights, "null weights");
-    dim3 g(L.gi, B);
+  if (want_pre) {
+    const size_t lds_p = row_stage_bytes(1, kWJT, K_post);
+    if (int r = allow_lds((const void *)pre_err_kernel, lds_p)) return r;
     BATS_TIMED("pre_err_kernel", st,
-               hipLaunchKernelGGL(pre_err_kernel, g, dim3(L.lanes), lds_p, st, I, K_pre, J, K_post,
-                                  pre_times, pre_exp_tau_s, pre_exp_tau, weights, t, n_eff, S1,
-                                  S2, (double)tau_s, (double)tau, scratch, pre_errors));
+               hipLaunchKernelGGL(pre_err_kernel, dim3(L.gi, B), dim3(L.lanes), lds_p, st, I,
+                                  K_pre, J, K_post, pre_times, pre_exp_tau_s, pre_exp_tau,
+                                  weights, t, n_eff, S1, S2, (double)tau_s, (double)tau,
+                                  scratch, pre_errors));
     if (int r = check_launch("pre_err_kernel")) return r;
   }
   return 0;

@@ -202,6 +202,46 @@ __global__ __launch_bounds__(kBwdThreads) void wgrad_reduce_kernel(
   grad[e] = (float)(((s0 + s1) + (s2 + s3)) * inv_b);
 }
 
+// Many slices (one per sample): a block sums 64 consecutive elements, its 4 thread groups
+// take every 4th slice each with independent loads, and the 4 group sums are added in a
+// fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void wgrad_reduce_tall_kernel(
+    int slices, int64_t JI, double inv_b, const float *__restrict__ partial,
+    float *__restrict__ grad) {
+  __shared__ double s_part[4][64];
+  const int col = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * 64 + col;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (e < JI) {
+    int z = grp;
+    for (; z + 12 < slices; z += 16) {
+      acc[0] += (double)partial[(size_t)z * JI + e];
+      acc[1] += (double)partial[(size_t)(z + 4) * JI + e];
+      acc[2] += (double)partial[(size_t)(z + 8) * JI + e];
+      acc[3] += (double)partial[(size_t)(z + 12) * JI + e];
+    }
+    for (; z < slices; z += 4) acc[0] += (double)partial[(size_t)z * JI + e];
+  }
+  s_part[grp][col] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  __syncthreads();
+  if (grp == 0 && e < JI)
+    grad[e] = (float)(((s_part[0][col] + s_part[1][col]) + (s_part[2][col] + s_part[3][col])) *
+                      inv_b);
+}
+
+// Sums the per-slice partial gradients into grad, scaled by inv_b.
+static int reduce_slices(hipStream_t st, int slices, int64_t JI, double inv_b,
+                         const float *partial, float *grad) {
+  if (slices >= 32) {
+    hipLaunchKernelGGL(wgrad_reduce_tall_kernel, dim3((unsigned)((JI + 63) / 64)), dim3(256), 0,
+                       st, slices, JI, inv_b, partial, grad);
+    return check_launch("wgrad_reduce_tall_kernel");
+  }
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
+                     dim3(kBwdThreads), 0, st, slices, JI, inv_b, partial, grad);
+  return check_launch("wgrad_reduce_kernel");
+}
+
 // Lanes are pre neurons of one sample; the post rows are staged kWJT at a time and every
 // pre spike of the lane accumulates W[j,i] (V1 es/tau_s - V2 e/tau) over all post neurons.
 // Pre-spike sums of one lane are kept in float64 scratch rows across the row chunks.
@@ -649,7 +689,6 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   if (want_pre) BATS_REQUIRE(weights, "null weights");
   BATS_REQUIRE(K_pre == 0 || (pre_times && pre_exp_tau_s && pre_exp_tau), "null pre pointer");
   const double inv_b = grad_mode == 2 ? 1.0 : 1.0 / (double)B;
-  const unsigned red_blocks = (unsigned)((JI + kBwdThreads - 1) / kBwdThreads);
 
   if (L.small) {
     float *out = grad_mode == 1 ? grad : partial;
@@ -662,11 +701,8 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                                   1.0 / (double)tau, pre_times, pre_exp_tau_s, pre_exp_tau,
                                   weights, out, want_pre ? pre_errors : nullptr));
     if (int r = check_launch("small_bwd_kernel")) return r;
-    if (grad_mode != 1) {
-      hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(red_blocks), dim3(kBwdThreads), 0, st, B, JI,
-                         inv_b, partial, grad);
-      if (int r = check_launch("wgrad_reduce_kernel")) return r;
-    }
+    if (grad_mode != 1)
+      if (int r = reduce_slices(st, B, JI, inv_b, partial, grad)) return r;
     if (L.gi > 1) {  // every block read the raw errors; write the reset chain back now
       hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st, BJ,
                          K_post, n, t, a, x, z, errors, (double)c, (double)tau,
@@ -693,11 +729,8 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                                   pre_exp_tau, rec, t, S1, S2, grad_mode == 1 ? 1 : 0, grad,
                                   partial));
     if (int r = check_launch("wgrad_rec_kernel")) return r;
-    if (grad_mode != 1) {
-      hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(red_blocks), dim3(kBwdThreads), 0, st,
-                         F.slices, JI, inv_b, partial, grad);
-      if (int r = check_launch("wgrad_reduce_kernel")) return r;
-    }
+    if (grad_mode != 1)
+      if (int r = reduce_slices(st, F.slices, JI, inv_b, partial, grad)) return r;
   }
   if (want_pre) {
     const size_t lds_p = row_stage_bytes(1, kWJT, K_post);

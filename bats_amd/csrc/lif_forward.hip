@@ -69,9 +69,47 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *keys, int N, bool global_
   }
 }
 
+// Bitonic network over 1024 keys held two per thread (keys 2t and 2t+1) by a 512-thread
+// block: the partner of a compare-exchange at distance j >= 2 is thread t ^ (j/2), reached
+// with a lane shuffle inside the wave (j/2 < 64) or through LDS (j/2 >= 64, 6 of the 55
+// stages, each between two barriers). Ascending order on exit.
+__device__ __forceinline__ void bitonic1024_regs(uint64_t &x0, uint64_t &x1,
+                                                 uint64_t *__restrict__ lds) {
+  const int t = threadIdx.x;
+  for (int k = 2; k <= 1024; k <<= 1) {
+    const bool up = ((2 * t) & k) == 0;
+    for (int j = k >> 1; j >= 1; j >>= 1) {
+      if (j == 1) {
+        const uint64_t lo = x0 < x1 ? x0 : x1, hi = x0 < x1 ? x1 : x0;
+        x0 = up ? lo : hi;
+        x1 = up ? hi : lo;
+        continue;
+      }
+      const int pj = j >> 1;
+      uint64_t y0, y1;
+      if (pj < 64) {
+        y0 = __shfl_xor(x0, pj, 64);
+        y1 = __shfl_xor(x1, pj, 64);
+      } else {
+        __syncthreads();
+        lds[2 * t] = x0;
+        lds[2 * t + 1] = x1;
+        __syncthreads();
+        const int pt = t ^ pj;
+        y0 = lds[2 * pt];
+        y1 = lds[2 * pt + 1];
+      }
+      const bool keep_min = ((t & pj) == 0) == up;
+      x0 = keep_min ? (x0 < y0 ? x0 : y0) : (x0 < y0 ? y0 : x0);
+      x1 = keep_min ? (x1 < y1 ? x1 : y1) : (x1 < y1 ? y1 : x1);
+    }
+  }
+}
+
 // One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
 // sample's Event list padded to a multiple of kEventPad, the event indices, the count, and
-// the per-slot float32 exps the backward reads.
+// the per-slot float32 exps the backward reads. Up to 1024 events the sort runs in
+// registers (bitonic1024_regs); longer lists use the LDS (or global) network.
 __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
     float tmax, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
@@ -88,27 +126,69 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     s_fill = 0;
   }
   __syncthreads();
-  int local = 0;
-  for (int s = threadIdx.x; s < S; s += blockDim.x) local += isfinite(times[s]) ? 1 : 0;
-  if (local) atomicAdd(&s_count, local);
-  __syncthreads();
-  const int P = s_count;
-  int N = 1;
-  while (N < P) N <<= 1;
-  const bool use_global = N > lds_cap;
-  uint64_t *keys = use_global ? gkeys + (size_t)b * gcap : lkeys;
-  for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    const float t = times[s];
-    if (isfinite(t)) {
-      const int pos = atomicAdd(&s_fill, 1);
-      keys[pos] = ((uint64_t)float_key(t) << 32) | (uint32_t)s;
+  const int lane = threadIdx.x & 63;
+  // compaction into LDS in one pass: kUnroll independent loads per thread in flight, and a
+  // wave's finite slots take consecutive positions from one atomic. Slots past the LDS
+  // capacity are only counted; such a sample is compacted again into global memory.
+  constexpr int kUnroll = 8;
+  const int nth = blockDim.x;
+  for (int s0 = 0; s0 < S; s0 += kUnroll * nth) {
+    float tv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int s = s0 + u * nth + threadIdx.x;
+      tv[u] = s < S ? times[s] : kInfF;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int s = s0 + u * nth + threadIdx.x;
+      const bool fin = isfinite(tv[u]);
+      const unsigned long long m = __ballot(fin);
+      int base = 0;
+      if (lane == 0 && m) base = atomicAdd(&s_fill, __popcll(m));
+      base = __shfl(base, 0, 64);
+      const int pos = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+      if (fin && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(tv[u]) << 32) | (uint32_t)s;
     }
   }
   __syncthreads();
-  for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
-  if (use_global) __threadfence();
-  __syncthreads();
-  if (P > 1) bitonic_sort(keys, N, use_global);
+  const int P = s_fill;
+  int N = 1;
+  while (N < P) N <<= 1;
+  const bool regs = P <= 1024;
+  const bool use_global = !regs && N > lds_cap;
+  uint64_t *keys = use_global ? gkeys + (size_t)b * gcap : lkeys;
+  if (use_global) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    for (int s0 = 0; s0 < S; s0 += nth) {
+      const int s = s0 + threadIdx.x;
+      const float t = s < S ? times[s] : kInfF;
+      const bool fin = isfinite(t);
+      const unsigned long long m = __ballot(fin);
+      int base = 0;
+      if (lane == 0 && m) base = atomicAdd(&s_count, __popcll(m));
+      base = __shfl(base, 0, 64);
+      if (fin) keys[base + (int)__popcll(m & ((1ull << lane) - 1ull))] =
+          ((uint64_t)float_key(t) << 32) | (uint32_t)s;
+    }
+    __syncthreads();
+  }
+  if (regs) {
+    uint64_t x0 = 2 * threadIdx.x < P ? keys[2 * threadIdx.x] : ~0ull;
+    uint64_t x1 = 2 * threadIdx.x + 1 < P ? keys[2 * threadIdx.x + 1] : ~0ull;
+    if (P > 1) bitonic1024_regs(x0, x1, keys);
+    __syncthreads();
+    keys[2 * threadIdx.x] = x0;
+    keys[2 * threadIdx.x + 1] = x1;
+    __syncthreads();
+  } else {
+    for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
+    if (use_global) __threadfence();
+    __syncthreads();
+    bitonic_sort(keys, N, use_global);
+  }
 
   Event *ev = events + (size_t)b * Pcap;
   FastEvent *fv = fast + (size_t)b * Pcap;
@@ -402,7 +482,7 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   uint64_t *gkeys = reinterpret_cast<uint64_t *>(base + off);
   out->Pcap = Pcap;
   if (S > 0) {
-    const int lds_entries = std::min(N, cap);
+    const int lds_entries = std::max(1024, std::min(N, cap));  // the register sort's exchange
     BATS_TIMED("sort_events_kernel", st,
                hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
                                   sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,

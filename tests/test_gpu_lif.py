@@ -111,6 +111,22 @@ def test_forward_empty_batch():
     assert n.sum() == 0 and np.all(np.isinf(tt))
 
 
+@pytest.mark.parametrize("n_events", [1024, 1025, 2800])
+def test_forward_event_list_sort_paths(n_events):
+    # up to 1024 events per sample the sort runs in registers, above it in LDS; the first two
+    # cases sit on either side of that boundary with every slot finite
+    rng = np.random.default_rng(7)
+    if n_events <= 1025:
+        t = rng.uniform(0.0, 1.0, size=(3, n_events, 1)).astype(F32)
+    else:
+        t, _ = multi_spike_inputs(rng, 3, 1000, 8, t_max=1.0)
+        counts = np.isfinite(t).sum(axis=(1, 2))
+        assert (counts > 1025).all() and (counts < 16384).all()
+    W = (rng.normal(0.2, 1.0, size=(40, t.shape[1])) * (30.0 / np.sqrt(t.shape[1]))).astype(F32)
+    ref, _ = check_forward(t, W, 0.130, 1.0, 1.0, 12, 1.0, f"sort path {n_events}")
+    assert ref.n.sum() > 0
+
+
 def test_forward_large_event_list_global_sort():
     # 800 x 30 slots per sample with ~20k spikes: exceeds the LDS sort capacity
     rng = np.random.default_rng(6)

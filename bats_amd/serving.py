@@ -110,13 +110,18 @@ class InferenceServer:
 
     def __init__(self, layers: Sequence[LayerSpec], n_inputs: int, max_batch: int = 1024,
                  max_simulation: float = 1.0, memory_budget: Optional[int] = None,
-                 device=None, min_batch_bucket: int = 8):
+                 device=None, min_batch_bucket: int = 8, min_events_bucket: int = 64,
+                 min_spikes_bucket: int = 1):
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
         self.n_inputs = int(n_inputs)
         self.max_batch = int(max_batch)
         self.max_simulation = float(max_simulation)
         self.min_batch_bucket = int(min_batch_bucket)
+        # larger minimum buckets mean fewer distinct padded shapes (graphs) at the cost of
+        # padding work; a server with a known request mix sets them to its upper ends
+        self.min_events_bucket = int(min_events_bucket)
+        self.min_spikes_bucket = int(min_spikes_bucket)
         self.w_dev = [ops.as_device(L.weights, torch.float32, self.device) for L in self.layers]
         self.memory_budget = memory_budget
         self.memory_in_use = 0
@@ -130,8 +135,8 @@ class InferenceServer:
     def request_bytes(self, r: Request) -> int:
         """Device bytes a request holds while admitted: its rows of the event buffers, the
         per-neuron input buffer and every layer's outputs and workspace rows."""
-        P = _bucket(len(r.indices), 64)
-        K_in = max(r.multiplicity, 1)
+        P = _bucket(len(r.indices), self.min_events_bucket)
+        K_in = _bucket(max(r.multiplicity, 1), self.min_spikes_bucket)
         b = 8 * P + 4 * self.n_inputs * K_in * 3 + 64 * self.n_inputs * K_in
         for L in self.layers:
             b += 4 * L.weights.shape[0] * (1 + 5 * L.max_n_spike)
@@ -164,8 +169,8 @@ class InferenceServer:
 
     def _launch(self, batch: List[Request], stream) -> _GraphBatch:
         Bb = min(_bucket(len(batch), self.min_batch_bucket), _bucket(self.max_batch, 1))
-        Pb = _bucket(max(len(r.indices) for r in batch), 64)
-        Kb = _bucket(max(max(r.multiplicity for r in batch), 1), 1)
+        Pb = _bucket(max(len(r.indices) for r in batch), self.min_events_bucket)
+        Kb = _bucket(max(max(r.multiplicity for r in batch), 1), self.min_spikes_bucket)
         g = self._graph_for(Bb, Pb, Kb, stream)
         hi, ht = g.h_idx.numpy(), g.h_t.numpy()
         hi.fill(-1)

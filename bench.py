@@ -204,35 +204,34 @@ def plain_bench(args):
 
 
 def serve_bench(args):
-    """Inference serving of the same network (weights as trained-from, inputs of the same
-    distribution): one request = one latency-coded sample as an event list. Measures the
-    saturated throughput (all requests queued at once), then p50/p99 latency (arrival ->
-    result on the host) under Poisson arrivals at 25%, 50% and 90% of it."""
+    """Inference serving of the same network: one request = one event list of
+    P ~ U{50..2000} events, input index ~ U{0..783} with replacement (several spikes per input
+    neuron), times ~ U[0, 1]. Measures the saturated throughput (all requests queued at once),
+    then p50/p99 latency (arrival -> result on the host) under Poisson arrivals at 25%, 50% and
+    90% of it, and the output spike counts of the first batch against the plain form's."""
     import torch
-    from bats_amd.serving import InferenceServer, LayerSpec, Request, run_poisson
+    from bats_amd.serving import InferenceServer, LayerSpec, make_requests, run_poisson
 
     _, _, _, w1, w2 = make_inputs(args.seed, 8, args.variant)
     layers = [LayerSpec(w1, TAU_S, THETA_H, THETA_H, K_HID),
               LayerSpec(w2, TAU_S, THETA_O, THETA_O, K_OUT)]
-    srv = InferenceServer(layers, N_IN, max_batch=args.serve_batch, max_simulation=T_MAX)
-    rng = np.random.default_rng(args.seed + 1)
+    # padded shapes: events to 2048, spikes per input neuron to 16 (the request mix's upper
+    # ends), batches to powers of two
+    srv = InferenceServer(layers, N_IN, max_batch=args.serve_batch, max_simulation=T_MAX,
+                          min_events_bucket=2048, min_spikes_bucket=16)
+    seed = [args.seed + 1]
 
     def requests(count):
-        out = []
-        for i in range(count):
-            x = rng.uniform(0.0, 1.0, size=N_IN)
-            idx = np.nonzero(x > 0.1)[0].astype(np.int32)
-            tm = (1.0 - x[idx]).astype(np.float32)
-            order = np.argsort(tm, kind="stable")
-            out.append(Request(rid=i, indices=idx[order], times=tm[order], arrival=0.0))
-        return out
+        seed[0] += 1
+        return make_requests(count, n_inputs=N_IN, p_min=50, p_max=2000, seed=seed[0])
 
     Bb = 8
     shapes = []
     while Bb <= args.serve_batch:
-        shapes.append((Bb, 1024, 1))
+        shapes.append((Bb, 2048, 16))
         Bb *= 2
     srv.warmup(shapes)
+    agreement = plain_count_agreement(args, layers, requests(256), srv)
     # saturated throughput
     reqs = requests(args.serve_requests)
     srv.completed = []
@@ -253,7 +252,45 @@ def serve_bench(args):
                       "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
                       "requests": res["n"]})
     return {"saturated_samples_per_s": round(sat, 1), "max_batch": args.serve_batch,
-            "loads": loads}
+            "requests": "P ~ U{50..2000} events, idx ~ U{0..783} with replacement, t ~ U[0,1]",
+            "count_agreement_with_plain": agreement, "loads": loads}
+
+
+def plain_count_agreement(args, layers, reqs, srv):
+    """Serves `reqs` and runs the plain form on the same requests (per-neuron buffers built
+    from the event lists); returns the fraction of equal output spike counts."""
+    import torch
+    from bats_amd import Network, ops
+    from bats_amd.Layers import InputLayer
+    from bats_amd.plain import PlainLIFLayer
+
+    srv.completed = []
+    for r in reqs:
+        srv.submit(r)
+    srv.drain()
+    torch.cuda.synchronize()
+    served = {r.rid: r.counts for r in srv.completed}
+    P = max(len(r.indices) for r in reqs)
+    idx = np.full((len(reqs), P), -1, dtype=np.int32)
+    tm = np.full((len(reqs), P), np.inf, dtype=np.float32)
+    for k, r in enumerate(reqs):
+        idx[k, :len(r.indices)] = r.indices
+        tm[k, :len(r.times)] = r.times
+    per_t, per_n = ops.events_to_per_neuron(idx, tm, N_IN)
+    net = Network()
+    inp = InputLayer(n_neurons=N_IN, name="input")
+    net.add_layer(inp, input=True)
+    prev = inp
+    for L, name in zip(layers, ("hidden", "output")):
+        lay = PlainLIFLayer(prev, tau_s=L.tau_s, theta=L.theta, delta_theta=L.delta_theta,
+                            n_neurons=L.weights.shape[0], weight_initializer=lambda a, b, w=L.weights: w,
+                            max_n_spike=L.max_n_spike, name=name)
+        net.add_layer(lay)
+        prev = lay
+    net.forward(per_t, per_n, max_simulation=T_MAX)
+    plain_n = prev.spike_trains[1].cpu().numpy()
+    same = [np.array_equal(served[r.rid], plain_n[k]) for k, r in enumerate(reqs)]
+    return float(np.mean(same))
 
 
 def first_forward_counts(args):
@@ -267,16 +304,17 @@ def first_forward_counts(args):
 
 
 # ---------------------------------------------------------------- CPU baseline (oracle)
-def cpu_baseline(args):
-    """The FP64 NumPy oracle's training step on a bounded sample of the same workload."""
+def _oracle_steps(job):
+    """One CPU worker: the FP64 NumPy oracle's training step on its shard, BLAS on one
+    thread; returns (samples, seconds)."""
+    seed, batch, variant, lo, hi, min_seconds = job
     from threadpoolctl import threadpool_limits
     from oracle import lif as olif
     from oracle import losses as oloss
     from oracle import optim as ooptim
 
-    t, n, labels, w1, w2 = make_inputs(args.seed, args.batch, args.variant)
-    S = args.cpu_sample
-    t, labels = t[:S], labels[:S]
+    t, n, labels, w1, w2 = make_inputs(seed, batch, variant)
+    t, labels = t[lo:hi], labels[lo:hi]
     with threadpool_limits(limits=1):
         t0 = time.perf_counter()
         reps = 0
@@ -291,9 +329,40 @@ def cpu_baseline(args):
             opt.step([olif.batch_mean(g1), olif.batch_mean(g2)])
             reps += 1
             el = time.perf_counter() - t0
-            if el >= args.cpu_min_seconds or reps >= 3:
+            if el >= min_seconds or reps >= 3:
                 break
-    return dict(value=S * reps / el, seconds=el, reps=reps)
+    return (hi - lo) * reps, el
+
+
+def cpu_baseline(args):
+    """The oracle's training step on bounded samples of the same workload: one core on
+    args.cpu_sample samples, then every worker core (at most 16) on its own shard of the
+    batch at once. Runs before the GPU is initialised (the workers are forked)."""
+    import multiprocessing as mp
+    one_n, one_s = _oracle_steps((args.seed, args.batch, args.variant, 0, args.cpu_sample,
+                                  args.cpu_min_seconds))
+    cores = max(1, min(16, os.cpu_count() or 1))
+    per = max(1, min(args.cpu_sample // 2, args.batch // cores))
+    jobs = [(args.seed, args.batch, args.variant, (k * per) % args.batch,
+             (k * per) % args.batch + per, args.cpu_min_seconds / 2) for k in range(cores)]
+    with mp.get_context("fork").Pool(cores) as pool:
+        t0 = time.perf_counter()
+        res = pool.map(_oracle_steps, jobs)
+        wall = time.perf_counter() - t0
+    all_n = sum(r[0] for r in res)
+    return dict(one=one_n / one_s, one_seconds=one_s, one_samples=one_n, all=all_n / wall,
+                all_seconds=wall, cores=cores, per=per)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload, from the calibrated FETCH_SIZE /
+    WRITE_SIZE counters recorded in profiles/traffic.json (None when absent)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[kernel]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -323,6 +392,7 @@ def main():
               "see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    c = None if args.skip_cpu else cpu_baseline(args)  # before the GPU is initialised
     r = gpu_bench(args)
     B = args.batch
     # whole-job rate of the faster launch mode (both time exactly K steps)
@@ -337,7 +407,8 @@ def main():
     if solve_ms:
         achieved = alg_bytes / (solve_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic("lif_solve_seq_kernel") if B == 256 else None,
                 "kernel": "lif_solve_seq_kernel (784->800 layer)",
                 "kernel_avg_us": round(solve_ms * 1e3, 2), "algorithmic_bytes": alg_bytes}
 
@@ -354,12 +425,15 @@ def main():
     serve = None if args.skip_serve else serve_bench(args)
 
     cpu = None
-    if not args.skip_cpu:
-        c = cpu_baseline(args)
-        cpu = {"value": round(c["value"], 3), "unit": "samples/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_sample} samples of the same batch, full training step of the "
-                         f"FP64 NumPy oracle, {c['reps']} rep(s), {c['seconds']:.1f} s, "
-                         f"BLAS limited to 1 thread"}
+    if c is not None:
+        cpu = {"value": round(c["all"], 3), "unit": "samples/s", "cores": c["cores"],
+               "kind": "port",
+               "sample": f"full training step of the FP64 NumPy oracle (BLAS on 1 thread per "
+                         f"process) on shards of {c['per']} samples of the same batch, "
+                         f"{c['cores']} forked processes at once, {c['all_seconds']:.1f} s wall; "
+                         f"1 core: {c['one']:.3f} samples/s on {args.cpu_sample} samples "
+                         f"({c['one_seconds']:.1f} s)",
+               "value_1_core": round(c["one"], 3)}
 
     step_bytes = 176e6 * B / 256.0
     line = {

@@ -29,16 +29,15 @@ namespace bats {
 constexpr int kBwdThreads = 256;
 
 // Compact record of a post row for the weight gradient of wide layers: its first kRK spike
-// times and their suffix sums rounded to float32 (rows with more spikes read their float64
-// sums from S1/S2). Slots past ne hold t = -inf (never above a pre-spike time).
+// times and per-spike terms e f1, e f2 rounded to float32 (V1(t) is the sum of e f1 over the
+// spikes after t; spikes past the kRK-th are read as float64 suffix sums from S1/S2).
 constexpr int kRK = 8;  // spikes of a row held in its record
 struct alignas(16) RowRec {
-  float t0, s10, s20;  // first spike: time and suffix sums (the one-spike fast path)
+  float t[kRK];   // spike times
+  float d1[kRK];  // per-spike terms e_k f1_k and e_k f2_k (float32)
+  float d2[kRK];
   int ne;
-  float t[kRK];        // spike times, -inf past ne
-  float s1[kRK + 1];   // suffix sums; slot ne holds 0 (when ne <= kRK)
-  float s2[kRK + 1];
-  float pad[2];
+  int pad[7];
 };
 static_assert(sizeof(RowRec) == 128, "RowRec layout");
 
@@ -68,41 +67,39 @@ __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
   int ne = 0;
   while (ne < nk && !isinf(t[row + ne])) ++ne;
   double s1 = 0.0, s2 = 0.0;
-  float r1[kRK + 1], r2[kRK + 1];
+  float r1[kRK], r2[kRK];
 #pragma unroll
-  for (int k = 0; k <= kRK; ++k) r1[k] = r2[k] = 0.f;
+  for (int k = 0; k < kRK; ++k) r1[k] = r2[k] = 0.f;
   for (int k = ne - 1; k >= 0; --k) {
     const double ak = (double)a[row + k], xk = (double)x[row + k], zk = (double)z[row + k];
     const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
     const float f2 = (float)(tau / xk);
     const double ek = (double)errors[row + k];
-    s1 += ek * (double)f1;
-    s2 += ek * (double)f2;
+    const double d1 = ek * (double)f1, d2 = ek * (double)f2;
+    s1 += d1;
+    s2 += d2;
     S1[row + k] = s1;
     S2[row + k] = s2;
 #pragma unroll
     for (int q = 0; q < kRK; ++q) {
       if (q == k) {
-        r1[q] = (float)s1;
-        r2[q] = (float)s2;
+        r1[q] = (float)d1;
+        r2[q] = (float)d2;
       }
     }
   }
   n_eff[lane] = ne;
   if (rec) {
     RowRec r;
-    r.t0 = ne > 0 ? t[row] : -kInfF;
-    r.s10 = r1[0];
-    r.s20 = r2[0];
+#pragma unroll
+    for (int q = 0; q < kRK; ++q) {
+      r.t[q] = q < ne ? t[row + q] : kInfF;
+      r.d1[q] = r1[q];
+      r.d2[q] = r2[q];
+    }
     r.ne = ne;
 #pragma unroll
-    for (int q = 0; q < kRK; ++q) r.t[q] = q < ne ? t[row + q] : -kInfF;
-#pragma unroll
-    for (int q = 0; q <= kRK; ++q) {
-      r.s1[q] = r1[q];
-      r.s2[q] = r2[q];
-    }
-    r.pad[0] = r.pad[1] = 0.f;
+    for (int q = 0; q < 7; ++q) r.pad[q] = 0;
     rec[lane] = r;
   }
 }
@@ -288,26 +285,30 @@ __global__ __launch_bounds__(1024) void pre_err_kernel(
 }
 
 // ------------------------------------------------------------------ wide layers
-// Weight gradient from the row records: lanes are pre neurons, a block owns kRT post
-// neurons and a slice of the batch. The block first copies the records of all its (sample,
-// row) pairs into LDS (one coalesced load, one barrier); the walk over the slice then reads
-// them as wave-uniform LDS broadcasts and touches global memory only for the lanes' own pre
-// spikes (the next sample's slot is loaded while the current one is processed) and for
-// rows with more than kRK spikes. Per row and lane: kk = number of post spikes at or before
-// t_pre, q = es s1[kk] - e s2[kk] in float32 from the float32 record (one compare and a
-// select for one-spike rows), accumulated in float64.
+// Weight gradient from per-spike lists: lanes are pre neurons, a block owns kRT post neurons
+// and a slice of the batch. The block first gathers, for every sample of its slice, the
+// spikes of its rows into an LDS list of (t_post, e f1, e f2, row) entries (empty rows
+// contribute no entry; at most kRK spikes per row). The walk over a sample is then one
+// wave-uniform loop over its entries: q = es e f1 - e e f2 in float32 if t_pre < t_post,
+// added to the float64 accumulator of the entry's row (a register selected by the
+// wave-uniform row index). Spikes of a row past its kRK-th come from the float64 suffix sums
+// in global memory. With one slot per pre neuron, the next sample's pre spike is loaded while
+// the current one is processed; a lane without a spike carries t = +inf and zero exps.
 constexpr int kRT = 16;
 constexpr int kMaxRecSamples = 32;
 
+__host__ __device__ inline size_t list_stage_bytes(int nb) {
+  return (size_t)nb * kRT * kRK * 16 + (size_t)nb * kRT * 8 + (size_t)nb * 8;
+}
+
 template <bool kSingle>
-__global__ __launch_bounds__(1024) void wgrad_rec_kernel(
+__global__ __launch_bounds__(1024) void wgrad_list_kernel(
     int B, int I, int K_pre, int J, int K, int b_per_slice, const float *__restrict__ pre_t,
     const float *__restrict__ pre_es, const float *__restrict__ pre_e,
     const RowRec *__restrict__ rec, const float *__restrict__ post_t,
     const double *__restrict__ S1, const double *__restrict__ S2, int per_sample,
     float *__restrict__ grad, float *__restrict__ partial) {
   extern __shared__ __align__(16) char smem[];
-  RowRec *srec = reinterpret_cast<RowRec *>(smem);  // [b_per_slice][kRT]
   const int tid = threadIdx.x, nth = blockDim.x;
   const int i = blockIdx.x * nth + tid;
   const bool active = i < I;
@@ -316,24 +317,49 @@ __global__ __launch_bounds__(1024) void wgrad_rec_kernel(
   const int jn = min(kRT, J - j0);
   const int b0 = blockIdx.z * b_per_slice;
   const int b1 = min(B, b0 + b_per_slice);
-  // the next pre slot (one slot per pre neuron): its loads overlap the record copy
+  const int nb = b1 - b0;
+  float4 *list = reinterpret_cast<float4 *>(smem);                   // [nb][kRT*kRK]
+  int *s_ne = reinterpret_cast<int *>(list + (size_t)nb * kRT * kRK);  // [nb][kRT]
+  int *s_off = s_ne + nb * kRT;                                        // [nb][kRT]
+  int *s_cnt = s_off + nb * kRT;                                       // [nb]
+  unsigned *s_slow = reinterpret_cast<unsigned *>(s_cnt + nb);         // [nb]
+
+  // the next pre slot (one slot per pre neuron): its loads overlap the list build
   float ntp = kInfF, nes = 0.f, nee = 0.f;
-  if (kSingle && b0 < b1 && active) {
+  if (kSingle && nb > 0 && active) {
     const size_t pb = (size_t)b0 * I + i;
     ntp = pre_t[pb];
     nes = pre_es[pb];
     nee = pre_e[pb];
   }
-  {
-    constexpr int kQ = sizeof(RowRec) / 16;  // 16-byte quads per record
-    const int quads = (b1 - b0) * kRT * kQ;
-    const float4 *src = reinterpret_cast<const float4 *>(rec);
-    float4 *dst = reinterpret_cast<float4 *>(srec);
-    for (int e = tid; e < quads; e += nth) {
-      const int r = e / kQ, s = r / kRT, u = r - s * kRT;
-      float4 v = make_float4(-kInfF, 0.f, 0.f, 0.f);  // ne = 0 (bit pattern of 0.f)
-      if (u < jn) v = src[((size_t)(b0 + s) * J + j0 + u) * kQ + (e - r * kQ)];
-      dst[e] = v;
+  // 1. spike counts of the slice's rows
+  for (int e = tid; e < nb * kRT; e += nth) {
+    const int s = e / kRT, u = e - s * kRT;
+    s_ne[e] = u < jn ? rec[(size_t)(b0 + s) * J + j0 + u].ne : 0;
+  }
+  __syncthreads();
+  // 2. list offsets, counts and the rows with more than kRK spikes
+  for (int e = tid; e < nb * kRT; e += nth) {
+    const int s = e / kRT, u = e - s * kRT;
+    int off = 0;
+    for (int q = 0; q < u; ++q) off += min(s_ne[s * kRT + q], kRK);
+    s_off[e] = off;
+    if (u == kRT - 1) {
+      s_cnt[s] = off + min(s_ne[e], kRK);
+      unsigned m = 0;
+      for (int q = 0; q < kRT; ++q) m |= (s_ne[s * kRT + q] > kRK ? 1u : 0u) << q;
+      s_slow[s] = m;
+    }
+  }
+  __syncthreads();
+  // 3. entries
+  for (int e = tid; e < nb * kRT * kRK; e += nth) {
+    const int r = e / kRK, k = e - r * kRK;
+    const int s = r / kRT, u = r - s * kRT;
+    if (k < min(s_ne[r], kRK)) {
+      const RowRec *rr = rec + (size_t)(b0 + s) * J + j0 + u;
+      list[(size_t)s * kRT * kRK + s_off[r] + k] =
+          make_float4(rr->t[k], rr->d1[k], rr->d2[k], __int_as_float(u));
     }
   }
   __syncthreads();
@@ -342,42 +368,26 @@ __global__ __launch_bounds__(1024) void wgrad_rec_kernel(
 #pragma unroll
   for (int u = 0; u < kRT; ++u) acc[u] = 0.0;
 
-  auto rows = [&](int s, size_t grow0, float tp, float es, float e) {
-    const RowRec *rs = srec + s * kRT;
-    unsigned slow = 0;  // rows with more than kRK spikes (wave-uniform mask)
-#pragma unroll
-    for (int u = 0; u < kRT; ++u) {
-      if (u < jn) {
-        const float4 r0 = *reinterpret_cast<const float4 *>(rs + u);
-        const int ne = __builtin_amdgcn_readfirstlane(__float_as_int(r0.w));
-        if (ne == 1) {
-          const float q0 = fmaf(es, r0.y, -(e * r0.z));
-          acc[u] += (double)((tp < r0.x) ? q0 : 0.f);
-        } else if (ne > 1 && ne <= kRK) {
-          int kk = 0;
-#pragma unroll
-          for (int k = 0; k < kRK; ++k)
-            if (k < ne) kk += (rs[u].t[k] <= tp) ? 1 : 0;
-          const float q = fmaf(es, rs[u].s1[kk], -(e * rs[u].s2[kk]));
-          acc[u] += (double)q;
-        } else if (ne > kRK) {
-          slow |= 1u << u;
-        }
-      }
+  auto walk = [&](int s, size_t grow0, float tp, float es, float e) {
+    const float4 *ls = list + (size_t)s * kRT * kRK;
+    const int n = __builtin_amdgcn_readfirstlane(s_cnt[s]);
+    for (int q = 0; q < n; ++q) {
+      const float4 E = ls[q];
+      const int u = __builtin_amdgcn_readfirstlane(__float_as_int(E.w));
+      const float v = (tp < E.x) ? fmaf(es, E.y, -(e * E.z)) : 0.f;
+      acc[u] += (double)v;
     }
+    unsigned slow = __builtin_amdgcn_readfirstlane(s_slow[s]);
     while (slow) {
       const int u = __builtin_ctz(slow);
       slow &= slow - 1;
-      const int ne = __builtin_amdgcn_readfirstlane(rs[u].ne);
+      const int ne = __builtin_amdgcn_readfirstlane(s_ne[s * kRT + u]);
       const size_t row = grow0 + (size_t)u * K;
-      int kk = 0;
-      for (int k = 0; k < ne; ++k) kk += (post_t[row + k] <= tp) ? 1 : 0;
+      int kk = kRK;  // first spike past the list's with t_post > t_pre
+      for (int k = kRK; k < ne; ++k) kk += (post_t[row + k] <= tp) ? 1 : 0;
       const double v1 = kk < ne ? S1[row + kk] : 0.0;
       const double v2 = kk < ne ? S2[row + kk] : 0.0;
-      const double d = fma((double)es, v1, -((double)e * v2));
-#pragma unroll
-      for (int q = 0; q < kRT; ++q)
-        if (q == u) acc[q] += d;
+      acc[u] += fma((double)es, v1, -((double)e * v2));
     }
   };
 
@@ -394,12 +404,12 @@ __global__ __launch_bounds__(1024) void wgrad_rec_kernel(
       }
       // a slot without a spike has no defined exps: its contribution is forced to zero
       const bool has = isfinite(tp);
-      rows(s, grow0, has ? tp : kInfF, has ? es : 0.f, has ? e : 0.f);
+      walk(s, grow0, has ? tp : kInfF, has ? es : 0.f, has ? e : 0.f);
     } else {
       const size_t pbase = ((size_t)b * I + ic) * K_pre;
       const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
       for (int m = 0; m < np; ++m)
-        rows(s, grow0, pre_t[pbase + m], pre_es[pbase + m], pre_e[pbase + m]);
+        walk(s, grow0, pre_t[pbase + m], pre_es[pbase + m], pre_e[pbase + m]);
     }
     if (per_sample) {
 #pragma unroll
@@ -427,12 +437,16 @@ constexpr int kST = 16;
 
 struct DenseStage {
   double *s1, *s2;            // [R][K+1]; slot ne holds 0
-  float *t, *a, *x, *z, *er;  // [R][K]
+  float *t;                   // [R][KT] (KT = K rounded up to 4: 16-byte rows)
+  float *a, *x, *z, *er;      // [R][K]
   int *nn, *ne;               // [R]
 };
 
+__host__ __device__ inline int stage_row_stride(int K) { return (K + 3) & ~3; }
+
 __host__ __device__ inline size_t dense_stage_bytes(int R, int K) {
-  return (size_t)R * (K + 1) * 16 + (size_t)R * K * 20 + (size_t)R * 8;
+  return (size_t)R * (K + 1) * 16 + (size_t)R * stage_row_stride(K) * 4 +
+         (size_t)R * K * 16 + (size_t)R * 8;
 }
 
 __device__ __forceinline__ DenseStage dense_stage(char *smem, int R, int K) {
@@ -440,13 +454,29 @@ __device__ __forceinline__ DenseStage dense_stage(char *smem, int R, int K) {
   d.s1 = reinterpret_cast<double *>(smem);
   d.s2 = d.s1 + (size_t)R * (K + 1);
   d.t = reinterpret_cast<float *>(d.s2 + (size_t)R * (K + 1));
-  d.a = d.t + (size_t)R * K;
+  d.a = d.t + (size_t)R * stage_row_stride(K);
   d.x = d.a + (size_t)R * K;
   d.z = d.x + (size_t)R * K;
   d.er = d.z + (size_t)R * K;
   d.nn = reinterpret_cast<int *>(d.er + (size_t)R * K);
   d.ne = d.nn + R;
   return d;
+}
+
+// Number of the first nj times of an ascending LDS row (16-byte aligned) that are <= tp:
+// wave-uniform float4 reads, independent of each other.
+__device__ __forceinline__ int count_le(const float *__restrict__ row, int nj, float tp) {
+  const float4 *r4 = reinterpret_cast<const float4 *>(row);
+  int kk = 0;
+#pragma unroll 2
+  for (int q = 0; q < nj; q += 4) {
+    const float4 v = r4[q >> 2];
+    kk += (v.x <= tp) ? 1 : 0;
+    kk += (q + 1 < nj && v.y <= tp) ? 1 : 0;
+    kk += (q + 2 < nj && v.z <= tp) ? 1 : 0;
+    kk += (q + 3 < nj && v.w <= tp) ? 1 : 0;
+  }
+  return kk;
 }
 
 template <bool kPre>
@@ -476,10 +506,20 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
   float wv[kST];
 #pragma unroll
   for (int u = 0; u < kST; ++u) wv[u] = (kPre && active && u < J) ? W[(size_t)u * I + i] : 0.f;
+  // the sample's pre-spike errors start at zero, written coalesced (a lane-per-row zero fill
+  // would touch a separate cache line per lane and store); the barrier below orders these
+  // stores before the values written at the end
+  if (kPre) {
+    const size_t pb0 = (size_t)b * I * K_pre + (size_t)blockIdx.x * nth * K_pre;
+    const int span = min(nth, I - (int)blockIdx.x * nth) * K_pre;
+    for (int e = tid; e < span; e += nth) pre_errors[pb0 + e] = 0.f;
+  }
   // 1. the sample's rows
   const size_t rbase = (size_t)b * J * K;
+  const int KT = stage_row_stride(K);
   for (int e = tid; e < J * K; e += nth) {
-    st.t[e] = t[rbase + e];
+    const int r = e / K;
+    st.t[r * KT + (e - r * K)] = t[rbase + e];
     st.a[e] = a[rbase + e];
     st.x[e] = x[rbase + e];
     st.z[e] = z[rbase + e];
@@ -492,7 +532,7 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
     const int r = tid;
     const int nk = min(st.nn[r], K);
     float *er = st.er + r * K;
-    const float *tr = st.t + r * K, *ar = st.a + r * K, *xr = st.x + r * K, *zr = st.z + r * K;
+    const float *tr = st.t + r * KT, *ar = st.a + r * K, *xr = st.x + r * K, *zr = st.z + r * K;
     double C = 0.0;
     for (int k = nk - 1; k >= 0; --k) {
       const float zk = zr[k];
@@ -541,9 +581,7 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
       if (u < J) {
         const int nj = __builtin_amdgcn_readfirstlane(st.ne[u]);
         if (nj > 0) {
-          const float *tr = st.t + u * K;
-          int kk = 0;
-          for (int k = 0; k < nj; ++k) kk += (tr[k] <= tp) ? 1 : 0;
+          const int kk = count_le(st.t + u * KT, nj, tp);
           const double v1 = st.s1[(size_t)u * (K + 1) + kk];
           const double v2 = st.s2[(size_t)u * (K + 1) + kk];
           acc[u] = fma(es, v1, acc[u]);
@@ -554,8 +592,6 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
     }
     if (kPre) pre_errors[pbase + m] = (float)pacc;
   }
-  if (kPre && active)
-    for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
   if (active) {
 #pragma unroll
     for (int u = 0; u < kST; ++u)
@@ -720,15 +756,15 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   {
     const BwdLayout F = bwd_layout(B, I, K_pre, J, K_post, grad_mode == 1);
     const int gj = (J + kRT - 1) / kRT;
-    auto kern = K_pre == 1 ? wgrad_rec_kernel<true> : wgrad_rec_kernel<false>;
-    const size_t lds = sizeof(RowRec) * (size_t)F.b_per_slice * kRT;
+    auto kern = K_pre == 1 ? wgrad_list_kernel<true> : wgrad_list_kernel<false>;
+    const size_t lds = list_stage_bytes(F.b_per_slice);
     if (int r = allow_lds((const void *)kern, lds)) return r;
-    BATS_TIMED("wgrad_rec_kernel", st,
+    BATS_TIMED("wgrad_list_kernel", st,
                hipLaunchKernelGGL(kern, dim3(F.gi, gj, F.slices), dim3(F.lanes), lds, st, B, I,
                                   K_pre, J, K_post, F.b_per_slice, pre_times, pre_exp_tau_s,
                                   pre_exp_tau, rec, t, S1, S2, grad_mode == 1 ? 1 : 0, grad,
                                   partial));
-    if (int r = check_launch("wgrad_rec_kernel")) return r;
+    if (int r = check_launch("wgrad_list_kernel")) return r;
     if (grad_mode != 1)
       if (int r = reduce_slices(st, F.slices, JI, inv_b, partial, grad)) return r;
   }

@@ -529,19 +529,21 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   char *ws = static_cast<char *>(workspace);
   float *WT = reinterpret_cast<float *>(ws);
   void *ev_ws = ws + align_up(sizeof(float) * (size_t)I * J, 256);
+  const int always_exact = (c > 0.f) ? 0 : 1;
+  const long long BJ = (long long)B * J;
+  const bool wave_mode = g_solver_mode == 2 || (g_solver_mode == 0 && BJ < kSeqMinLanes);
   if (S > 0) {
     BATS_REQUIRE(pre_times && weights && pre_exp_tau_s && pre_exp_tau, "null input pointer");
-    dim3 tg((I + 31) / 32, (J + 31) / 32);
-    hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I);
-    if (int r = check_launch("transpose_kernel")) return r;
+    if (!wave_mode) {  // the wave-per-neuron solver reads W's rows directly
+      dim3 tg((I + 31) / 32, (J + 31) / 32);
+      hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I);
+      if (int r = check_launch("transpose_kernel")) return r;
+    }
   }
   EventLists el;
   if (int r = build_event_lists(st, B, S, K_pre, pre_times, tau_s, tau, c, max_simulation,
                                 pre_exp_tau_s, pre_exp_tau, ev_ws, &el))
     return r;
-  const int always_exact = (c > 0.f) ? 0 : 1;
-  const long long BJ = (long long)B * J;
-  const bool wave_mode = g_solver_mode == 2 || (g_solver_mode == 0 && BJ < kSeqMinLanes);
   if (wave_mode && S > 0) {
     BATS_REQUIRE(BJ < 2147483647LL, "too many (sample, neuron) pairs");
     const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));

@@ -28,7 +28,7 @@ import numpy as np
 
 METRIC = "train samples/s (784-800-10, 30 spk/neuron) vs plain form; p50/p99 serve latency"
 KERNELS = ("lif_solve_seq_kernel", "lif_solve_wave_kernel", "sort_events_kernel",
-           "wgrad_rec_kernel", "post_prep_kernel", "small_bwd_kernel", "pre_err_kernel")
+           "wgrad_list_kernel", "post_prep_kernel", "small_bwd_kernel", "pre_err_kernel")
 HBM_PEAK_GBS = 8000.0
 
 # network of the workload
@@ -88,7 +88,11 @@ def gpu_bench(args):
     n_d = torch.as_tensor(n, device=dev)
     lab_d = torch.as_tensor(labels, device=dev)
     loss_fct = SpikeCountClassLoss(target_true=TARGET_TRUE, target_false=TARGET_FALSE)
-    opt = AdamOptimizer(learning_rate=LR, capturable=True)
+    # stream launches keep Adam's step counter on the host (as the reference does); the
+    # captured graph needs it on the device
+    opt_stream = AdamOptimizer(learning_rate=LR)
+    opt_graph = AdamOptimizer(learning_rate=LR, capturable=True)
+    opt = opt_stream
 
     def step():
         net.reset()
@@ -133,6 +137,7 @@ def gpu_bench(args):
     eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     # graph capture of the whole step
+    opt = opt_graph
     s = torch.cuda.Stream(device=dev)
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):

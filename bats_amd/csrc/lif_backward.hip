@@ -7,19 +7,20 @@
 // Regrouping the reference's per-post-spike sums gives
 //   dW[j,i]      = sum_b sum_m  es_m V1_j(t_m) - e_m V2_j(t_m)       (m: spikes of pre i)
 //   pre_err[b,i,m] = sum_j W[j,i] (V1_j(t_m) es_m / tau_s - V2_j(t_m) e_m / tau)
-// so the backward is three kernels:
-//   1. post_prep_kernel  one lane per (sample, post neuron): the reset-chain error scan
+// so the backward of a wide layer is three kernels:
+//   1. row_prep_kernel   one lane per (sample, post neuron): the reset-chain error scan
 //      (propagate_recurrent_errors.cu:17-22, written back in place like the reference),
 //      the factors f1, f2 (compute_factors.cu:20-30, rounded to float32 as stored there),
-//      and the float64 suffix sums S1[k] = sum_{k'>=k} e f1, S2[k] = sum_{k'>=k} e f2.
-//      V_j(t) is then S[first k with t_jk > t].
-//   2. wgrad_kernel      lanes over pre neurons i, a tile of post neurons j, a slice of the
-//      batch; the post rows of a few samples at a time are staged in LDS (wave-uniform
-//      reads), the lookup counts the post spikes at or before t_m; float64 accumulation;
-//      a deterministic fixed-order reduction over batch slices (wgrad_reduce_kernel)
-//      finishes the mean.
+//      and per spike the terms d1 = e f1, d2 = e f2 (plus, when pre-spike errors are
+//      wanted, the float64 suffix sums S1[k] = sum_{k'>=k} d1, S2[k] = sum_{k'>=k} d2, so
+//      that V_j(t) = S[first k with t_jk > t]).
+//   2. wgrad_tile_kernel lanes over pre neurons i, a tile of 16 post neurons, a slice of the
+//      batch: the terms of the post spikes later than the lane's pre spike, float64
+//      accumulation per row; a deterministic fixed-order reduction over batch slices
+//      (wgrad_reduce_kernel) finishes the mean.
 //   3. pre_err_kernel    lanes over pre neurons i of one sample, post rows staged in LDS
 //      a tile at a time.
+// Layers of at most 16 post neurons (an output layer) run all of it in small_bwd_kernel.
 #include "common.hpp"
 
 #include <algorithm>
@@ -28,48 +29,142 @@ namespace bats {
 
 constexpr int kBwdThreads = 256;
 
-// Compact record of a post row for the weight gradient of wide layers: its first kRK spike
-// times and per-spike terms e f1, e f2 rounded to float32 (V1(t) is the sum of e f1 over the
-// spikes after t; spikes past the kRK-th are read as float64 suffix sums from S1/S2).
-constexpr int kRK = 8;  // spikes of a row held in its record
-struct alignas(16) RowRec {
-  float t[kRK];   // spike times
-  float d1[kRK];  // per-spike terms e_k f1_k and e_k f2_k (float32)
-  float d2[kRK];
-  int ne;
-  int pad[7];
-};
-static_assert(sizeof(RowRec) == 128, "RowRec layout");
+// One lane per post row (sample b, neuron j): the reset-chain error scan
+// (propagate_recurrent_errors.cu:17-22, written back in place like the reference), the
+// factors f1, f2 (compute_factors.cu:20-30, rounded to float32 as stored there) and the
+// per-spike weight-gradient terms d1 = e f1, d2 = e f2. Outputs (each optional):
+//   n_eff [B*J]      spikes before the row's first infinite time
+//   S1, S2 [B*J*K]   float64 suffix sums of d1, d2 (the wide pre-spike error kernel)
+//   rec              one record of tile_rec_slots(K) float4 per (sample, tile of kWT rows):
+//                      slots 0..15   the first spike of row u as {t, d1, d2, 0}, or
+//                                    {-inf, 0, 0, 0} for an empty row (it never counts)
+//                      slot 16       x: mask of non-empty rows | mask of rows with more
+//                                    than one spike << 16;  y: number of tail spikes
+//                      slots 17..20  per row u (uint32 u of the 16): tail count << 16 |
+//                                    tail offset
+//                      slots 21..    the later spikes of row 0, row 1, ... (the tails)
+// Lanes are (sample, tile, row in tile), so the kWT rows of a tile are kWT consecutive lanes
+// of one wave and a record's layout comes from 16-lane prefix sums and ballots.
+// Rows of up to kPrepFast spikes (all but a few in 10^4 at the 784-800-10 shape) load their
+// slots with independent loads into registers, so that a row costs one memory round trip
+// instead of one per spike; longer rows run the same arithmetic from global memory.
+// apply_reset = 0: the errors already hold the reset-chain result.
+constexpr int kPrepFast = 8;
+constexpr int kWT = 16;  // post rows per weight-gradient tile
 
-// apply_reset = 0: the errors already hold the reset-chain result, only the suffix sums are
-// formed. rec (optional) receives the row records.
-__global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
-    int BJ, int K, const int *__restrict__ n, const float *__restrict__ t,
+constexpr int kRecMask = 16;   // header slot: row masks and the tail total
+constexpr int kRecInfo = 17;   // first of the 4 slots of per-row tail counts and offsets
+constexpr int kRecHead = 21;   // first tail slot
+
+__host__ __device__ inline int tile_rec_slots(int K) { return kRecHead + kWT * (K - 1); }
+
+__global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
+    int B, int J, int K, const int *__restrict__ n, const float *__restrict__ t,
     const float *__restrict__ a, const float *__restrict__ x, const float *__restrict__ z,
     float *__restrict__ errors, double c, double tau, double dtt, int apply_reset,
     int *__restrict__ n_eff, double *__restrict__ S1, double *__restrict__ S2,
-    RowRec *__restrict__ rec) {
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lane >= BJ) return;
-  const size_t row = (size_t)lane * K;
-  const int nk = min(n[lane], K);
-  // reset chain, reverse order (propagate_recurrent_errors.cu:17-22)
-  double C = 0.0;
-  for (int k = apply_reset ? nk - 1 : -1; k >= 0; --k) {
-    const float zk = z[row + k];
-    if (isinf(zk)) continue;
-    const double ek = (double)errors[row + k] + C * (double)zk;
-    const float ek32 = (float)ek;
-    errors[row + k] = ek32;
-    C = C + (double)ek32 * dtt / (double)x[row + k];
-  }
-  // spikes used by the gradient: those before the first infinite time
-  int ne = 0;
-  while (ne < nk && !isinf(t[row + ne])) ++ne;
-  double s1 = 0.0, s2 = 0.0;
-  float r1[kRK], r2[kRK];
+    float4 *__restrict__ rec) {
+  const int gj = (J + kWT - 1) / kWT;
+  const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tiles = (int64_t)B * gj;  // lane < tiles * kWT for every launched lane group
+  const int64_t tile = lane / kWT;
+  const int u = (int)(lane % kWT);
+  const int b = (int)(tile / gj);
+  const int j = (int)(tile % gj) * kWT + u;
+  const bool valid = tile < tiles;
+  const bool active = valid && j < J;
+  const size_t r = active ? (size_t)b * J + j : 0;
+  const size_t row = r * K;
+  const int nk = active ? min(n[r], K) : 0;
+  const bool fast = nk <= kPrepFast;
+
+  float tv[kPrepFast], av[kPrepFast], xv[kPrepFast], zv[kPrepFast], ev[kPrepFast];
 #pragma unroll
-  for (int k = 0; k < kRK; ++k) r1[k] = r2[k] = 0.f;
+  for (int k = 0; k < kPrepFast; ++k) {
+    const bool in = fast && k < nk;
+    tv[k] = in ? t[row + k] : kInfF;
+    av[k] = in ? a[row + k] : 1.f;
+    xv[k] = in ? x[row + k] : 1.f;
+    zv[k] = in ? z[row + k] : kInfF;
+    ev[k] = in ? errors[row + k] : 0.f;
+  }
+  int ne = 0;
+  if (fast) {
+    if (apply_reset) {
+      double C = 0.0;
+#pragma unroll
+      for (int k = kPrepFast - 1; k >= 0; --k) {
+        if (k < nk && !isinf(zv[k])) {
+          const double ek = (double)ev[k] + C * (double)zv[k];
+          ev[k] = (float)ek;
+          C = C + (double)ev[k] * dtt / (double)xv[k];
+          errors[row + k] = ev[k];
+        }
+      }
+    }
+    bool run = true;
+#pragma unroll
+    for (int k = 0; k < kPrepFast; ++k) {
+      run = run & (k < nk) & !isinf(tv[k]);
+      ne += run ? 1 : 0;
+    }
+  } else {
+    double C = 0.0;
+    for (int k = apply_reset ? nk - 1 : -1; k >= 0; --k) {
+      const float zk = z[row + k];
+      if (isinf(zk)) continue;
+      const double ek = (double)errors[row + k] + C * (double)zk;
+      const float ek32 = (float)ek;
+      errors[row + k] = ek32;
+      C = C + (double)ek32 * dtt / (double)x[row + k];
+    }
+    while (ne < nk && !isinf(t[row + ne])) ++ne;
+  }
+  if (active && n_eff) n_eff[r] = ne;
+  // the row's tail (spikes after the first) in its tile record
+  const int tc = ne > 1 ? ne - 1 : 0;
+  int incl = tc;
+#pragma unroll
+  for (int d = 1; d < kWT; d <<= 1) {
+    const int y = __shfl_up(incl, d, kWT);
+    if (u >= d) incl += y;
+  }
+  const int toff = incl - tc;
+  const int ntail = __shfl(incl, kWT - 1, kWT);
+  const int grp = (threadIdx.x & 63) / kWT;
+  const unsigned nz = (unsigned)(__ballot(ne > 0) >> (grp * kWT)) & 0xffffu;
+  const unsigned tl = (unsigned)(__ballot(ne > 1) >> (grp * kWT)) & 0xffffu;
+  float4 *tr = rec ? rec + (size_t)tile * tile_rec_slots(K) : nullptr;
+  if (tr && valid) {
+    if (u == 0) tr[kRecMask] = make_float4(__uint_as_float(nz | (tl << 16)), __int_as_float(ntail),
+                                           0.f, 0.f);
+    reinterpret_cast<unsigned *>(tr + kRecInfo)[u] = ((unsigned)tc << 16) | (unsigned)toff;
+    if (ne == 0) tr[u] = make_float4(-kInfF, 0.f, 0.f, 0.f);
+  }
+  if (!active) return;
+  // slot of spike k: k = 0 -> u, k >= 1 -> kRecHead + toff + k - 1
+  float4 *ent0 = tr ? tr + u : nullptr;
+  float4 *entt = tr ? tr + kRecHead + toff - 1 : nullptr;
+  double s1 = 0.0, s2 = 0.0;
+  if (fast) {
+#pragma unroll
+    for (int k = kPrepFast - 1; k >= 0; --k) {
+      if (k < ne) {
+        const double ak = (double)av[k], xk = (double)xv[k], zk = (double)zv[k];
+        const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
+        const float f2 = (float)(tau / xk);
+        const double d1 = (double)ev[k] * (double)f1, d2 = (double)ev[k] * (double)f2;
+        s1 += d1;
+        s2 += d2;
+        if (S1) {
+          S1[row + k] = s1;
+          S2[row + k] = s2;
+        }
+        if (tr) (k == 0 ? ent0 : entt + k)[0] = make_float4(tv[k], (float)d1, (float)d2, 0.f);
+      }
+    }
+    return;
+  }
   for (int k = ne - 1; k >= 0; --k) {
     const double ak = (double)a[row + k], xk = (double)x[row + k], zk = (double)z[row + k];
     const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
@@ -78,30 +173,24 @@ __global__ __launch_bounds__(kBwdThreads) void post_prep_kernel(
     const double d1 = ek * (double)f1, d2 = ek * (double)f2;
     s1 += d1;
     s2 += d2;
-    S1[row + k] = s1;
-    S2[row + k] = s2;
-#pragma unroll
-    for (int q = 0; q < kRK; ++q) {
-      if (q == k) {
-        r1[q] = (float)d1;
-        r2[q] = (float)d2;
-      }
+    if (S1) {
+      S1[row + k] = s1;
+      S2[row + k] = s2;
     }
+    if (tr) (k == 0 ? ent0 : entt + k)[0] = make_float4(t[row + k], (float)d1, (float)d2, 0.f);
   }
-  n_eff[lane] = ne;
-  if (rec) {
-    RowRec r;
-#pragma unroll
-    for (int q = 0; q < kRK; ++q) {
-      r.t[q] = q < ne ? t[row + q] : kInfF;
-      r.d1[q] = r1[q];
-      r.d2[q] = r2[q];
-    }
-    r.ne = ne;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) r.pad[q] = 0;
-    rec[lane] = r;
-  }
+}
+
+static int launch_row_prep(hipStream_t st, int B, int J, int K, const int *n, const float *t,
+                           const float *a, const float *x, const float *z, float *errors,
+                           double c, double tau, double dtt, int *n_eff, double *S1,
+                           double *S2, float4 *rec) {
+  const int64_t lanes = (int64_t)B * ((J + kWT - 1) / kWT) * kWT;
+  if (lanes <= 0) return 0;
+  hipLaunchKernelGGL(row_prep_kernel, dim3((unsigned)((lanes + kBwdThreads - 1) / kBwdThreads)),
+                     dim3(kBwdThreads), 0, st, B, J, K, n, t, a, x, z, errors, c, tau, dtt, 1,
+                     n_eff, S1, S2, rec);
+  return check_launch("row_prep_kernel");
 }
 
 // Post-neuron rows staged in LDS for SB samples at a time: spike counts, times and suffix
@@ -285,115 +374,123 @@ __global__ __launch_bounds__(1024) void pre_err_kernel(
 }
 
 // ------------------------------------------------------------------ wide layers
-// Weight gradient from per-spike lists: lanes are pre neurons, a block owns kRT post neurons
-// and a slice of the batch. The block first gathers, for every sample of its slice, the
-// spikes of its rows into an LDS list of (t_post, e f1, e f2, row) entries (empty rows
-// contribute no entry; at most kRK spikes per row). The walk over a sample is then one
-// wave-uniform loop over its entries: q = es e f1 - e e f2 in float32 if t_pre < t_post,
-// added to the float64 accumulator of the entry's row (a register selected by the
-// wave-uniform row index). Spikes of a row past its kRK-th come from the float64 suffix sums
-// in global memory. With one slot per pre neuron, the next sample's pre spike is loaded while
-// the current one is processed; a lane without a spike carries t = +inf and zero exps.
-constexpr int kRT = 16;
-constexpr int kMaxRecSamples = 32;
+// Weight gradient: one wave per (block of 64 pre neurons, tile of kWT post neurons, slice of
+// the batch); lane = pre neuron i. For each sample of the slice and each pre spike (t_p, es,
+// e) of the lane, every spike k of every row u of the tile adds
+//   q = es d1_k - e d2_k   if t_p < t_k      (d = the row_prep_kernel terms)
+// to the float64 accumulator of row u, summing V1(t_p) es - V2(t_p) e term by term. The row
+// code is unrolled, so each row has its own register accumulator. A row costs one bit test
+// of the record's non-empty mask when it is empty (more than half of a hidden layer's rows),
+// its first spike sits at a fixed slot, and only rows with more than one spike (about one in
+// eight) run a loop over their tail. q is formed in float32 (the terms are float32) and added
+// in float64.
+//
+// The sample's tile record is fetched one sample ahead with one coalesced load of its first
+// 64 slots, parked in the wave's own LDS window and read back with wave-uniform (broadcast)
+// LDS reads; a sample whose tails overrun the window reads them from global memory.
+//
+// Block order: the workgroups of one (tile, slice) read the same records, so consecutive
+// logical blocks are placed on one XCD (workgroups are dealt to the 8 XCDs round robin) and
+// share its L2.
+constexpr int kRecWindow = 64;
 
-__host__ __device__ inline size_t list_stage_bytes(int nb) {
-  return (size_t)nb * kRT * kRK * 16 + (size_t)nb * kRT * 8 + (size_t)nb * 8;
+__device__ __forceinline__ int xcd_logical_block(int p, int nblk) {
+  constexpr int kXcd = 8;
+  const int q = nblk / kXcd, r = nblk % kXcd;
+  const int xcd = p % kXcd, k = p / kXcd;
+  return xcd * q + min(xcd, r) + k;
+}
+
+// A workgroup holds kTileWaves independent waves (consecutive logical tasks, each with its
+// own LDS window; no workgroup barrier), so that few workgroups carry many waves.
+constexpr int kTileWaves = 4;
+
+// LDS ordering inside one wave: its DS instructions execute in order, so a fence at
+// wavefront scope (which also keeps the compiler from moving LDS accesses across it) is all
+// that a wave-private window needs.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <bool kSingle>
-__global__ __launch_bounds__(1024) void wgrad_list_kernel(
-    int B, int I, int K_pre, int J, int K, int b_per_slice, const float *__restrict__ pre_t,
-    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
-    const RowRec *__restrict__ rec, const float *__restrict__ post_t,
-    const double *__restrict__ S1, const double *__restrict__ S2, int per_sample,
+__global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
+    int B, int I, int K_pre, int J, int K, int gi, int gj, int b_per_slice, int ntasks,
+    const float *__restrict__ pre_t, const float *__restrict__ pre_es,
+    const float *__restrict__ pre_e, const float4 *__restrict__ rec, int per_sample,
     float *__restrict__ grad, float *__restrict__ partial) {
-  extern __shared__ __align__(16) char smem[];
-  const int tid = threadIdx.x, nth = blockDim.x;
-  const int i = blockIdx.x * nth + tid;
+  __shared__ float4 wins[kTileWaves][kRecWindow];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+  const int L = xcd_logical_block(blockIdx.x, gridDim.x) * kTileWaves + wv;
+  if (L >= ntasks) return;  // whole wave; no workgroup barrier follows
+  float4 *win = wins[wv];
+  const int bi = L % gi;
+  const int tj = (L / gi) % gj;
+  const int sl = L / (gi * gj);
+  const int lane = threadIdx.x & 63;
+  const int i = bi * 64 + lane;
   const bool active = i < I;
   const int ic = active ? i : 0;
-  const int j0 = blockIdx.y * kRT;
-  const int jn = min(kRT, J - j0);
-  const int b0 = blockIdx.z * b_per_slice;
+  const int j0 = tj * kWT;
+  const int jn = min(kWT, J - j0);
+  const int b0 = sl * b_per_slice;
   const int b1 = min(B, b0 + b_per_slice);
-  const int nb = b1 - b0;
-  float4 *list = reinterpret_cast<float4 *>(smem);                   // [nb][kRT*kRK]
-  int *s_ne = reinterpret_cast<int *>(list + (size_t)nb * kRT * kRK);  // [nb][kRT]
-  int *s_off = s_ne + nb * kRT;                                        // [nb][kRT]
-  int *s_cnt = s_off + nb * kRT;                                       // [nb]
-  unsigned *s_slow = reinterpret_cast<unsigned *>(s_cnt + nb);         // [nb]
+  const int R = tile_rec_slots(K);
+  const int wn = min(kRecWindow, R);  // slots of a record held in the window
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
 
-  // the next pre slot (one slot per pre neuron): its loads overlap the list build
-  float ntp = kInfF, nes = 0.f, nee = 0.f;
-  if (kSingle && nb > 0 && active) {
-    const size_t pb = (size_t)b0 * I + i;
-    ntp = pre_t[pb];
-    nes = pre_es[pb];
-    nee = pre_e[pb];
-  }
-  // 1. spike counts of the slice's rows
-  for (int e = tid; e < nb * kRT; e += nth) {
-    const int s = e / kRT, u = e - s * kRT;
-    s_ne[e] = u < jn ? rec[(size_t)(b0 + s) * J + j0 + u].ne : 0;
-  }
-  __syncthreads();
-  // 2. list offsets, counts and the rows with more than kRK spikes
-  for (int e = tid; e < nb * kRT; e += nth) {
-    const int s = e / kRT, u = e - s * kRT;
-    int off = 0;
-    for (int q = 0; q < u; ++q) off += min(s_ne[s * kRT + q], kRK);
-    s_off[e] = off;
-    if (u == kRT - 1) {
-      s_cnt[s] = off + min(s_ne[e], kRK);
-      unsigned m = 0;
-      for (int q = 0; q < kRT; ++q) m |= (s_ne[s * kRT + q] > kRK ? 1u : 0u) << q;
-      s_slow[s] = m;
-    }
-  }
-  __syncthreads();
-  // 3. entries
-  for (int e = tid; e < nb * kRT * kRK; e += nth) {
-    const int r = e / kRK, k = e - r * kRK;
-    const int s = r / kRT, u = r - s * kRT;
-    if (k < min(s_ne[r], kRK)) {
-      const RowRec *rr = rec + (size_t)(b0 + s) * J + j0 + u;
-      list[(size_t)s * kRT * kRK + s_off[r] + k] =
-          make_float4(rr->t[k], rr->d1[k], rr->d2[k], __int_as_float(u));
-    }
-  }
-  __syncthreads();
-
-  double acc[kRT];
+  double acc[kWT];
 #pragma unroll
-  for (int u = 0; u < kRT; ++u) acc[u] = 0.0;
+  for (int u = 0; u < kWT; ++u) acc[u] = 0.0;
 
-  auto walk = [&](int s, size_t grow0, float tp, float es, float e) {
-    const float4 *ls = list + (size_t)s * kRT * kRK;
-    const int n = __builtin_amdgcn_readfirstlane(s_cnt[s]);
-    for (int q = 0; q < n; ++q) {
-      const float4 E = ls[q];
-      const int u = __builtin_amdgcn_readfirstlane(__float_as_int(E.w));
-      const float v = (tp < E.x) ? fmaf(es, E.y, -(e * E.z)) : 0.f;
-      acc[u] += (double)v;
+  // the next sample's record window and pre slot, loaded during the current walk
+  float4 nrec = zero4;
+  float ntp = kInfF, nes = 0.f, nee = 0.f;
+  if (b0 < b1) {
+    if (lane < wn) nrec = rec[((size_t)b0 * gj + tj) * R + lane];
+    if (kSingle && active) {
+      const size_t pb = (size_t)b0 * I + i;
+      ntp = pre_t[pb];
+      nes = pre_es[pb];
+      nee = pre_e[pb];
     }
-    unsigned slow = __builtin_amdgcn_readfirstlane(s_slow[s]);
-    while (slow) {
-      const int u = __builtin_ctz(slow);
-      slow &= slow - 1;
-      const int ne = __builtin_amdgcn_readfirstlane(s_ne[s * kRT + u]);
-      const size_t row = grow0 + (size_t)u * K;
-      int kk = kRK;  // first spike past the list's with t_post > t_pre
-      for (int k = kRK; k < ne; ++k) kk += (post_t[row + k] <= tp) ? 1 : 0;
-      const double v1 = kk < ne ? S1[row + kk] : 0.0;
-      const double v2 = kk < ne ? S2[row + kk] : 0.0;
-      acc[u] += fma((double)es, v1, -((double)e * v2));
-    }
-  };
-
+  }
   for (int b = b0; b < b1; ++b) {
-    const int s = b - b0;
-    const size_t grow0 = ((size_t)b * J + j0) * K;
+    const float4 *grec = rec + ((size_t)b * gj + tj) * R;
+    const unsigned masks = __builtin_amdgcn_readlane(__float_as_uint(nrec.x), kRecMask);
+    const int ntail = __builtin_amdgcn_readlane(__float_as_int(nrec.y), kRecMask);
+    wave_lds_fence();  // the previous sample's window reads are done
+    win[lane] = nrec;
+    wave_lds_fence();
+    if (b + 1 < b1 && lane < wn) nrec = grec[(size_t)gj * R + lane];
+    const bool tails_in_window = kRecHead + ntail <= wn;
+    const unsigned *info = reinterpret_cast<const unsigned *>(win + kRecInfo);
+    auto walk = [&](float tp, float es, float e) {
+      auto term = [&](int u, const float4 E) {
+        float q = fmaf(es, E.y, -(e * E.z));
+        // formed for every lane: an opaque barrier keeps the compiler from turning the
+        // select into a branch around the term
+        asm volatile("" : "+v"(q));
+        q = (tp < E.x) ? q : 0.f;
+        acc[u] += (double)q;
+      };
+#pragma unroll
+      for (int u = 0; u < kWT; ++u) {
+        if (masks & (1u << u)) {
+          term(u, win[u]);
+          if (masks & (0x10000u << u)) {
+            const unsigned inf = __builtin_amdgcn_readfirstlane(info[u]);
+            const int c = (int)(inf >> 16), o = kRecHead + (int)(inf & 0xffffu);
+            if (tails_in_window) {
+              for (int k = 0; k < c; ++k) term(u, win[o + k]);
+            } else {
+              for (int k = 0; k < c; ++k) term(u, grec[o + k]);
+            }
+          }
+        }
+      }
+    };
     if (kSingle) {
       const float tp = ntp, es = nes, e = nee;
       if (b + 1 < b1 && active) {
@@ -404,16 +501,15 @@ __global__ __launch_bounds__(1024) void wgrad_list_kernel(
       }
       // a slot without a spike has no defined exps: its contribution is forced to zero
       const bool has = isfinite(tp);
-      walk(s, grow0, has ? tp : kInfF, has ? es : 0.f, has ? e : 0.f);
+      walk(has ? tp : kInfF, has ? es : 0.f, has ? e : 0.f);
     } else {
       const size_t pbase = ((size_t)b * I + ic) * K_pre;
       const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
-      for (int m = 0; m < np; ++m)
-        walk(s, grow0, pre_t[pbase + m], pre_es[pbase + m], pre_e[pbase + m]);
+      for (int m = 0; m < np; ++m) walk(pre_t[pbase + m], pre_es[pbase + m], pre_e[pbase + m]);
     }
     if (per_sample) {
 #pragma unroll
-      for (int u = 0; u < kRT; ++u) {
+      for (int u = 0; u < kWT; ++u) {
         if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
         acc[u] = 0.0;
       }
@@ -421,8 +517,8 @@ __global__ __launch_bounds__(1024) void wgrad_list_kernel(
   }
   if (!per_sample && active) {
 #pragma unroll
-    for (int u = 0; u < kRT; ++u)
-      if (u < jn) partial[((size_t)blockIdx.z * J + j0 + u) * I + i] = (float)acc[u];
+    for (int u = 0; u < kWT; ++u)
+      if (u < jn) partial[((size_t)sl * J + j0 + u) * I + i] = (float)acc[u];
   }
 }
 
@@ -605,7 +701,12 @@ struct BwdLayout {
   bool small;
 };
 
-static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_sample = false) {
+// Waves the weight-gradient grid aims at: about six per SIMD of the 256-CU device, so that
+// the scalar-load latency of one wave is covered by the others and the last round is short.
+constexpr int kWgradTargetWaves = 6144;
+
+static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_sample = false,
+                            bool want_pre = true) {
   BwdLayout L{};
   L.gi = std::max(1, (I + 1023) / 1024);
   const int per_block = (I + L.gi - 1) / L.gi;
@@ -615,31 +716,31 @@ static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_samp
     L.slices = std::max(B, 1);
     L.b_per_slice = 1;
   } else {
-    const int gj = (J + kRT - 1) / kRT;
-    const int base = std::max(1, L.gi * gj);
-    // about 512 blocks (two per CU), rounded down so that the last round is not a tail; a
-    // block's record stage holds at most kMaxRecSamples samples (64 KB of LDS)
-    int slices = per_sample ? B : std::max(1, std::min(B, 512 / base));
-    slices = std::min(std::max(B, 1), std::max(slices, (B + kMaxRecSamples - 1) / kMaxRecSamples));
+    const int waves = std::max(1, ((I + 63) / 64) * ((J + kWT - 1) / kWT));
+    int slices = per_sample ? B : (kWgradTargetWaves + waves - 1) / waves;
+    slices = std::max(1, std::min(std::max(B, 1), slices));
     L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
     L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
   }
   // rows prepared in global memory: the wide path, and the error write-back of wide
   // narrow-layer launches (more than one block of pre neurons)
   const bool rows = !L.small || L.gi > 1;
+  const bool sums = !L.small && J > kWJT && want_pre;  // suffix sums for pre_err_kernel
   size_t off = 0;
   L.neff = off;
-  if (rows) off = align_up(off + sizeof(int) * (size_t)B * J, 256);
+  if (rows) off = align_up(off + sizeof(int) * ((size_t)B * J + kWT), 256);
   L.s1 = off;
-  if (rows) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  if (sums) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
   L.s2 = off;
-  if (rows) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
+  if (sums) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
   L.rec = off;
-  if (rows) off = align_up(off + sizeof(RowRec) * (size_t)B * J, 256);
+  if (!L.small)
+    off = align_up(off + sizeof(float4) * (size_t)B * ((J + kWT - 1) / kWT) * tile_rec_slots(K),
+                   256);
   L.partial = off;
   off = align_up(off + sizeof(float) * (size_t)L.slices * J * I, 256);
   L.scratch = off;
-  if (J > kWJT) off = align_up(off + sizeof(double) * (size_t)B * I * K_pre, 256);
+  if (sums) off = align_up(off + sizeof(double) * (size_t)B * I * K_pre, 256);
   L.total = off;
   return L;
 }
@@ -656,11 +757,8 @@ static int allow_lds(const void *kernel, size_t bytes) {
 int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const float *a,
               const float *x, const float *z, float *errors, float c, float tau, float dtt,
               int *n_eff, double *S1, double *S2) {
-  if (BJ <= 0) return 0;
-  hipLaunchKernelGGL(post_prep_kernel, dim3((BJ + kBwdThreads - 1) / kBwdThreads),
-                     dim3(kBwdThreads), 0, st, BJ, K, n, t, a, x, z, errors, (double)c,
-                     (double)tau, (double)dtt, 1, n_eff, S1, S2, (RowRec *)nullptr);
-  return check_launch("post_prep_kernel");
+  return launch_row_prep(st, 1, BJ, K, n, t, a, x, z, errors, (double)c, (double)tau,
+                         (double)dtt, n_eff, S1, S2, nullptr);
 }
 
 }  // namespace bats
@@ -672,7 +770,8 @@ extern "C" size_t bats_lif_backward_workspace_size(int B, int I, int K_pre, int 
   return bwd_layout(B, I, K_pre, J, K_post).total;
 }
 
-// Largest post-spike buffer the backward's LDS row stage supports.
+// Largest post-spike buffer the backward supports (LDS row stages of the narrow-layer and
+// pre-spike error kernels).
 constexpr int kMaxBackwardK = 384;
 
 extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, int K_post,
@@ -707,20 +806,15 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   int *n_eff = reinterpret_cast<int *>(ws + L.neff);
   double *S1 = reinterpret_cast<double *>(ws + L.s1);
   double *S2 = reinterpret_cast<double *>(ws + L.s2);
-  RowRec *rec = reinterpret_cast<RowRec *>(ws + L.rec);
+  float4 *rec = reinterpret_cast<float4 *>(ws + L.rec);
   float *partial = reinterpret_cast<float *>(ws + L.partial);
   double *scratch = reinterpret_cast<double *>(ws + L.scratch);
-  const int BJ = B * J;
-  const unsigned prep_blocks = (unsigned)((BJ + kBwdThreads - 1) / kBwdThreads);
-  if (JI == 0) {  // no weights: only the reset chain of the errors
-    if (BJ > 0) {
-      hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st, BJ,
-                         K_post, n, t, a, x, z, errors, (double)c, (double)tau,
-                         (double)delta_theta_tau, 1, n_eff, S1, S2, (RowRec *)nullptr);
-      return check_launch("post_prep_kernel");
-    }
-    return 0;
-  }
+  // the reset chain alone, written back in place
+  auto reset_only = [&]() -> int {
+    return launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c, (double)tau,
+                           (double)delta_theta_tau, nullptr, nullptr, nullptr, nullptr);
+  };
+  if (JI == 0) return reset_only();  // no weights: only the reset chain of the errors
   const bool want_pre = pre_errors && K_pre > 0;
   if (want_pre) BATS_REQUIRE(weights, "null weights");
   BATS_REQUIRE(K_pre == 0 || (pre_times && pre_exp_tau_s && pre_exp_tau), "null pre pointer");
@@ -739,32 +833,33 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
     if (int r = check_launch("small_bwd_kernel")) return r;
     if (grad_mode != 1)
       if (int r = reduce_slices(st, B, JI, inv_b, partial, grad)) return r;
-    if (L.gi > 1) {  // every block read the raw errors; write the reset chain back now
-      hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st, BJ,
-                         K_post, n, t, a, x, z, errors, (double)c, (double)tau,
-                         (double)delta_theta_tau, 1, n_eff, S1, S2, (RowRec *)nullptr);
-      if (int r = check_launch("post_prep_kernel")) return r;
-    }
+    // with several blocks of pre neurons every block read the raw errors; write the reset
+    // chain back now
+    if (L.gi > 1) return reset_only();
     return 0;
   }
 
-  BATS_TIMED("post_prep_kernel", st,
-             hipLaunchKernelGGL(post_prep_kernel, dim3(prep_blocks), dim3(kBwdThreads), 0, st,
-                                BJ, K_post, n, t, a, x, z, errors, (double)c, (double)tau,
-                                (double)delta_theta_tau, 1, n_eff, S1, S2, rec));
-  if (int r = check_launch("post_prep_kernel")) return r;
+  const bool sums = J > kWJT && want_pre;
+  int prep_rc = 0;
+  BATS_TIMED("row_prep_kernel", st,
+             prep_rc = launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c,
+                                       (double)tau, (double)delta_theta_tau, n_eff,
+                                       sums ? S1 : nullptr, sums ? S2 : nullptr, rec));
+  if (prep_rc) return prep_rc;
   {
     const BwdLayout F = bwd_layout(B, I, K_pre, J, K_post, grad_mode == 1);
-    const int gj = (J + kRT - 1) / kRT;
-    auto kern = K_pre == 1 ? wgrad_list_kernel<true> : wgrad_list_kernel<false>;
-    const size_t lds = list_stage_bytes(F.b_per_slice);
-    if (int r = allow_lds((const void *)kern, lds)) return r;
-    BATS_TIMED("wgrad_list_kernel", st,
-               hipLaunchKernelGGL(kern, dim3(F.gi, gj, F.slices), dim3(F.lanes), lds, st, B, I,
-                                  K_pre, J, K_post, F.b_per_slice, pre_times, pre_exp_tau_s,
-                                  pre_exp_tau, rec, t, S1, S2, grad_mode == 1 ? 1 : 0, grad,
+    const int gi = (I + 63) / 64, gj = (J + kWT - 1) / kWT;
+    const int64_t ntasks = (int64_t)gi * gj * F.slices;
+    BATS_REQUIRE(ntasks < (int64_t)1 << 31, "weight-gradient grid too large");
+    const int64_t nblk = (ntasks + kTileWaves - 1) / kTileWaves;
+    auto kern = K_pre == 1 ? wgrad_tile_kernel<true> : wgrad_tile_kernel<false>;
+    BATS_TIMED("wgrad_tile_kernel", st,
+               hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(64 * kTileWaves), 0, st, B,
+                                  I, K_pre, J, K_post, gi, gj, F.b_per_slice, (int)ntasks,
+                                  pre_times, pre_exp_tau_s,
+                                  pre_exp_tau, rec, grad_mode == 1 ? 1 : 0, grad,
                                   partial));
-    if (int r = check_launch("wgrad_list_kernel")) return r;
+    if (int r = check_launch("wgrad_tile_kernel")) return r;
     if (grad_mode != 1)
       if (int r = reduce_slices(st, F.slices, JI, inv_b, partial, grad)) return r;
   }

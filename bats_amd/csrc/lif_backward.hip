@@ -36,15 +36,11 @@ constexpr int kBwdThreads = 256;
 //   n_eff [B*J]      spikes before the row's first infinite time
 //   S1, S2 [B*J*K]   float64 suffix sums of d1, d2 (the wide pre-spike error kernel)
 //   rec              one record of tile_rec_slots(K) float4 per (sample, tile of kWT rows):
-//                      slots 0..15   the first spike of row u as {t, d1, d2, 0}, or
-//                                    {-inf, 0, 0, 0} for an empty row (it never counts)
-//                      slot 16       x: mask of non-empty rows | mask of rows with more
-//                                    than one spike << 16;  y: number of tail spikes
-//                      slots 17..20  per row u (uint32 u of the 16): tail count << 16 |
-//                                    tail offset
-//                      slots 21..    the later spikes of row 0, row 1, ... (the tails)
+//                      slot 0     x: number of spikes in the tile (int)
+//                      slots 1..  the spikes of row 0, row 1, ... as {t, d1, d2, r} with r
+//                                 the byte offset of the row's accumulators (row * 512)
 // Lanes are (sample, tile, row in tile), so the kWT rows of a tile are kWT consecutive lanes
-// of one wave and a record's layout comes from 16-lane prefix sums and ballots.
+// of one wave and a record's layout comes from a 16-lane prefix sum.
 // Rows of up to kPrepFast spikes (all but a few in 10^4 at the 784-800-10 shape) load their
 // slots with independent loads into registers, so that a row costs one memory round trip
 // instead of one per spike; longer rows run the same arithmetic from global memory.
@@ -52,11 +48,9 @@ constexpr int kBwdThreads = 256;
 constexpr int kPrepFast = 8;
 constexpr int kWT = 16;  // post rows per weight-gradient tile
 
-constexpr int kRecMask = 16;   // header slot: row masks and the tail total
-constexpr int kRecInfo = 17;   // first of the 4 slots of per-row tail counts and offsets
-constexpr int kRecHead = 21;   // first tail slot
+constexpr int kRecHead = 1;  // first spike slot of a tile record
 
-__host__ __device__ inline int tile_rec_slots(int K) { return kRecHead + kWT * (K - 1); }
+__host__ __device__ inline int tile_rec_slots(int K) { return kRecHead + kWT * K; }
 
 __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
     int B, int J, int K, const int *__restrict__ n, const float *__restrict__ t,
@@ -121,30 +115,20 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
     while (ne < nk && !isinf(t[row + ne])) ++ne;
   }
   if (active && n_eff) n_eff[r] = ne;
-  // the row's tail (spikes after the first) in its tile record
-  const int tc = ne > 1 ? ne - 1 : 0;
-  int incl = tc;
+  // the row's spikes in its tile record
+  int incl = ne;
 #pragma unroll
   for (int d = 1; d < kWT; d <<= 1) {
     const int y = __shfl_up(incl, d, kWT);
     if (u >= d) incl += y;
   }
-  const int toff = incl - tc;
-  const int ntail = __shfl(incl, kWT - 1, kWT);
-  const int grp = (threadIdx.x & 63) / kWT;
-  const unsigned nz = (unsigned)(__ballot(ne > 0) >> (grp * kWT)) & 0xffffu;
-  const unsigned tl = (unsigned)(__ballot(ne > 1) >> (grp * kWT)) & 0xffffu;
+  const int excl = incl - ne;
+  const int total = __shfl(incl, kWT - 1, kWT);
   float4 *tr = rec ? rec + (size_t)tile * tile_rec_slots(K) : nullptr;
-  if (tr && valid) {
-    if (u == 0) tr[kRecMask] = make_float4(__uint_as_float(nz | (tl << 16)), __int_as_float(ntail),
-                                           0.f, 0.f);
-    reinterpret_cast<unsigned *>(tr + kRecInfo)[u] = ((unsigned)tc << 16) | (unsigned)toff;
-    if (ne == 0) tr[u] = make_float4(-kInfF, 0.f, 0.f, 0.f);
-  }
+  if (tr && valid && u == 0) tr[0] = make_float4(__int_as_float(total), 0.f, 0.f, 0.f);
   if (!active) return;
-  // slot of spike k: k = 0 -> u, k >= 1 -> kRecHead + toff + k - 1
-  float4 *ent0 = tr ? tr + u : nullptr;
-  float4 *entt = tr ? tr + kRecHead + toff - 1 : nullptr;
+  float4 *ent = tr ? tr + kRecHead + excl : nullptr;
+  const float roff = __int_as_float(u * 64 * (int)sizeof(double));
   double s1 = 0.0, s2 = 0.0;
   if (fast) {
 #pragma unroll
@@ -160,7 +144,7 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
           S1[row + k] = s1;
           S2[row + k] = s2;
         }
-        if (tr) (k == 0 ? ent0 : entt + k)[0] = make_float4(tv[k], (float)d1, (float)d2, 0.f);
+        if (ent) ent[k] = make_float4(tv[k], (float)d1, (float)d2, roff);
       }
     }
     return;
@@ -177,7 +161,7 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
       S1[row + k] = s1;
       S2[row + k] = s2;
     }
-    if (tr) (k == 0 ? ent0 : entt + k)[0] = make_float4(t[row + k], (float)d1, (float)d2, 0.f);
+    if (ent) ent[k] = make_float4(t[row + k], (float)d1, (float)d2, roff);
   }
 }
 
@@ -378,21 +362,23 @@ __global__ __launch_bounds__(1024) void pre_err_kernel(
 // the batch); lane = pre neuron i. For each sample of the slice and each pre spike (t_p, es,
 // e) of the lane, every spike k of every row u of the tile adds
 //   q = es d1_k - e d2_k   if t_p < t_k      (d = the row_prep_kernel terms)
-// to the float64 accumulator of row u, summing V1(t_p) es - V2(t_p) e term by term. The row
-// code is unrolled, so each row has its own register accumulator. A row costs one bit test
-// of the record's non-empty mask when it is empty (more than half of a hidden layer's rows),
-// its first spike sits at a fixed slot, and only rows with more than one spike (about one in
-// eight) run a loop over their tail. q is formed in float32 (the terms are float32) and added
-// in float64.
+// to the float64 accumulator of (row u, lane), summing V1(t_p) es - V2(t_p) e term by term.
+// The accumulators live in the wave's own LDS array acc[u][lane] and each term is added
+// with one LDS float64 atomic that returns nothing (a lane only ever touches its own
+// addresses, so the order of its additions is fixed): the walk is one loop over the tile's
+// spike list with no per-row code, and empty rows (more than half of a hidden layer's rows)
+// cost nothing. q is formed in float32 (the terms are float32).
 //
 // The sample's tile record is fetched one sample ahead with one coalesced load of its first
-// 64 slots, parked in the wave's own LDS window and read back with wave-uniform (broadcast)
-// LDS reads; a sample whose tails overrun the window reads them from global memory.
+// kRecWindow slots, parked in the wave's own LDS window and read back with wave-uniform
+// (broadcast) LDS reads; spikes past the window come from global memory.
 //
-// Block order: the workgroups of one (tile, slice) read the same records, so consecutive
-// logical blocks are placed on one XCD (workgroups are dealt to the 8 XCDs round robin) and
-// share its L2.
+// A workgroup holds kTileWaves independent waves (consecutive logical tasks, each with its
+// own LDS; no workgroup barrier). Block order: the waves of one (tile, slice) read the same
+// records, so consecutive logical blocks are placed on one XCD (workgroups are dealt to the
+// 8 XCDs round robin) and share its L2.
 constexpr int kRecWindow = 64;
+constexpr int kTileWaves = 4;
 
 __device__ __forceinline__ int xcd_logical_block(int p, int nblk) {
   constexpr int kXcd = 8;
@@ -401,13 +387,9 @@ __device__ __forceinline__ int xcd_logical_block(int p, int nblk) {
   return xcd * q + min(xcd, r) + k;
 }
 
-// A workgroup holds kTileWaves independent waves (consecutive logical tasks, each with its
-// own LDS window; no workgroup barrier), so that few workgroups carry many waves.
-constexpr int kTileWaves = 4;
-
 // LDS ordering inside one wave: its DS instructions execute in order, so a fence at
 // wavefront scope (which also keeps the compiler from moving LDS accesses across it) is all
-// that a wave-private window needs.
+// that a wave-private array needs.
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -421,10 +403,12 @@ __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
     const float *__restrict__ pre_e, const float4 *__restrict__ rec, int per_sample,
     float *__restrict__ grad, float *__restrict__ partial) {
   __shared__ float4 wins[kTileWaves][kRecWindow];
+  __shared__ double accs[kTileWaves][kWT * 64];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const int L = xcd_logical_block(blockIdx.x, gridDim.x) * kTileWaves + wv;
   if (L >= ntasks) return;  // whole wave; no workgroup barrier follows
   float4 *win = wins[wv];
+  double *acc = accs[wv];
   const int bi = L % gi;
   const int tj = (L / gi) % gj;
   const int sl = L / (gi * gj);
@@ -439,10 +423,10 @@ __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
   const int R = tile_rec_slots(K);
   const int wn = min(kRecWindow, R);  // slots of a record held in the window
   const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  char *acc_lane = reinterpret_cast<char *>(acc + lane);
 
-  double acc[kWT];
 #pragma unroll
-  for (int u = 0; u < kWT; ++u) acc[u] = 0.0;
+  for (int u = 0; u < kWT; ++u) acc[u * 64 + lane] = 0.0;
 
   // the next sample's record window and pre slot, loaded during the current walk
   float4 nrec = zero4;
@@ -458,38 +442,25 @@ __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
   }
   for (int b = b0; b < b1; ++b) {
     const float4 *grec = rec + ((size_t)b * gj + tj) * R;
-    const unsigned masks = __builtin_amdgcn_readlane(__float_as_uint(nrec.x), kRecMask);
-    const int ntail = __builtin_amdgcn_readlane(__float_as_int(nrec.y), kRecMask);
+    const int cnt = __builtin_amdgcn_readlane(__float_as_int(nrec.x), 0);
     wave_lds_fence();  // the previous sample's window reads are done
     win[lane] = nrec;
     wave_lds_fence();
     if (b + 1 < b1 && lane < wn) nrec = grec[(size_t)gj * R + lane];
-    const bool tails_in_window = kRecHead + ntail <= wn;
-    const unsigned *info = reinterpret_cast<const unsigned *>(win + kRecInfo);
+    const int nw = min(cnt, wn - kRecHead);  // spikes inside the window
     auto walk = [&](float tp, float es, float e) {
-      auto term = [&](int u, const float4 E) {
+      auto term = [&](const float4 E) {
         float q = fmaf(es, E.y, -(e * E.z));
         // formed for every lane: an opaque barrier keeps the compiler from turning the
         // select into a branch around the term
         asm volatile("" : "+v"(q));
         q = (tp < E.x) ? q : 0.f;
-        acc[u] += (double)q;
+        double *a = reinterpret_cast<double *>(acc_lane + __float_as_int(E.w));
+        __hip_atomic_fetch_add(a, (double)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       };
-#pragma unroll
-      for (int u = 0; u < kWT; ++u) {
-        if (masks & (1u << u)) {
-          term(u, win[u]);
-          if (masks & (0x10000u << u)) {
-            const unsigned inf = __builtin_amdgcn_readfirstlane(info[u]);
-            const int c = (int)(inf >> 16), o = kRecHead + (int)(inf & 0xffffu);
-            if (tails_in_window) {
-              for (int k = 0; k < c; ++k) term(u, win[o + k]);
-            } else {
-              for (int k = 0; k < c; ++k) term(u, grec[o + k]);
-            }
-          }
-        }
-      }
+#pragma unroll 2
+      for (int k = 0; k < nw; ++k) term(win[kRecHead + k]);
+      for (int k = nw; k < cnt; ++k) term(grec[kRecHead + k]);
     };
     if (kSingle) {
       const float tp = ntp, es = nes, e = nee;
@@ -508,17 +479,19 @@ __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
       for (int m = 0; m < np; ++m) walk(pre_t[pbase + m], pre_es[pbase + m], pre_e[pbase + m]);
     }
     if (per_sample) {
+      wave_lds_fence();
 #pragma unroll
       for (int u = 0; u < kWT; ++u) {
-        if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u];
-        acc[u] = 0.0;
+        if (u < jn && active) grad[((size_t)b * J + j0 + u) * I + i] = (float)acc[u * 64 + lane];
+        acc[u * 64 + lane] = 0.0;
       }
     }
   }
   if (!per_sample && active) {
+    wave_lds_fence();
 #pragma unroll
     for (int u = 0; u < kWT; ++u)
-      if (u < jn) partial[((size_t)sl * J + j0 + u) * I + i] = (float)acc[u];
+      if (u < jn) partial[((size_t)sl * J + j0 + u) * I + i] = (float)acc[u * 64 + lane];
   }
 }
 

@@ -103,6 +103,32 @@ static __device__ __attribute__((noinline)) SolveState solve_crossings_ool(
 // after its event walk, instead of once per candidate interval inside it.
 constexpr double kZBand = 1e-12;
 
+// Square root and quotient for the candidate root: the hardware reciprocal (square root)
+// estimate refined by two Newton steps (each about doubles the correct bits), without the
+// rescaling that guards denormal and overflowing operands (the solver's discriminants and
+// denominators are normal numbers). Results are within an ulp or two of the correctly
+// rounded value, far inside kZBand, where every accept/reject decision is made.
+__device__ __forceinline__ double root_sqrt(double d) {  // d >= 0
+  if (d == 0.0) return 0.0;
+  const double y = __builtin_amdgcn_rsq(d);
+  double x = d * y, h = 0.5 * y;
+  const double r = fma(-x, h, 0.5);
+  x = fma(x, r, x);
+  h = fma(h, r, h);
+  const double e = fma(-x, x, d);
+  return fma(e, h, x);
+}
+
+__device__ __forceinline__ double root_div(double num, double den) {
+  double r = __builtin_amdgcn_rcp(den);
+  double e = fma(-den, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-den, r, 1.0);
+  r = fma(r, e, r);
+  const double q = num * r;
+  return fma(fma(-den, q, num), r, q);
+}
+
 struct SolveStateZ {
   double b;
   int n;
@@ -136,34 +162,28 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
   double zlast = z_lo;
   bool last_is_event = true;
   while (true) {
+    // the candidate root, formed without branches (a negative discriminant or a zero
+    // denominator makes it invalid below, exactly where the reference's loop stops)
     const double disc = b * b - (4.0 * a) * c;
-    if (disc < 0.0) break;
-    const double x = sqrt(disc);
+    const double x = root_sqrt(fmax(disc, 0.0));
     const double den = x + b;
-    if (den == 0.0) break;
-    const double z = (2.0 * a) / den;
-    if (z < 0.0) break;
-    // t <= last  -> reject
-    bool above;
-    if (z > zlast * (1.0 + kZBand)) {
-      above = true;
-    } else if (z < zlast * (1.0 - kZBand)) {
-      above = false;
-    } else {
-      const double last = last_is_event ? (double)ev->t : tau_log(tau, zlast);
-      above = tau_log(tau, z) > last;
+    const double z = root_div(2.0 * a, den);
+    const bool valid = (disc >= 0.0) & (den != 0.0) & (z >= 0.0);
+    // t > last and t <= tl, decided in z space; only a root within kZBand of an end (rare)
+    // takes the branch that compares float64 logarithms
+    const bool lo_clear = z > zlast * (1.0 + kZBand), lo_rej = z < zlast * (1.0 - kZBand);
+    const bool hi_clear = z < z_hi * (1.0 - kZBand);
+    const bool hi_rej = hi_finite & (z > z_hi * (1.0 + kZBand));
+    bool above = lo_clear, below = hi_clear;
+    if (valid & ((!lo_clear & !lo_rej) | (!hi_clear & !hi_rej))) {
+      const double tz = tau_log(tau, z);
+      if (!lo_clear & !lo_rej) {
+        const double last = last_is_event ? (double)ev->t : tau_log(tau, zlast);
+        above = tz > last;
+      }
+      if (!hi_clear & !hi_rej) below = !(tz > (double)ev->tl);
     }
-    if (!above) break;
-    // t > tl  -> reject
-    bool below;
-    if (z < z_hi * (1.0 - kZBand)) {
-      below = true;
-    } else if (hi_finite && z > z_hi * (1.0 + kZBand)) {
-      below = false;
-    } else {
-      below = !(tau_log(tau, z) > (double)ev->tl);
-    }
-    if (!below) break;
+    if (!(valid & above & below)) break;
     const size_t slot = row + n;
     out_a[slot] = (float)a;
     out_x[slot] = (float)x;

@@ -11,15 +11,14 @@
 //   1. row_prep_kernel   one lane per (sample, post neuron): the reset-chain error scan
 //      (propagate_recurrent_errors.cu:17-22, written back in place like the reference),
 //      the factors f1, f2 (compute_factors.cu:20-30, rounded to float32 as stored there),
-//      and per spike the terms d1 = e f1, d2 = e f2 (plus, when pre-spike errors are
-//      wanted, the float64 suffix sums S1[k] = sum_{k'>=k} d1, S2[k] = sum_{k'>=k} d2, so
-//      that V_j(t) = S[first k with t_jk > t]).
+//      and per spike the terms d1 = e f1, d2 = e f2, written as per-tile spike records
+//      (V_j(t) is the sum of the d of the spikes of row j later than t).
 //   2. wgrad_tile_kernel lanes over pre neurons i, a tile of 16 post neurons, a slice of the
 //      batch: the terms of the post spikes later than the lane's pre spike, float64
 //      accumulation per row; a deterministic fixed-order reduction over batch slices
 //      (wgrad_reduce_kernel) finishes the mean.
-//   3. pre_err_kernel    lanes over pre neurons i of one sample, post rows staged in LDS
-//      a tile at a time.
+//   3. pre_err_tile_kernel lanes over pre neurons i of one sample, the same tile records
+//      walked tile by tile with the tile's block of W in LDS.
 // Layers of at most 16 post neurons (an output layer) run all of it in small_bwd_kernel.
 #include "common.hpp"
 
@@ -34,7 +33,7 @@ constexpr int kBwdThreads = 256;
 // factors f1, f2 (compute_factors.cu:20-30, rounded to float32 as stored there) and the
 // per-spike weight-gradient terms d1 = e f1, d2 = e f2. Outputs (each optional):
 //   n_eff [B*J]      spikes before the row's first infinite time
-//   S1, S2 [B*J*K]   float64 suffix sums of d1, d2 (the wide pre-spike error kernel)
+//   S1, S2 [B*J*K]   float64 suffix sums of d1, d2 (the conv backward reads them)
 //   rec              one record of tile_rec_slots(K) float4 per (sample, tile of kWT rows):
 //                      slot 0     x: number of spikes in the tile (int)
 //                      slots 1..  the spikes of row 0, row 1, ... as {t, d1, d2, r} with r
@@ -103,16 +102,42 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
       ne += run ? 1 : 0;
     }
   } else {
-    double C = 0.0;
-    for (int k = apply_reset ? nk - 1 : -1; k >= 0; --k) {
-      const float zk = z[row + k];
-      if (isinf(zk)) continue;
-      const double ek = (double)errors[row + k] + C * (double)zk;
-      const float ek32 = (float)ek;
-      errors[row + k] = ek32;
-      C = C + (double)ek32 * dtt / (double)x[row + k];
+    // longer rows: the same arithmetic over chunks of kPrepFast slots, each chunk's slots
+    // loaded in one round (the reset chain runs from the last chunk to the first)
+    if (apply_reset) {
+      double C = 0.0;
+      for (int c1 = nk; c1 > 0; c1 -= kPrepFast) {
+        const int c0 = max(0, c1 - kPrepFast);
+        float zc[kPrepFast], ec[kPrepFast], xc[kPrepFast];
+#pragma unroll
+        for (int k = 0; k < kPrepFast; ++k) {
+          const bool in = c0 + k < c1;
+          zc[k] = in ? z[row + c0 + k] : kInfF;
+          ec[k] = in ? errors[row + c0 + k] : 0.f;
+          xc[k] = in ? x[row + c0 + k] : 1.f;
+        }
+#pragma unroll
+        for (int k = kPrepFast - 1; k >= 0; --k) {
+          if (c0 + k < c1 && !isinf(zc[k])) {
+            const double ek = (double)ec[k] + C * (double)zc[k];
+            ec[k] = (float)ek;
+            C = C + (double)ec[k] * dtt / (double)xc[k];
+            errors[row + c0 + k] = ec[k];
+          }
+        }
+      }
     }
-    while (ne < nk && !isinf(t[row + ne])) ++ne;
+    bool run = true;
+    for (int c0 = 0; c0 < nk && run; c0 += kPrepFast) {
+      float tc[kPrepFast];
+#pragma unroll
+      for (int k = 0; k < kPrepFast; ++k) tc[k] = c0 + k < nk ? t[row + c0 + k] : kInfF;
+#pragma unroll
+      for (int k = 0; k < kPrepFast; ++k) {
+        run = run & !isinf(tc[k]);
+        ne += run ? 1 : 0;
+      }
+    }
   }
   if (active && n_eff) n_eff[r] = ne;
   // the row's spikes in its tile record
@@ -149,19 +174,33 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
     }
     return;
   }
-  for (int k = ne - 1; k >= 0; --k) {
-    const double ak = (double)a[row + k], xk = (double)x[row + k], zk = (double)z[row + k];
-    const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
-    const float f2 = (float)(tau / xk);
-    const double ek = (double)errors[row + k];
-    const double d1 = ek * (double)f1, d2 = ek * (double)f2;
-    s1 += d1;
-    s2 += d2;
-    if (S1) {
-      S1[row + k] = s1;
-      S2[row + k] = s2;
+  for (int c1 = ne; c1 > 0; c1 -= kPrepFast) {
+    const int c0 = max(0, c1 - kPrepFast);
+#pragma unroll
+    for (int k = 0; k < kPrepFast; ++k) {
+      const bool in = c0 + k < c1;
+      tv[k] = in ? t[row + c0 + k] : kInfF;
+      av[k] = in ? a[row + c0 + k] : 1.f;
+      xv[k] = in ? x[row + c0 + k] : 1.f;
+      zv[k] = in ? z[row + c0 + k] : 1.f;
+      ev[k] = in ? errors[row + c0 + k] : 0.f;
     }
-    if (ent) ent[k] = make_float4(t[row + k], (float)d1, (float)d2, roff);
+#pragma unroll
+    for (int k = kPrepFast - 1; k >= 0; --k) {
+      if (c0 + k < c1) {
+        const double ak = (double)av[k], xk = (double)xv[k], zk = (double)zv[k];
+        const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
+        const float f2 = (float)(tau / xk);
+        const double d1 = (double)ev[k] * (double)f1, d2 = (double)ev[k] * (double)f2;
+        s1 += d1;
+        s2 += d2;
+        if (S1) {
+          S1[row + c0 + k] = s1;
+          S2[row + c0 + k] = s2;
+        }
+        if (ent) ent[c0 + k] = make_float4(tv[k], (float)d1, (float)d2, roff);
+      }
+    }
   }
 }
 
@@ -171,89 +210,34 @@ static int launch_row_prep(hipStream_t st, int B, int J, int K, const int *n, co
                            double *S2, float4 *rec) {
   const int64_t lanes = (int64_t)B * ((J + kWT - 1) / kWT) * kWT;
   if (lanes <= 0) return 0;
-  hipLaunchKernelGGL(row_prep_kernel, dim3((unsigned)((lanes + kBwdThreads - 1) / kBwdThreads)),
-                     dim3(kBwdThreads), 0, st, B, J, K, n, t, a, x, z, errors, c, tau, dtt, 1,
+  // small launches (an output layer's rows) in one-wave blocks, spread over many CUs
+  const int threads = lanes < 256 * 256 ? 64 : kBwdThreads;
+  hipLaunchKernelGGL(row_prep_kernel, dim3((unsigned)((lanes + threads - 1) / threads)),
+                     dim3(threads), 0, st, B, J, K, n, t, a, x, z, errors, c, tau, dtt, 1,
                      n_eff, S1, S2, rec);
   return check_launch("row_prep_kernel");
 }
 
-// Post-neuron rows staged in LDS for SB samples at a time: spike counts, times and suffix
-// sums of rows (b, j0 .. j0+jn-1), b = b0 .. b0+sb-1. Rows are copied whole (slots past a
-// row's spike count are never read), so one copy and two barriers serve SB samples.
-struct RowStage {
-  int *ne;     // [SB][JT]
-  float *t;    // [SB][JT][K]
-  double *s1;  // [SB][JT][K]
-  double *s2;  // [SB][JT][K]
-};
-
-__device__ __forceinline__ RowStage row_stage(char *smem, int SB, int JT, int K) {
-  RowStage r;
-  const size_t n = (size_t)SB * JT * K;
-  r.s1 = reinterpret_cast<double *>(smem);
-  r.s2 = r.s1 + n;
-  r.t = reinterpret_cast<float *>(r.s2 + n);
-  r.ne = reinterpret_cast<int *>(r.t + n);
-  return r;
-}
-
-__host__ __device__ constexpr size_t row_stage_bytes(int SB, int JT, int K) {
-  return (size_t)SB * JT * K * (2 * sizeof(double) + sizeof(float)) + (size_t)SB * JT * sizeof(int);
-}
-
-// Copies rows of samples b0 .. b0+sb-1 into the stage. Contains barriers: call from every
-// thread of the block.
-__device__ __forceinline__ void load_rows(const RowStage &st, int b0, int sb, int J, int j0,
-                                          int jn, int JT, int K,
-                                          const float *__restrict__ post_t,
-                                          const int *__restrict__ n_eff,
-                                          const double *__restrict__ S1,
-                                          const double *__restrict__ S2) {
-  const int tid = threadIdx.x, nth = blockDim.x;
-  __syncthreads();  // previous contents consumed
-  const int per = jn * K;
-  for (int e = tid; e < sb * per; e += nth) {
-    const int s = e / per;
-    const int r = e - s * per;
-    const size_t src = ((size_t)(b0 + s) * J + j0) * K + r;
-    const int dst = s * JT * K + r;
-    st.t[dst] = post_t[src];
-    st.s1[dst] = S1[src];
-    st.s2[dst] = S2[src];
-  }
-  for (int e = tid; e < sb * jn; e += nth) {
-    const int s = e / jn;
-    const int r = e - s * jn;
-    st.ne[s * JT + r] = n_eff[(size_t)(b0 + s) * J + j0 + r];
-  }
-  __syncthreads();
-}
-
-// V(t): S[first k with tpost[k] > tp], 0 when there is none; the post times of a row are
-// ascending, so the index is the count of times <= tp.
-__device__ __forceinline__ void lookup_count(const float *__restrict__ tr,
-                                             const double *__restrict__ s1,
-                                             const double *__restrict__ s2, int nj, float tp,
-                                             double &v1, double &v2) {
-  int kk = 0;
-  for (int k = 0; k < nj; ++k) kk += (tr[k] <= tp) ? 1 : 0;
-  v1 = (kk < nj) ? s1[kk] : 0.0;
-  v2 = (kk < nj) ? s2[kk] : 0.0;
-}
-
-// Number of leading finite slots of a spike row (independent loads, no dependent chain).
+// Number of leading finite slots of a spike row. The first 8 slots are read with independent
+// loads in one round; the rest of the row only when all 8 are finite (spike rows are short:
+// a hidden neuron of the 784-800-10 network fires about once per sample).
 __device__ __forceinline__ int leading_finite(const float *__restrict__ row, int K) {
+  constexpr int kHead = 8;
   int n = 0;
   bool run = true;
-#pragma unroll 8
-  for (int m = 0; m < K; ++m) {
-    run = run & (bool)isfinite(row[m]);  // no short-circuit: the loads stay independent
+#pragma unroll
+  for (int m = 0; m < kHead; ++m) {
+    const float v = m < K ? row[m] : kInfF;  // independent of run: all loads in one round
+    run = run & (bool)isfinite(v);
     n += run ? 1 : 0;
+  }
+  if (!run) return n;
+  for (int m = kHead; m < K; ++m) {
+    if (!isfinite(row[m])) break;
+    ++n;
   }
   return n;
 }
-
-constexpr int kWJT = 16;  // post rows per LDS stage of pre_err_kernel
 
 __global__ __launch_bounds__(kBwdThreads) void wgrad_reduce_kernel(
     int slices, int64_t JI, double inv_b, const float *__restrict__ partial,
@@ -310,51 +294,6 @@ static int reduce_slices(hipStream_t st, int slices, int64_t JI, double inv_b,
   hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((JI + kBwdThreads - 1) / kBwdThreads)),
                      dim3(kBwdThreads), 0, st, slices, JI, inv_b, partial, grad);
   return check_launch("wgrad_reduce_kernel");
-}
-
-// Lanes are pre neurons of one sample; the post rows are staged kWJT at a time and every
-// pre spike of the lane accumulates W[j,i] (V1 es/tau_s - V2 e/tau) over all post neurons.
-// Pre-spike sums of one lane are kept in float64 scratch rows across the row chunks.
-__global__ __launch_bounds__(1024) void pre_err_kernel(
-    int I, int K_pre, int J, int K, const float *__restrict__ pre_t,
-    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
-    const float *__restrict__ W, const float *__restrict__ post_t,
-    const int *__restrict__ n_eff, const double *__restrict__ S1,
-    const double *__restrict__ S2, double tau_s, double tau, double *__restrict__ acc_scratch,
-    float *__restrict__ pre_errors) {
-  extern __shared__ __align__(16) char smem[];
-  const RowStage st = row_stage(smem, 1, kWJT, K);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  const bool active = i < I;
-  const int ic = active ? i : 0;
-  const size_t pbase = ((size_t)b * I + ic) * K_pre;
-  const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
-  const bool single_chunk = J <= kWJT;
-  for (int j0 = 0; j0 < J; j0 += kWJT) {
-    const int jn = min(kWJT, J - j0);
-    load_rows(st, b, 1, J, j0, jn, kWJT, K, post_t, n_eff, S1, S2);
-    for (int m = 0; m < np; ++m) {
-      const float tp = pre_t[pbase + m];
-      const double es = (double)pre_es[pbase + m];
-      const double e = (double)pre_e[pbase + m];
-      double acc = (j0 == 0) ? 0.0 : acc_scratch[pbase + m];
-      for (int u = 0; u < jn; ++u) {
-        const int nj = st.ne[u];
-        if (nj == 0) continue;
-        double v1, v2;
-        lookup_count(st.t + u * K, st.s1 + u * K, st.s2 + u * K, nj, tp, v1, v2);
-        const double w = (double)W[(size_t)(j0 + u) * I + i];
-        acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
-      }
-      if (single_chunk || j0 + kWJT >= J)
-        pre_errors[pbase + m] = (float)acc;
-      else
-        acc_scratch[pbase + m] = acc;
-    }
-  }
-  if (active)
-    for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
 }
 
 // ------------------------------------------------------------------ wide layers
@@ -493,6 +432,105 @@ __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
     for (int u = 0; u < kWT; ++u)
       if (u < jn) partial[((size_t)sl * J + j0 + u) * I + i] = (float)acc[u * 64 + lane];
   }
+}
+
+// Pre-spike errors of wide layers (the previous layer is trainable):
+//   pre_err[b,i,m] = sum_j W[j,i] sum_{k: t_jk > t_m} (d1_k es_m / tau_s - d2_k e_m / tau)
+// One wave per (block of 64 pre neurons, sample); lane = pre neuron i. The wave walks the
+// sample's kWT-row tile records (row_prep_kernel) one tile at a time: the tile's 16 x 64
+// block of W sits in the wave's own LDS array (one coalesced load per row, fetched a tile
+// ahead, with the next record window), and each spike entry adds
+//   w[row] * (es'_m d1 - e'_m d2)     if t_m < t_k       (es' = es / tau_s, e' = e / tau)
+// for kPreM pre spikes of the lane at once, in float32 within the tile and float64 across
+// tiles. Lanes with more pre spikes take further passes over the records.
+constexpr int kPreM = 2;
+
+template <int M>
+__global__ __launch_bounds__(64 * kTileWaves) void pre_err_tile_kernel(
+    int B, int I, int K_pre, int J, int K, int gi, int ntasks, const float *__restrict__ pre_t,
+    const float *__restrict__ pre_es, const float *__restrict__ pre_e,
+    const float *__restrict__ W, const float4 *__restrict__ rec, double inv_tau_s,
+    double inv_tau, float *__restrict__ pre_errors) {
+  __shared__ float4 wins[kTileWaves][kRecWindow];
+  __shared__ float wts[kTileWaves][kWT * 64];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+  const int L = xcd_logical_block(blockIdx.x, gridDim.x) * kTileWaves + wv;
+  if (L >= ntasks) return;  // whole wave; no workgroup barrier follows
+  float4 *win = wins[wv];
+  float *wt = wts[wv];
+  const int bi = L % gi;
+  const int b = L / gi;
+  const int lane = threadIdx.x & 63;
+  const int i = bi * 64 + lane;
+  const bool active = i < I;
+  const int ic = active ? i : 0;
+  const int gj = (J + kWT - 1) / kWT;
+  const int R = tile_rec_slots(K);
+  const int wn = min(kRecWindow, R);
+  const size_t pbase = ((size_t)b * I + ic) * K_pre;
+  const int np = active ? leading_finite(pre_t + pbase, K_pre) : 0;
+  // passes over the records: enough for the lane of the wave with the most pre spikes
+  int npmax = np;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) npmax = max(npmax, __shfl_xor(npmax, d, 64));
+  npmax = __builtin_amdgcn_readfirstlane(npmax);
+  const float4 *grec0 = rec + (size_t)b * gj * R;
+  const float zf = 0.f;
+  for (int m0 = 0; m0 < npmax; m0 += M) {
+    float tp[M], esd[M], ed[M];
+    double pe[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const bool has = m0 + m < np;
+      tp[m] = has ? pre_t[pbase + m0 + m] : kInfF;
+      esd[m] = has ? (float)((double)pre_es[pbase + m0 + m] * inv_tau_s) : zf;
+      ed[m] = has ? (float)((double)pre_e[pbase + m0 + m] * inv_tau) : zf;
+      pe[m] = 0.0;
+    }
+    // tile 0's record window and W block
+    float4 nrec = lane < wn ? grec0[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float nw[kWT];
+#pragma unroll
+    for (int u = 0; u < kWT; ++u) nw[u] = (active && u < J) ? W[(size_t)u * I + i] : 0.f;
+    for (int tj = 0; tj < gj; ++tj) {
+      const float4 *grec = grec0 + (size_t)tj * R;
+      const int cnt = __builtin_amdgcn_readlane(__float_as_int(nrec.x), 0);
+      wave_lds_fence();  // the previous tile's reads are done
+      win[lane] = nrec;
+#pragma unroll
+      for (int u = 0; u < kWT; ++u) wt[u * 64 + lane] = nw[u];
+      wave_lds_fence();
+      if (tj + 1 < gj) {
+        if (lane < wn) nrec = grec[R + lane];
+        const int j1 = (tj + 1) * kWT;
+#pragma unroll
+        for (int u = 0; u < kWT; ++u)
+          nw[u] = (active && j1 + u < J) ? W[(size_t)(j1 + u) * I + i] : 0.f;
+      }
+      float pt[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) pt[m] = 0.f;
+      const int nwin = min(cnt, wn - kRecHead);
+      auto term = [&](const float4 E) {
+        // the entry's row: its accumulator offset (row * 512 bytes) / 2 is the row's W offset
+        const float w = wt[(__float_as_int(E.w) >> 3) + lane];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const float q = fmaf(esd[m], E.y, -(ed[m] * E.z));
+          pt[m] = fmaf((tp[m] < E.x) ? w : 0.f, q, pt[m]);
+        }
+      };
+      for (int k = 0; k < nwin; ++k) term(win[kRecHead + k]);
+      for (int k = nwin; k < cnt; ++k) term(grec[kRecHead + k]);
+#pragma unroll
+      for (int m = 0; m < M; ++m) pe[m] += (double)pt[m];
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (m0 + m < np) pre_errors[pbase + m0 + m] = (float)pe[m];
+  }
+  if (active)
+    for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
 }
 
 // ------------------------------------------------------------------ narrow layers
@@ -669,7 +707,7 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
 }
 
 struct BwdLayout {
-  size_t neff, s1, s2, rec, partial, scratch, total;
+  size_t neff, s1, s2, rec, partial, total;
   int slices, b_per_slice, lanes, gi;
   bool small;
 };
@@ -678,8 +716,7 @@ struct BwdLayout {
 // the scalar-load latency of one wave is covered by the others and the last round is short.
 constexpr int kWgradTargetWaves = 6144;
 
-static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_sample = false,
-                            bool want_pre = true) {
+static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_sample = false) {
   BwdLayout L{};
   L.gi = std::max(1, (I + 1023) / 1024);
   const int per_block = (I + L.gi - 1) / L.gi;
@@ -695,25 +732,13 @@ static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_samp
     L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
     L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
   }
-  // rows prepared in global memory: the wide path, and the error write-back of wide
-  // narrow-layer launches (more than one block of pre neurons)
-  const bool rows = !L.small || L.gi > 1;
-  const bool sums = !L.small && J > kWJT && want_pre;  // suffix sums for pre_err_kernel
   size_t off = 0;
-  L.neff = off;
-  if (rows) off = align_up(off + sizeof(int) * ((size_t)B * J + kWT), 256);
-  L.s1 = off;
-  if (sums) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
-  L.s2 = off;
-  if (sums) off = align_up(off + sizeof(double) * (size_t)B * J * K, 256);
   L.rec = off;
   if (!L.small)
     off = align_up(off + sizeof(float4) * (size_t)B * ((J + kWT - 1) / kWT) * tile_rec_slots(K),
                    256);
   L.partial = off;
   off = align_up(off + sizeof(float) * (size_t)L.slices * J * I, 256);
-  L.scratch = off;
-  if (sums) off = align_up(off + sizeof(double) * (size_t)B * I * K_pre, 256);
   L.total = off;
   return L;
 }
@@ -776,12 +801,10 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   BATS_REQUIRE(workspace != nullptr || L.total == 0, "null workspace");
   BATS_REQUIRE(workspace_bytes >= L.total, "workspace too small");
   char *ws = static_cast<char *>(workspace);
-  int *n_eff = reinterpret_cast<int *>(ws + L.neff);
   double *S1 = reinterpret_cast<double *>(ws + L.s1);
   double *S2 = reinterpret_cast<double *>(ws + L.s2);
   float4 *rec = reinterpret_cast<float4 *>(ws + L.rec);
   float *partial = reinterpret_cast<float *>(ws + L.partial);
-  double *scratch = reinterpret_cast<double *>(ws + L.scratch);
   // the reset chain alone, written back in place
   auto reset_only = [&]() -> int {
     return launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c, (double)tau,
@@ -812,12 +835,11 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
     return 0;
   }
 
-  const bool sums = J > kWJT && want_pre;
   int prep_rc = 0;
   BATS_TIMED("row_prep_kernel", st,
              prep_rc = launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c,
-                                       (double)tau, (double)delta_theta_tau, n_eff,
-                                       sums ? S1 : nullptr, sums ? S2 : nullptr, rec));
+                                       (double)tau, (double)delta_theta_tau, nullptr,
+                                       nullptr, nullptr, rec));
   if (prep_rc) return prep_rc;
   {
     const BwdLayout F = bwd_layout(B, I, K_pre, J, K_post, grad_mode == 1);
@@ -837,14 +859,16 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
       if (int r = reduce_slices(st, F.slices, JI, inv_b, partial, grad)) return r;
   }
   if (want_pre) {
-    const size_t lds_p = row_stage_bytes(1, kWJT, K_post);
-    if (int r = allow_lds((const void *)pre_err_kernel, lds_p)) return r;
-    BATS_TIMED("pre_err_kernel", st,
-               hipLaunchKernelGGL(pre_err_kernel, dim3(L.gi, B), dim3(L.lanes), lds_p, st, I,
-                                  K_pre, J, K_post, pre_times, pre_exp_tau_s, pre_exp_tau,
-                                  weights, t, n_eff, S1, S2, (double)tau_s, (double)tau,
-                                  scratch, pre_errors));
-    if (int r = check_launch("pre_err_kernel")) return r;
+    const int gi = (I + 63) / 64;
+    const int64_t ntasks = (int64_t)gi * B;
+    BATS_REQUIRE(ntasks < (int64_t)1 << 31, "pre-spike error grid too large");
+    const int64_t nblk = (ntasks + kTileWaves - 1) / kTileWaves;
+    BATS_TIMED("pre_err_tile_kernel", st,
+               hipLaunchKernelGGL(pre_err_tile_kernel<kPreM>, dim3((unsigned)nblk),
+                                  dim3(64 * kTileWaves), 0, st, B, I, K_pre, J, K_post, gi,
+                                  (int)ntasks, pre_times, pre_exp_tau_s, pre_exp_tau, weights,
+                                  rec, 1.0 / (double)tau_s, 1.0 / (double)tau, pre_errors));
+    if (int r = check_launch("pre_err_tile_kernel")) return r;
   }
   return 0;
 }

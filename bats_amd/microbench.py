@@ -20,7 +20,8 @@ from ._constants import lif_constants
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--what", choices=("forward1", "forward2", "backward1", "backward2", "copy"),
+    ap.add_argument("--what", choices=("forward1", "forward2", "backward1", "backward2",
+                                       "forward_hh", "backward_hh", "copy"),
                     default="forward1")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--batch", type=int, default=256)
@@ -39,6 +40,12 @@ def main():
     o = ops.lif_forward(h[1], w2, *c2, 1.0, 30)
     err2 = torch.randn(o[1].shape, device=dev)
     err1 = torch.randn(h[1].shape, device=dev)
+    if args.what in ("forward_hh", "backward_hh"):
+        # 800 -> 800 hidden layer of the 784-800-800-800-10 network, fed by the first
+        # hidden layer's multi-spike trains
+        w3 = torch.as_tensor(rng.normal(0, 1.0, size=(800, 800)).astype(np.float32), device=dev)
+        hh = ops.lif_forward(h[1], w3, *c1, 1.0, 30)
+        err3 = torch.randn(hh[1].shape, device=dev)
     src = torch.empty(args.copy_mib * 2 ** 18, dtype=torch.float32, device=dev).uniform_()
 
     def run():
@@ -50,6 +57,10 @@ def main():
             ops.lif_backward(*h[:5], err1.clone(), t, h[5], h[6], w1, *c1)
         elif args.what == "backward2":
             ops.lif_backward(*o[:5], err2.clone(), h[1], o[5], o[6], w2, *c2)
+        elif args.what == "forward_hh":
+            ops.lif_forward(h[1], w3, *c1, 1.0, 30)
+        elif args.what == "backward_hh":
+            ops.lif_backward(*hh[:5], err3.clone(), h[1], hh[5], hh[6], w3, *c1)
         else:
             torch.add(src, 0.0)
 

@@ -188,6 +188,19 @@ def test_backward_hidden_and_output_layers():
     check_backward(h, rng.normal(size=h.t.shape).astype(F32), "hidden layer")
 
 
+def test_backward_hidden_to_hidden_c3_shape():
+    # 784 -> 800 -> 800 with 30-spike buffers: the second hidden layer's pre-spike errors
+    # (the previous layer is trainable), its pre neurons firing several times
+    rng = np.random.default_rng(3)
+    t, _ = latency_inputs(rng, 8, 784)
+    W1 = (2.0 * rng.normal(0, 1.0, size=(800, 784))).astype(F32)
+    h1 = olif.lif_forward(t, W1, 0.130, 1.0, 1.0, 30, tmax=1.0)
+    assert h1.n.max() >= 3
+    W2 = (1.5 * rng.normal(0, 1.0, size=(800, 800))).astype(F32)
+    h2 = olif.lif_forward(h1.t, W2, 0.130, 1.0, 1.0, 30, tmax=1.0)
+    check_backward(h2, rng.normal(size=h2.t.shape).astype(F32), "hidden to hidden")
+
+
 @pytest.fixture
 def solver_mode():
     from bats_amd import _lib

@@ -28,8 +28,7 @@ import numpy as np
 
 METRIC = "train samples/s (784-800-10, 30 spk/neuron) vs plain form; p50/p99 serve latency"
 KERNELS = ("lif_solve_seq_kernel", "lif_solve_wave_kernel", "sort_events_kernel",
-           "wgrad_tile_kernel", "row_prep_kernel", "small_bwd_kernel",
-           "pre_err_kernel")
+           "wgrad_tile_kernel", "row_prep_kernel", "small_bwd_kernel")
 HBM_PEAK_GBS = 8000.0
 
 # network of the workload

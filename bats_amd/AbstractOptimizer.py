@@ -22,3 +22,10 @@ class AbstractOptimizer(ABC):
     @abstractmethod
     def step(self, gradient: List) -> List:
         pass
+
+    def store(self, dir_path) -> None:
+        """Writes the optimizer's state next to the layers' weight files (stateless
+        optimizers write nothing). The reference keeps no optimizer state on disk."""
+
+    def restore(self, dir_path) -> None:
+        """Reads what `store` wrote (nothing for stateless optimizers)."""

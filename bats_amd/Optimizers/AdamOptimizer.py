@@ -1,6 +1,7 @@
 """Adam (bats/Optimizers/AdamOptimizer.py:22-55) on the fused HIP Adam kernel."""
 from __future__ import annotations
 
+from pathlib import Path
 from typing import List, Optional
 
 import numpy as np
@@ -37,6 +38,46 @@ class AdamOptimizer(AbstractOptimizer):
 
     def load_state(self, m, v, t) -> None:
         self._m, self._v, self._t = m, v, int(t)
+        if self._t_dev is not None:
+            self._t_dev.fill_(int(t))
+
+    STATE_FILE = "adam_state.npz"
+
+    def store(self, dir_path) -> None:
+        """Writes `adam_state.npz`: the step counter, the hyperparameters and the float32
+        moment estimates m_<k>, v_<k> of every layer k that has them (k indexes the
+        gradient list of `step`), so that training resumes exactly where it stopped."""
+        m, v, t = self.state
+        arrays = {"t": np.int64(t), "n": np.int64(0 if m is None else len(m)),
+                  "learning_rate": np.float32(self._learning_rate),
+                  "beta_1": self._beta_1, "beta_2": self._beta_2, "epsilon": self._epsilon,
+                  "nadam": np.int8(self._nadam)}
+        for k, (mk, vk) in enumerate(zip(m or [], v or [])):
+            if mk is not None:
+                arrays[f"m_{k}"] = mk.detach().cpu().numpy()
+                arrays[f"v_{k}"] = vk.detach().cpu().numpy()
+        np.savez(Path(dir_path) / self.STATE_FILE, **arrays)
+
+    def restore(self, dir_path, device=None) -> None:
+        """Reads `adam_state.npz` if present (no pickled objects are loaded); the moments go
+        to `device` (the current GPU by default, the CPU when none is visible)."""
+        path = Path(dir_path) / self.STATE_FILE
+        if not path.exists():
+            return
+        if device is None:
+            device = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
+        with np.load(path, allow_pickle=False) as f:
+            if int(f["nadam"]) != int(self._nadam):
+                raise ValueError(f"{path} holds the state of another optimizer")
+            n = int(f["n"])
+            m = [None] * n
+            v = [None] * n
+            for k in range(n):
+                if f"m_{k}" in f:
+                    m[k] = torch.as_tensor(f[f"m_{k}"], device=device).clone()
+                    v[k] = torch.as_tensor(f[f"v_{k}"], device=device).clone()
+            self.learning_rate = float(f["learning_rate"])
+            self.load_state(m if n else None, v if n else None, int(f["t"]))
 
     def _bias(self):
         b1 = 1.0 - np.float64(self._beta_1) ** self._t

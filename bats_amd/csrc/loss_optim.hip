@@ -194,9 +194,46 @@ __global__ __launch_bounds__(kThreads) void events_to_neuron_kernel(
   }
 }
 
+// ---------------------------------------------------------------- input encoding
+// experiments/mnist/Dataset.py:66-71: t = window * (1 - px / max) in float64, +inf where that
+// equals the window (px = 0), stored as float32; one spike slot per input neuron. Grid-stride,
+// coalesced.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void latency_encode_kernel(
+    int64_t n, const T *__restrict__ px, double window, double max_value,
+    float *__restrict__ times, int *__restrict__ counts) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double t = window * (1.0 - (double)px[i] / max_value);
+    const bool fire = t != window;
+    times[i] = fire ? (float)t : kInfF;
+    counts[i] = fire ? 1 : 0;
+  }
+}
+
 }  // namespace bats
 
 using namespace bats;
+
+extern "C" int bats_latency_encode(void *stream, int64_t n, const void *pixels, int pixel_type,
+                                   double time_window, double max_value, float *times,
+                                   int *counts) {
+  BATS_REQUIRE(n >= 0, "negative size");
+  BATS_REQUIRE(pixel_type == 0 || pixel_type == 1, "pixel_type must be 0 (uint8) or 1 (float32)");
+  BATS_REQUIRE(max_value != 0.0, "max_value must be non-zero");
+  if (n == 0) return 0;
+  BATS_REQUIRE(pixels && times && counts, "null pointer");
+  const unsigned grid = std::min<unsigned>(grid_for(n), 4096);
+  if (pixel_type == 0)
+    hipLaunchKernelGGL(latency_encode_kernel<uint8_t>, dim3(grid), dim3(kThreads), 0,
+                       as_stream(stream), n, static_cast<const uint8_t *>(pixels),
+                       time_window, max_value, times, counts);
+  else
+    hipLaunchKernelGGL(latency_encode_kernel<float>, dim3(grid), dim3(kThreads), 0,
+                       as_stream(stream), n, static_cast<const float *>(pixels),
+                       time_window, max_value, times, counts);
+  return check_launch("latency_encode_kernel");
+}
 
 extern "C" int bats_count_loss(void *stream, int B, int O, int K, const int *n,
                                const int64_t *labels, float target_true, float target_false,

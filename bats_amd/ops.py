@@ -240,6 +240,29 @@ def events_to_per_neuron(indices, times, n_neurons: int, max_per_neuron: Optiona
     return out_t, out_n
 
 
+# ---------------------------------------------------------------- input encoding
+def latency_encode(pixels, time_window: float = 100e-3, max_value: float = 255.0):
+    """Latency code of a batch of images on the device (experiments/mnist/Dataset.py:66-71):
+    pixels [B, ...] uint8 or float -> (times [B, N, 1] float32, counts [B, N] int32) with
+    N = the product of the trailing dims, t = time_window * (1 - px / max_value) and +inf
+    where px = 0."""
+    _require_gpu()
+    if not isinstance(pixels, torch.Tensor):
+        pixels = torch.as_tensor(np.asarray(pixels))
+    dev = pixels.device if pixels.is_cuda else torch.device("cuda")
+    kind = 0 if pixels.dtype == torch.uint8 else 1
+    px = pixels.to(dev, torch.uint8 if kind == 0 else F32).contiguous()
+    B = int(px.shape[0])
+    N = int(px.numel() // max(B, 1))
+    times = torch.empty((B, N, 1), dtype=F32, device=dev)
+    counts = torch.empty((B, N), dtype=I32, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.bats_latency_encode(_stream(), px.numel(), _ptr(px), kind,
+                                       float(time_window), float(max_value), _ptr(times),
+                                       _ptr(counts)))
+    return times, counts
+
+
 # ---------------------------------------------------------------- conv LIF and pooling
 def _shape3(v):
     import ctypes

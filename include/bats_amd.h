@@ -215,6 +215,18 @@ int bats_sgd_step(void *stream, int64_t n, const float *grad, float *delta,
 /* weights += delta. Replaces bats/Layers/LIFLayer.py:112-113 (add_deltas). */
 int bats_add_deltas(void *stream, int64_t n, float *weights, const float *delta);
 
+/* ------------------------------------------------------------------ input encoding */
+
+/*
+ * Latency code of pixel intensities. Replaces the host-side Dataset.__to_spikes of
+ * experiments/mnist/Dataset.py:66-71: t = time_window * (1 - px / max_value) (evaluated in
+ * float64, stored as float32), +inf and a count of 0 where px = 0.
+ * pixels [n] uint8 (pixel_type 0) or float32 (pixel_type 1); times [n] (the [B,N,1] spike
+ * buffer of an input layer), counts [n] int32.
+ */
+int bats_latency_encode(void *stream, int64_t n, const void *pixels, int pixel_type,
+                        double time_window, double max_value, float *times, int *counts);
+
 /* ------------------------------------------------------------------ event lists */
 
 /*

@@ -222,3 +222,19 @@ def test_forward_both_solver_forms(solver_mode, mode):
     tb, _ = multi_spike_inputs(rng, 8, 100, 5, t_max=1.0)
     W3 = np.abs(rng.normal(0, 3.0, size=(64, 100))).astype(F32)
     check_forward(tb, W3, 0.05, 0.2, 0.2, 3, 0.6, f"buffer full mode {mode}")
+
+
+# ------------------------------------------------------------------ input encoding
+def test_latency_encoder_matches_reference_formula():
+    # experiments/mnist/Dataset.py:66-71 on uint8 and float pixels, zeros included
+    from bats_amd import ops
+    rng = np.random.default_rng(9)
+    px = rng.integers(0, 256, size=(5, 28, 28)).astype(np.uint8)
+    px[:, :3] = 0
+    want = 100e-3 * (1 - (px.reshape(5, 784, 1) / 255))
+    want[want == 100e-3] = np.inf
+    for arg in (px, torch.as_tensor(px, device="cuda"), px.astype(np.float32)):
+        t, n = ops.latency_encode(arg)
+        torch.cuda.synchronize()
+        assert np.array_equal(t.cpu().numpy(), want.astype(F32))
+        assert np.array_equal(n.cpu().numpy(), np.isfinite(want[:, :, 0]).astype(np.int32))

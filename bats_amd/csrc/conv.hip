@@ -157,42 +157,92 @@ __global__ __launch_bounds__(kConvMaxThreads) void conv_solve_kernel(
 }
 
 // ---------------------------------------------------------------- pooling
-__global__ void pool_forward_kernel(int B, int X, int Y, int C, int K,
-                                    const int *__restrict__ n, const float *__restrict__ t,
-                                    int *__restrict__ out_n, float *__restrict__ out_t,
-                                    int *__restrict__ out_src) {
+// One thread per pooled neuron merges the four children's finite spikes (ascending, ties in
+// the children's order: aggregate_spikes_conv.cu:13-41 followed by the wrapper's argsort,
+// aggregate_spikes_conv.py:24-26) with the four list heads held in registers. The +inf tail
+// of every pooled row (child 0's empty slots, then child 1's, ...) is written afterwards by
+// the whole block in one coalesced pass over the block's contiguous output rows, so that
+// out_src stays a permutation of the children's slots.
+constexpr int kPoolThreads = 256;
+
+__global__ __launch_bounds__(kPoolThreads) void pool_forward_kernel(
+    int B, int X, int Y, int C, int K, const int *__restrict__ n, const float *__restrict__ t,
+    int *__restrict__ out_n, float *__restrict__ out_t, int *__restrict__ out_src) {
+  __shared__ int s_cnt[kPoolThreads][5];  // finite count, then the children's counts
   const int Xo = X / 2, Yo = Y / 2;
   const int No = Xo * Yo * C;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= B * No) return;
-  const int b = gid / No, jo = gid - b * No;
-  const int x = jo / (Yo * C), r = jo - x * (Yo * C), y = r / C, c = r - y * C;
   const int N = X * Y * C;
-  size_t base[4];
-  int cnt = 0;
-  for (int d = 0; d < 4; ++d) {
-    const int i = ((2 * x + (d >> 1)) * Y + (2 * y + (d & 1))) * C + c;
-    base[d] = ((size_t)b * N + i) * K;
-    cnt += n[(size_t)b * N + i];
-  }
-  out_n[gid] = cnt;
-  int p[4] = {0, 0, 0, 0};
-  const size_t obase = (size_t)gid * 4 * K;
-  for (int s = 0; s < 4 * K; ++s) {
-    int best = -1;
-    float bv = 0.f;
+  const int64_t total = (int64_t)B * No;
+  const int64_t g0 = (int64_t)blockIdx.x * kPoolThreads;
+  const int64_t gid = g0 + threadIdx.x;
+  if (gid < total) {
+    const int b = (int)(gid / No), jo = (int)(gid - (int64_t)b * No);
+    const int x = jo / (Yo * C), r = jo - x * (Yo * C), y = r / C, c = r - y * C;
+    size_t base[4];
+    int nd[4];
+    int cnt = 0;
+#pragma unroll
     for (int d = 0; d < 4; ++d) {
-      if (p[d] >= K) continue;
-      const float v = t[base[d] + p[d]];
-      // ascending with +inf last; ties keep the children's order
-      if (best < 0 || v < bv) {
-        best = d;
-        bv = v;
-      }
+      const int i = ((2 * x + (d >> 1)) * Y + (2 * y + (d & 1))) * C + c;
+      base[d] = ((size_t)b * N + i) * K;
+      nd[d] = min(max(n[(size_t)b * N + i], 0), K);
+      cnt += nd[d];
     }
-    out_t[obase + s] = bv;
-    out_src[obase + s] = (int)(base[best] + p[best]);
-    ++p[best];
+    out_n[gid] = cnt;
+    s_cnt[threadIdx.x][0] = cnt;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) s_cnt[threadIdx.x][1 + d] = nd[d];
+    int p[4] = {0, 0, 0, 0};
+    float head[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) head[d] = nd[d] > 0 ? t[base[d]] : kInfF;
+    const size_t obase = (size_t)gid * 4 * K;
+    for (int s = 0; s < cnt; ++s) {
+      int best = 0;
+      float bv = head[0];
+      bool live = p[0] < nd[0];
+#pragma unroll
+      for (int d = 1; d < 4; ++d) {
+        const bool ok = p[d] < nd[d];
+        if (ok && (!live || head[d] < bv)) {
+          best = d;
+          bv = head[d];
+          live = true;
+        }
+      }
+      out_t[obase + s] = bv;
+      size_t srcs = 0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+        if (d == best) {
+          srcs = base[d] + p[d];
+          ++p[d];
+          head[d] = p[d] < nd[d] ? t[base[d] + p[d]] : kInfF;
+        }
+      out_src[obase + s] = (int)srcs;
+    }
+  }
+  __syncthreads();
+  // the +inf tails of the block's rows, coalesced
+  const int rows = (int)min<int64_t>(kPoolThreads, total - g0);
+  const int slots = 4 * K;
+  for (int e = threadIdx.x; e < rows * slots; e += kPoolThreads) {
+    const int rr = e / slots, s = e - rr * slots;
+    const int cnt = s_cnt[rr][0];
+    if (s < cnt) continue;
+    const int64_t g = g0 + rr;
+    const int b = (int)(g / No), jo = (int)(g - (int64_t)b * No);
+    const int x = jo / (Yo * C), r = jo - x * (Yo * C), y = r / C, c = r - y * C;
+    int q = s - cnt;
+    int d = 0;
+    while (d < 3 && q >= K - s_cnt[rr][1 + d]) {
+      q -= K - s_cnt[rr][1 + d];
+      ++d;
+    }
+    const int i = ((2 * x + (d >> 1)) * Y + (2 * y + (d & 1))) * C + c;
+    const size_t o = (size_t)g * slots + s;
+    out_t[o] = kInfF;
+    out_src[o] = (int)(((size_t)b * N + i) * K + s_cnt[rr][1 + d] + q);
   }
 }
 
@@ -505,7 +555,7 @@ extern "C" int bats_pool2x2_forward(void *stream, int B, const int *pre_shape, i
   BATS_REQUIRE((double)B * X * Y * C * K < 2147483647.0, "pooling input above 2^31 slots");
   if (total == 0 || K == 0) return 0;
   BATS_REQUIRE(n && t && out_n && out_t && out_src, "null pointer");
-  hipLaunchKernelGGL(pool_forward_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(pool_forward_kernel, dim3((unsigned)((total + kPoolThreads - 1) / kPoolThreads)), dim3(kPoolThreads), 0,
                      as_stream(stream), B, X, Y, C, K, n, t, out_n, out_t, out_src);
   return check_launch("pool_forward_kernel");
 }

@@ -383,6 +383,122 @@ def cpu_baseline(args):
                 all_seconds=wall, cores=cores, per=per)
 
 
+# ---------------------------------------------------------------- the other configurations
+def _time_steps(step, warmup, steps):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def config_legs(args):
+    """Training-step time of the other network configurations (stream launches, eager; each
+    starts from its seeded weights): the single-spike TTFS network, the deep dense network
+    and the convolutional network. Thresholds and weight scales are stated in the output;
+    `spikes` is the mean number of spikes per sample of each LIF layer in the first step."""
+    import torch
+    from bats_amd import Network
+    from bats_amd.Layers import (ConvInputLayer, ConvLIFLayer, InputLayer, LIFLayer,
+                                 PoolingLayer)
+    from bats_amd.Losses import SpikeCountClassLoss, TTFSSoftmaxCrossEntropy
+    from bats_amd.Optimizers import AdamOptimizer
+
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(args.seed + 1)
+    out = {}
+
+    def dense_net(sizes, Ks, thetas, scale):
+        net = Network()
+        inp = InputLayer(n_neurons=sizes[0], name="input")
+        net.add_layer(inp, input=True)
+        prev = inp
+        for k, (n_post, K, th) in enumerate(zip(sizes[1:], Ks, thetas)):
+            w = (scale * rng.normal(0.0, 1.0, size=(n_post, sizes[k]))).astype(np.float32)
+            lay = LIFLayer(previous_layer=prev, n_neurons=n_post, tau_s=TAU_S, theta=th,
+                           delta_theta=th, weight_initializer=lambda a, b, w=w: w,
+                           max_n_spike=K, name=f"l{k}")
+            net.add_layer(lay)
+            prev = lay
+        return net
+
+    def run(name, net, batch, inputs, loss_fct, desc, steps=10, warmup=3):
+        t_d, n_d = inputs
+        lab = torch.as_tensor(rng.integers(0, 10, size=batch), device=dev)
+        opt = AdamOptimizer(learning_rate=LR)
+
+        def step():
+            net.reset()
+            net.forward(t_d, n_d, max_simulation=T_MAX, training=True)
+            ot, on = net.output_spike_trains
+            _, err = loss_fct.compute_loss_and_errors(ot, on, lab)
+            grads = net.backward(err)
+            net.apply_deltas(opt.step([None if g is None else g[0] for g in grads]))
+
+        step()
+        spikes = [round(float(l.spike_trains[1].sum().item()) / batch, 1)
+                  for l in net.layers[1:] if isinstance(l, (LIFLayer, ConvLIFLayer))]
+        ms = _time_steps(step, warmup, steps)
+        out[name] = {"workload": desc, "batch": batch, "ms_per_step": round(ms, 4),
+                     "samples_per_s": round(batch / (ms * 1e-3), 1), "steps": steps,
+                     "spikes": spikes}
+
+    def latency(batch):
+        x = rng.uniform(0.0, 1.0, size=(batch, N_IN))
+        t = np.where(x > 0.1, 1.0 - x, np.inf).astype(np.float32)[:, :, None]
+        return (torch.as_tensor(t, device=dev),
+                torch.as_tensor(np.isfinite(t[:, :, 0]).astype(np.int32), device=dev))
+
+    # single-spike TTFS network (784-800-10, K = 1, batch 50, softmax on first spike times)
+    run("ttfs_784-800-10_b50", dense_net([N_IN, N_HID, N_OUT], [1, 1], [1.0, 1.3], 1.0), 50,
+        latency(50), TTFSSoftmaxCrossEntropy(tau=0.005),
+        "784-800-10, K=1/1, theta 1.0/1.3, W~N(0,1), TTFS loss tau 0.005, Adam")
+    # deep dense network (784-800-800-800-10, K = 30, batch 256)
+    run("deep_784-800x3-10_b256",
+        dense_net([N_IN, 800, 800, 800, N_OUT], [30, 30, 30, 30], [1.0, 1.0, 1.0, 1.3],
+                  1.0), 256, latency(256), SpikeCountClassLoss(target_true=15, target_false=3),
+        "784-800-800-800-10, K=30, theta 1.0 (1.3 output), W~N(0,1), count loss 15/3, Adam")
+    # convolutional network (batch 128): conv5x5x15 (K 30) -> pool -> conv5x5x40 (K 3) ->
+    # pool -> 300 (K 10) -> 10 (K 30), thresholds of experiments/mnist/train_conv.py
+    net = Network()
+    inp = ConvInputLayer(neurons_shape=np.array([28, 28, 1]), name="input")
+    net.add_layer(inp, input=True)
+    # N(0, 1/sqrt(fan_in)) with gains 1, 8, 8, 8: the plain scale leaves conv2 and every
+    # later layer silent at these thresholds; with the gains every layer fires
+    w1 = rng.normal(0.0, 1.0 / 5.0, size=(15, 5, 5, 1)).astype(np.float32)
+    w2 = (8.0 * rng.normal(0.0, 1.0 / np.sqrt(5 * 5 * 15), size=(40, 5, 5, 15))).astype(np.float32)
+    w3 = (8.0 * rng.normal(0.0, 1.0 / np.sqrt(640), size=(300, 640))).astype(np.float32)
+    w4 = (8.0 * rng.normal(0.0, 1.0 / np.sqrt(300), size=(10, 300))).astype(np.float32)
+    c1 = ConvLIFLayer(inp, filters_shape=np.array([5, 5, 15]), tau_s=TAU_S, theta=0.04,
+                      delta_theta=0.04, weight_initializer=lambda *a: w1, max_n_spike=30,
+                      name="conv1")
+    net.add_layer(c1)
+    p1 = PoolingLayer(c1, name="pool1")
+    net.add_layer(p1)
+    c2 = ConvLIFLayer(p1, filters_shape=np.array([5, 5, 40]), tau_s=TAU_S, theta=0.8,
+                      delta_theta=0.8, weight_initializer=lambda *a: w2, max_n_spike=3,
+                      name="conv2")
+    net.add_layer(c2)
+    p2 = PoolingLayer(c2, name="pool2")
+    net.add_layer(p2)
+    d3 = LIFLayer(p2, n_neurons=300, tau_s=TAU_S, theta=0.6, delta_theta=0.6,
+                  weight_initializer=lambda *a: w3, max_n_spike=10, name="fc")
+    net.add_layer(d3)
+    d4 = LIFLayer(d3, n_neurons=10, tau_s=TAU_S, theta=0.3, delta_theta=0.3,
+                  weight_initializer=lambda *a: w4, max_n_spike=30, name="output")
+    net.add_layer(d4)
+    run("conv_28x28-c15-p-c40-p-300-10_b128", net, 128, latency(128),
+        SpikeCountClassLoss(target_true=15, target_false=3),
+        "conv5x5x15 K30 -> pool -> conv5x5x40 K3 -> pool -> 300 K10 -> 10 K30, thresholds "
+        "0.04/0.8/0.6/0.3, W~N(0,1/sqrt(fan_in)) x gains 1/8/8/8 (active variant), count loss "
+        "15/3, Adam")
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` at this workload, from the calibrated FETCH_SIZE /
     WRITE_SIZE counters recorded in profiles/traffic.json (None when absent)."""
@@ -414,6 +530,8 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--skip-configs", action="store_true",
+                    help="skip the TTFS, deep and conv training-step legs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -454,6 +572,7 @@ def main():
                  "output_count_agreement": float(np.mean(opt_counts[1] == pr["counts"][1]))}
 
     serve = None if args.skip_serve else serve_bench(args)
+    legs = None if args.skip_configs else config_legs(args)
 
     cpu = None
     if c is not None:
@@ -476,6 +595,7 @@ def main():
                                f"t_max=1.0, W~{'N(0,1) active' if args.variant == 'active' else 'N(0,1/sqrt(fan_in)) literal'} variant",
                    "global_batch": B, "seed": args.seed, "launch": launch},
         "roofline": roof, "cpu_baseline": cpu, "plain_form": plain, "serve": serve,
+        "other_configs": legs,
         "stream_ms_per_step": round(r["eager_ms"], 4),
         "graph_ms_per_step": round(r["graph_ms"], 4),
         "step_hbm_fraction": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),

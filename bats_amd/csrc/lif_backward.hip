@@ -533,6 +533,159 @@ __global__ __launch_bounds__(64 * kTileWaves) void pre_err_tile_kernel(
     for (int m = np; m < K_pre; ++m) pre_errors[pbase + m] = 0.f;
 }
 
+// ------------------------------------------------------------------ multi-spike pre layers
+// When the pre neurons fire several times (the previous layer is a LIF layer with K_pre > 1),
+// a lane per pre neuron would walk a tile once per spike of the busiest neuron of its wave.
+// These kernels give each lane one pre spike instead: compact_spikes_kernel lists, per
+// sample, the finite pre slots in neuron order as {t, es, e, neuron} with their slot index.
+
+constexpr int kCompThreads = 256;
+
+__global__ __launch_bounds__(kCompThreads) void compact_spikes_kernel(
+    int I, int K_pre, const float *__restrict__ pre_t, const float *__restrict__ pre_es,
+    const float *__restrict__ pre_e, int cap, float4 *__restrict__ spk, int *__restrict__ slot,
+    int *__restrict__ count) {
+  __shared__ int s_wave[kCompThreads / 64];
+  __shared__ int s_base;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) s_base = 0;
+  for (int i0 = 0; i0 < I; i0 += kCompThreads) {
+    const int i = i0 + tid;
+    const size_t row = ((size_t)b * I + (i < I ? i : 0)) * K_pre;
+    const int np = i < I ? leading_finite(pre_t + row, K_pre) : 0;
+    int incl = np;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    __syncthreads();  // s_base of the previous chunk is final; s_wave is free
+    if (lane == 63) s_wave[wv] = incl;
+    __syncthreads();
+    int off = s_base + incl - np;
+    for (int w = 0; w < wv; ++w) off += s_wave[w];
+    float4 *o = spk + (size_t)b * cap + off;
+    int *os = slot + (size_t)b * cap + off;
+    for (int m = 0; m < np; ++m) {
+      o[m] = make_float4(pre_t[row + m], pre_es[row + m], pre_e[row + m], __int_as_float(i));
+      os[m] = i * K_pre + m;
+    }
+    __syncthreads();
+    if (tid == kCompThreads - 1) {
+      int tot = 0;
+      for (int w = 0; w < kCompThreads / 64; ++w) tot += s_wave[w];
+      s_base += tot;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) count[b] = s_base;
+}
+
+// Weight gradient with a lane per pre spike: one workgroup per (tile of kWT post rows, batch
+// slice) accumulates acc[row][pre neuron] in float64 in LDS. Per sample the tile's record
+// (row_prep_kernel) is copied to LDS, and each wave takes 64-spike chunks of the sample's
+// compacted pre spikes and walks the record, adding es d1 - e d2 (t_pre < t_post) with a
+// no-return LDS atomic. Lanes of one neuron (and, at a chunk boundary, two waves) may add to
+// one address, so the float64 sums are exact up to the order of those additions.
+__host__ __device__ inline size_t spk_wgrad_lds_bytes(int I, int K) {
+  return (size_t)kWT * I * sizeof(double) + (size_t)tile_rec_slots(K) * sizeof(float4);
+}
+
+__global__ __launch_bounds__(1024) void wgrad_spk_kernel(
+    int B, int I, int J, int K, int gj, int b_per_slice, int cap, const float4 *__restrict__ spk,
+    const int *__restrict__ count, const float4 *__restrict__ rec,
+    float *__restrict__ partial) {
+  extern __shared__ __align__(16) char smem[];
+  double *acc = reinterpret_cast<double *>(smem);  // [kWT][I]
+  float4 *rb = reinterpret_cast<float4 *>(acc + (size_t)kWT * I);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nth >> 6;
+  const int tj = blockIdx.x, sl = blockIdx.y;
+  const int j0 = tj * kWT, jn = min(kWT, J - j0);
+  const int b0 = sl * b_per_slice, b1 = min(B, b0 + b_per_slice);
+  const int R = tile_rec_slots(K);
+  for (int e = tid; e < kWT * I; e += nth) acc[e] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const float4 *grec = rec + ((size_t)b * gj + tj) * R;
+    __syncthreads();  // the previous record is consumed (and acc is zeroed)
+    const int cnt = __float_as_int(grec[0].x);
+    for (int e = tid; e < cnt; e += nth) rb[e] = grec[kRecHead + e];
+    __syncthreads();
+    const int P = count[b];
+    for (int c = wv; c * 64 < P; c += nw) {
+      const int sidx = c * 64 + lane;
+      const bool valid = sidx < P;
+      const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
+      double *al = acc + (valid ? __float_as_int(S.w) : 0);
+      for (int k = 0; k < cnt; ++k) {
+        const float4 E = rb[k];
+        float q = fmaf(S.y, E.y, -(S.z * E.z));
+        asm volatile("" : "+v"(q));
+        q = (S.x < E.x) ? q : 0.f;
+        if (valid)
+          __hip_atomic_fetch_add(al + (size_t)(__float_as_int(E.w) >> 9) * I, (double)q,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < jn * I; e += nth) {
+    const int u = e / I, i = e - u * I;
+    partial[((size_t)sl * J + j0 + u) * I + i] = (float)acc[e];
+  }
+}
+
+// Pre-spike errors with a lane per pre spike: a workgroup of kTileWaves waves per (256 pre
+// spikes, sample) walks all tile records of the sample; per tile each lane stages its 16
+// weights W[row][neuron] in the wave's LDS block, and every entry adds
+//   w[row] * (es' d1 - e' d2)  if t_pre < t_post   (float32 in the tile, float64 across).
+// pre_errors must be zero-filled beforehand (slots without a spike keep 0).
+__global__ __launch_bounds__(64 * kTileWaves) void pre_err_spk_kernel(
+    int I, int K_pre, int J, int K, int gj, int cap, const float4 *__restrict__ spk,
+    const int *__restrict__ slot, const int *__restrict__ count, const float *__restrict__ W,
+    const float4 *__restrict__ rec, double inv_tau_s, double inv_tau,
+    float *__restrict__ pre_errors) {
+  extern __shared__ __align__(16) char smem[];
+  float *wl = reinterpret_cast<float *>(smem);  // [kTileWaves][kWT][64]
+  float4 *rb = reinterpret_cast<float4 *>(wl + kTileWaves * kWT * 64);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int b = blockIdx.y;
+  const int P = count[b];
+  const int s0 = blockIdx.x * (64 * kTileWaves);
+  if (s0 >= P) return;  // whole block
+  const int sidx = s0 + tid;
+  const bool valid = sidx < P;
+  const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
+  const int i = valid ? __float_as_int(S.w) : 0;
+  const float esd = (float)((double)S.y * inv_tau_s), ed = (float)((double)S.z * inv_tau);
+  float *ww = wl + wv * kWT * 64;
+  const int R = tile_rec_slots(K);
+  double pe = 0.0;
+  for (int tj = 0; tj < gj; ++tj) {
+    const float4 *grec = rec + ((size_t)b * gj + tj) * R;
+    const int j0 = tj * kWT;
+    __syncthreads();  // the previous tile is consumed
+    const int cnt = __float_as_int(grec[0].x);
+    for (int e = tid; e < cnt; e += 64 * kTileWaves) rb[e] = grec[kRecHead + e];
+#pragma unroll
+    for (int u = 0; u < kWT; ++u)
+      ww[u * 64 + lane] = (valid && j0 + u < J) ? W[(size_t)(j0 + u) * I + i] : 0.f;
+    __syncthreads();
+    float pt = 0.f;
+    for (int k = 0; k < cnt; ++k) {
+      const float4 E = rb[k];
+      const float w = ww[(__float_as_int(E.w) >> 9) * 64 + lane];
+      const float q = fmaf(esd, E.y, -(ed * E.z));
+      pt = fmaf((S.x < E.x) ? w : 0.f, q, pt);
+    }
+    pe += (double)pt;
+  }
+  if (valid) pre_errors[(size_t)b * I * K_pre + slot[(size_t)b * cap + sidx]] = (float)pe;
+}
+
 // ------------------------------------------------------------------ narrow layers
 // Layers with at most kST post neurons (an output layer): one block per (pre-neuron block,
 // sample). The sample's raw post rows are copied to LDS, one thread per row runs the reset
@@ -707,10 +860,14 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
 }
 
 struct BwdLayout {
-  size_t neff, s1, s2, rec, partial, total;
+  size_t rec, spk, slot, count, partial, total;
   int slices, b_per_slice, lanes, gi;
-  bool small;
+  int spk_slices, spk_b_per_slice;  // the lane-per-pre-spike weight gradient's batch slices
+  bool small, spk_path;
 };
+
+// LDS a workgroup may use (the spike-lane weight gradient needs kWT x I float64 accumulators)
+constexpr size_t kMaxWorkgroupLds = 160 * 1024 - 1024;
 
 // Waves the weight-gradient grid aims at: about six per SIMD of the 256-CU device, so that
 // the scalar-load latency of one wave is covered by the others and the last round is short.
@@ -732,13 +889,32 @@ static BwdLayout bwd_layout(int B, int I, int K_pre, int J, int K, bool per_samp
     L.b_per_slice = B > 0 ? (B + slices - 1) / slices : 1;
     L.slices = B > 0 ? (B + L.b_per_slice - 1) / L.b_per_slice : 1;
   }
+  // multi-spike pre layers: compacted pre spikes, and (when its tile fits in LDS) the
+  // spike-lane weight gradient over ~2 workgroups per CU
+  const bool multi = !L.small && K_pre > 1;
+  L.spk_path = multi && !per_sample && spk_wgrad_lds_bytes(I, K) <= kMaxWorkgroupLds;
+  if (L.spk_path) {
+    const int gj = (J + kWT - 1) / kWT;
+    const int sl = std::max(1, std::min(std::max(B, 1), (512 + gj - 1) / gj));
+    L.spk_b_per_slice = B > 0 ? (B + sl - 1) / sl : 1;
+    L.spk_slices = B > 0 ? (B + L.spk_b_per_slice - 1) / L.spk_b_per_slice : 1;
+  }
+  const size_t cap = (size_t)I * K_pre;
   size_t off = 0;
+  L.spk = off;
+  if (multi) off = align_up(off + sizeof(float4) * (size_t)B * cap, 256);
+  L.slot = off;
+  if (multi) off = align_up(off + sizeof(int) * (size_t)B * cap, 256);
+  L.count = off;
+  if (multi) off = align_up(off + sizeof(int) * (size_t)B, 256);
   L.rec = off;
   if (!L.small)
     off = align_up(off + sizeof(float4) * (size_t)B * ((J + kWT - 1) / kWT) * tile_rec_slots(K),
                    256);
   L.partial = off;
-  off = align_up(off + sizeof(float) * (size_t)L.slices * J * I, 256);
+  off = align_up(off + sizeof(float) * (size_t)std::max(L.slices, L.spk_path ? L.spk_slices : 0) *
+                           J * I,
+                 256);
   L.total = off;
   return L;
 }
@@ -801,9 +977,10 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   BATS_REQUIRE(workspace != nullptr || L.total == 0, "null workspace");
   BATS_REQUIRE(workspace_bytes >= L.total, "workspace too small");
   char *ws = static_cast<char *>(workspace);
-  double *S1 = reinterpret_cast<double *>(ws + L.s1);
-  double *S2 = reinterpret_cast<double *>(ws + L.s2);
   float4 *rec = reinterpret_cast<float4 *>(ws + L.rec);
+  float4 *spk = reinterpret_cast<float4 *>(ws + L.spk);
+  int *spk_slot = reinterpret_cast<int *>(ws + L.slot);
+  int *spk_count = reinterpret_cast<int *>(ws + L.count);
   float *partial = reinterpret_cast<float *>(ws + L.partial);
   // the reset chain alone, written back in place
   auto reset_only = [&]() -> int {
@@ -841,7 +1018,27 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                                        (double)tau, (double)delta_theta_tau, nullptr,
                                        nullptr, nullptr, rec));
   if (prep_rc) return prep_rc;
-  {
+  const bool multi = K_pre > 1;
+  const int cap = I * K_pre;
+  if (multi) {
+    BATS_REQUIRE((int64_t)I * K_pre < (int64_t)1 << 31, "pre spike buffer too large");
+    BATS_TIMED("compact_spikes_kernel",
+               st, hipLaunchKernelGGL(compact_spikes_kernel, dim3(B), dim3(kCompThreads), 0, st,
+                                      I, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau, cap, spk,
+                                      spk_slot, spk_count));
+    if (int r = check_launch("compact_spikes_kernel")) return r;
+  }
+  if (L.spk_path && grad_mode != 1) {
+    const int gj = (J + kWT - 1) / kWT;
+    const size_t lds = spk_wgrad_lds_bytes(I, K_post);
+    if (int r = allow_lds((const void *)wgrad_spk_kernel, lds)) return r;
+    BATS_TIMED("wgrad_spk_kernel", st,
+               hipLaunchKernelGGL(wgrad_spk_kernel, dim3(gj, L.spk_slices), dim3(1024), lds, st,
+                                  B, I, J, K_post, gj, L.spk_b_per_slice, cap, spk, spk_count,
+                                  rec, partial));
+    if (int r = check_launch("wgrad_spk_kernel")) return r;
+    if (int r = reduce_slices(st, L.spk_slices, JI, inv_b, partial, grad)) return r;
+  } else {
     const BwdLayout F = bwd_layout(B, I, K_pre, J, K_post, grad_mode == 1);
     const int gi = (I + 63) / 64, gj = (J + kWT - 1) / kWT;
     const int64_t ntasks = (int64_t)gi * gj * F.slices;
@@ -858,7 +1055,24 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
     if (grad_mode != 1)
       if (int r = reduce_slices(st, F.slices, JI, inv_b, partial, grad)) return r;
   }
-  if (want_pre) {
+  if (want_pre && multi) {
+    const int64_t nfill = (int64_t)B * I * K_pre;
+    hipLaunchKernelGGL(fill_kernel<float>,
+                       dim3((unsigned)std::min<int64_t>((nfill + 255) / 256, 65535)), dim3(256), 0,
+                       st, pre_errors, nfill, 0.f);
+    if (int r = check_launch("fill pre errors")) return r;
+    const int gj = (J + kWT - 1) / kWT;
+    const size_t lds = (size_t)kTileWaves * kWT * 64 * sizeof(float) +
+                       (size_t)tile_rec_slots(K_post) * sizeof(float4);
+    if (int r = allow_lds((const void *)pre_err_spk_kernel, lds)) return r;
+    const int groups = (cap + 64 * kTileWaves - 1) / (64 * kTileWaves);
+    BATS_TIMED("pre_err_spk_kernel", st,
+               hipLaunchKernelGGL(pre_err_spk_kernel, dim3(groups, B), dim3(64 * kTileWaves), lds,
+                                  st, I, K_pre, J, K_post, gj, cap, spk, spk_slot, spk_count,
+                                  weights, rec, 1.0 / (double)tau_s, 1.0 / (double)tau,
+                                  pre_errors));
+    if (int r = check_launch("pre_err_spk_kernel")) return r;
+  } else if (want_pre) {
     const int gi = (I + 63) / 64;
     const int64_t ntasks = (int64_t)gi * B;
     BATS_REQUIRE(ntasks < (int64_t)1 << 31, "pre-spike error grid too large");

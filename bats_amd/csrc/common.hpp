@@ -106,6 +106,15 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
                       float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
                       void *ws, EventLists *out);
 
+// Workgroups are dealt to the 8 XCDs round robin; this maps dispatch index p of nblk to a
+// logical index such that consecutive logical blocks run on one XCD and share its L2.
+__device__ __forceinline__ int xcd_logical_block(int p, int nblk) {
+  constexpr int kXcd = 8;
+  const int q = nblk / kXcd, r = nblk % kXcd;
+  const int xcd = p % kXcd, k = p / kXcd;
+  return xcd * q + min(xcd, r) + k;
+}
+
 // Fill kernel (used instead of hipMemsetAsync so that every call stays graph-capturable).
 template <typename T>
 __global__ void fill_kernel(T *__restrict__ p, int64_t n, T v) {

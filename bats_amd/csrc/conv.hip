@@ -1,14 +1,10 @@
 // Convolutional LIF layer (valid, stride 1) and 2x2 spike pooling for gfx950.
 //
 // Forward: the per-sample event lists of the dense layer (sort_events_kernel) are shared;
-// conv_solve_kernel gives every lane one (position, output channel) of a spatial tile of the
-// output. The block compacts the sample's events that fall inside the tile's joint receptive
-// field into LDS (order preserved by a block scan), and each lane walks them, using only the
-// events inside its own receptive field. The crossing check of an interval is made when the
-// lane reaches its next receptive-field event (or at the end, against t_max): an event outside
-// the field has a zero weight and leaves a and b unchanged, so this is the same decision
-// sequence as the reference's kernel, which checks after every event of the sample
-// (compute_spike_times_conv.cu:155-173).
+// conv_solve_kernel gives one wave a small tile of output positions and channels, keeps the
+// events inside the tile's joint receptive field with a ballot, and runs the dense solver's
+// per-event arithmetic (clamp filter, deferred closed form) on them, each lane using only the
+// events inside its own field (compute_spike_times_conv.cu:155-173).
 //
 // Backward: post_prep_kernel of the dense layer prepares the per-spike suffix sums; then
 // conv_wgrad_kernel (one block per filter tap and sample, lanes over output positions, block
@@ -42,115 +38,185 @@ __global__ void conv_filter_transpose_kernel(const float *__restrict__ W, float 
   WT[(size_t)r * Co + c] = W[e];
 }
 
-struct ConvEv {
-  Event ev;
-  int px, py, pc, pad;
+// ---------------------------------------------------------------- forward solver
+// One wave per (sample, tile of PX x PY output positions, group of CW output channels);
+// lane = (position in the tile, channel). The sample's time-sorted event list is read 64
+// pre-neuron indices at a time and a ballot keeps the events inside the tile's joint
+// receptive field; each kept event is processed by the whole wave with wave-uniform event
+// data (scalar loads), kConvBatch events at a time so that their weight loads overlap.
+// A lane uses only the events inside its own receptive field ("own" events) and checks the
+// interval (t_last, min(t_u, t_max)] since its previous own event when it reaches the next
+// own event u, and (t_last, t_max] at the end: an event outside the field has weight zero and
+// leaves a and b unchanged, so this is the decision sequence of the reference's kernel, which
+// checks after every event of the sample (compute_spike_times_conv.cu:155-173). The check is
+// the dense solver's: the clamp filter, then the closed form in z = exp(t/tau) space with
+// the spike times formed after the walk.
+constexpr int kConvWaves = 4;
+constexpr int kConvBatch = 8;
+
+struct ConvTiling {
+  int PX, PY, CW;        // output positions per wave (PX x PY) and channels per wave
+  int tilesX, tilesY, cgroups;
+  float inv_ci, inv_y;   // 1/Ci and 1/Y (float32) for the pre-neuron index decomposition
 };
 
-constexpr int kConvMaxThreads = 1024;
+// Interval ends of the conv solver: the lane's previous own event and the event closing the
+// interval (null: t_max).
+struct ConvEnds {
+  const Event *lo_ev;
+  const Event *hi_ev;
+  float tmax;
+  __device__ float lo() const { return lo_ev->t; }
+  __device__ float hi() const { return hi_ev ? fminf(hi_ev->t, tmax) : tmax; }
+};
 
-__global__ __launch_bounds__(kConvMaxThreads) void conv_solve_kernel(
-    const Event *__restrict__ events, const int *__restrict__ counts, int Pcap, ConvGeom g,
-    int TX, int TY, const float *__restrict__ WT, int K, double c, double dtt, double tau,
-    float tmax, int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
+// n / d for 0 <= n < 2^24 and d >= 1 with inv = 1.0f / d: the float32 quotient is within one
+// of the exact one (exact for d = 1), and one correction step fixes it.
+__device__ __forceinline__ int div_small(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  const int r = n - q * d;
+  q += (r >= d) ? 1 : 0;
+  q -= (r < 0) ? 1 : 0;
+  return q;
+}
+
+// kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every own interval is solved
+// with the reference's loop in t space.
+template <bool kAlwaysExact>
+__global__ __launch_bounds__(64 * kConvWaves) void conv_solve_kernel(
+    const Event *__restrict__ events, const FastEvent *__restrict__ fast,
+    const int *__restrict__ evidx, const int *__restrict__ counts, int Pcap, ConvGeom g,
+    ConvTiling tl, int units, const float *__restrict__ WT, int K, double c, double dtt,
+    double tau, float tmax, int *__restrict__ out_n, float *__restrict__ out_t,
     float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
-  extern __shared__ __align__(16) char smem[];
-  ConvEv *sev = reinterpret_cast<ConvEv *>(smem);
-  int *s_scan = reinterpret_cast<int *>(sev + blockDim.x);
-  const int tilesY = (g.Yo + TY - 1) / TY;
-  const int x0 = (blockIdx.x / tilesY) * TX;
-  const int y0 = (blockIdx.x % tilesY) * TY;
-  const int b = blockIdx.y;
-  const int l = threadIdx.x;
-  const int cc = l % g.Co;
-  const int pos = l / g.Co;
-  const int x = x0 + pos / TY;
-  const int y = y0 + pos % TY;
-  const bool active = pos < TX * TY && x < g.Xo && y < g.Yo;
-  const int j = active ? (x * g.Yo + y) * g.Co + cc : 0;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int unit = xcd_logical_block(blockIdx.x, gridDim.x) * kConvWaves + wave;
+  if (unit >= units) return;  // whole wave; no workgroup barrier follows
+  const int lane = threadIdx.x & 63;
+  const int tiles = tl.tilesX * tl.tilesY;
+  const int cg = unit % tl.cgroups;
+  const int tile = (unit / tl.cgroups) % tiles;
+  const int b = unit / (tl.cgroups * tiles);
+  const int x0 = (tile / tl.tilesY) * tl.PX, y0 = (tile % tl.tilesY) * tl.PY;
+  const int pos = lane / tl.CW, cl = lane - pos * tl.CW;
+  const int ox = pos / tl.PY, oy = pos - ox * tl.PY;
+  const int cc = cg * tl.CW + cl;
+  const bool active =
+      pos < tl.PX * tl.PY && x0 + ox < g.Xo && y0 + oy < g.Yo && cc < g.Co;
+  const int j = active ? ((x0 + ox) * g.Yo + (y0 + oy)) * g.Co + cc : 0;
   const size_t row = ((size_t)b * g.Xo * g.Yo * g.Co + j) * K;
-  // joint receptive field of the tile
-  const int rx0 = x0, rx1 = min(g.X, x0 + TX + g.fx - 1);
-  const int ry0 = y0, ry1 = min(g.Y, y0 + TY + g.fy - 1);
+  // weight of an own event: WT[ebase - loff], ebase = ((dx0 fy + dy0) Ci + pc) Co with
+  // (dx0, dy0) the event's offset from the tile origin
+  const int loff = (ox * g.fy + oy) * g.Ci * g.Co - cc;
+  const int ux = tl.PX + g.fx - 1, uy = tl.PY + g.fy - 1;  // joint field of the tile
   const int P = counts[b];
   const Event *ev = events + (size_t)b * Pcap;
-  const double c4 = 4.0 * c;
-  const double zmax = exp((double)tmax / tau);
-  double a = 0.0, bb = 0.0, zq_last = 0.0;
-  float t_last = -kInfF;
-  bool seen = false, live = active;
-  int n = 0;
-  for (int base = 0; base < P; base += blockDim.x) {
-    const int q = base + l;
-    int flag = 0;
-    Event e{};
-    int px = 0, py = 0, pc = 0;
-    if (q < P) {
-      e = ev[q];
-      px = e.idx / (g.Y * g.Ci);
-      const int r = e.idx - px * (g.Y * g.Ci);
-      py = r / g.Ci;
-      pc = r - py * g.Ci;
-      flag = (px >= rx0 && px < rx1 && py >= ry0 && py < ry1) ? 1 : 0;
+  const FastEvent *fv = fast + (size_t)b * Pcap;
+  const int *ix = evidx + (size_t)b * Pcap;
+  const double inv2c = 0.5 / c;
+  const double zmax_hi = fmin(exp((double)tmax / tau), 1e300) * (1.0 + kFilterEps);
+  constexpr double kLoToHi = (1.0 + kFilterEps) / (1.0 - kFilterEps);
+  // an inactive lane, or one whose buffer is full, carries a = -inf (it fails a > 0); the
+  // always-exact form uses `live`
+  double a = (kAlwaysExact || active) ? 0.0 : -kInfD, bb = 0.0;
+  double zq_last = 0.0;  // zq_lo of the lane's previous own event
+  int last_q = 0, n = 0;
+  bool live = active, seen = false;
+  int qn = (lane < P) ? ix[lane] : -1;
+  for (int base = 0; base < P; base += 64) {
+    const int qi = qn;
+    if (base + 64 < P) qn = (base + 64 + lane < P) ? ix[base + 64 + lane] : -1;
+    int ebase = 0, dpk = 0;
+    bool inu = false;
+    if (qi >= 0) {
+      const int r = div_small(qi, g.Ci, tl.inv_ci);
+      const int pc = qi - r * g.Ci;
+      const int px = div_small(r, g.Y, tl.inv_y);
+      const int dx0 = px - x0, dy0 = r - px * g.Y - y0;
+      inu = ((unsigned)dx0 < (unsigned)ux) & ((unsigned)dy0 < (unsigned)uy);
+      ebase = ((dx0 * g.fy + dy0) * g.Ci + pc) * g.Co;
+      dpk = (dx0 << 16) | dy0;
     }
-    // block-wide exclusive scan of the flags (Hillis-Steele in LDS)
-    __syncthreads();  // the previous chunk has been consumed
-    s_scan[l] = flag;
-    __syncthreads();
-    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-      const int v = (l >= d) ? s_scan[l - d] : 0;
-      __syncthreads();
-      s_scan[l] += v;
-      __syncthreads();
-    }
-    const int total = s_scan[blockDim.x - 1];
-    if (flag) {
-      ConvEv ce;
-      ce.ev = e;
-      ce.px = px;
-      ce.py = py;
-      ce.pc = pc;
-      ce.pad = 0;
-      sev[s_scan[l] - 1] = ce;
-    }
-    __syncthreads();
-    for (int u = 0; u < total; ++u) {
-      const int dx = sev[u].px - x, dy = sev[u].py - y;
-      if (live && dx >= 0 && dx < g.fx && dy >= 0 && dy < g.fy) {
-        const float tu = sev[u].ev.t;
-        if (seen) {
-          // interval (t_last, min(tu, t_max)] with the state after the previous field event
-          const float tl = fminf(tu, tmax);
-          const double zl = (tu <= tmax) ? sev[u].ev.zq : zmax;
-          if (always_exact || crossing_filter3(a, bb, zq_last, zl, c, c4)) {
-            const SolveState r = solve_crossings_ool(a, bb, n, (double)t_last, (double)tl, c,
-                                                     dtt, tau, K, row, out_t, out_a, out_x,
-                                                     out_z);
+    unsigned long long m = __ballot(inu);
+    while (m) {
+      int ql[kConvBatch];
+#pragma unroll
+      for (int k = 0; k < kConvBatch; ++k) {
+        ql[k] = m ? (int)__builtin_ctzll(m) : -1;
+        m &= m - 1;
+      }
+      float w[kConvBatch];
+      bool own[kConvBatch];
+#pragma unroll
+      for (int k = 0; k < kConvBatch; ++k) {
+        own[k] = false;
+        w[k] = 0.f;
+        if (ql[k] >= 0) {
+          const int eb = __builtin_amdgcn_readlane(ebase, ql[k]);
+          const int dp = __builtin_amdgcn_readlane(dpk, ql[k]);
+          own[k] = active & ((unsigned)((dp >> 16) - ox) < (unsigned)g.fx) &
+                   ((unsigned)((dp & 0xffff) - oy) < (unsigned)g.fy);
+          if (own[k]) w[k] = WT[eb - loff];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kConvBatch; ++k) {
+        if (ql[k] < 0) break;
+        const int q = base + ql[k];
+        const FastEvent f = fv[q];
+        const double zl_hi = fmin(f.zq_lo * kLoToHi, zmax_hi);
+        if (kAlwaysExact) {
+          if (own[k] & live & seen) {
+            const SolveState r =
+                solve_crossings_ool(a, bb, n, (double)ev[last_q].t,
+                                    (double)fminf(ev[q].t, tmax), c, dtt, tau, K, row, out_t,
+                                    out_a, out_x, out_z);
             bb = r.b;
             n = r.n;
             live = r.live != 0;
           }
+        } else {
+          const double zv = fmin(fmax(bb * inv2c, zq_last), zl_hi);
+          const double gq = fma(fma(c, zv, -bb), zv, a);
+          if (own[k] & (a > 0.0) & (gq <= kFilterEps * a)) {
+            const SolveStateZ r = solve_crossings_deferred_ends(
+                a, bb, n, zq_last, zl_hi, ConvEnds{ev + last_q, ev + q, tmax}, c, inv2c, dtt,
+                tau, K, row, out_t, out_a, out_x, out_z);
+            bb = r.b;
+            n = r.n;
+            if (!r.live) a = -kInfD;
+          }
         }
-        if (live) {
-          const double w = (double)WT[((size_t)(dx * g.fy + dy) * g.Ci + sev[u].pc) * g.Co + cc];
-          a = fma(w, sev[u].ev.es, a);
-          bb = fma(w, sev[u].ev.e, bb);
-          t_last = tu;
-          zq_last = sev[u].ev.zq;
+        const double w64 = (double)w[k];
+        a = fma(w64, f.es, a);
+        bb = fma(w64, f.e, bb);
+        if (own[k]) {
+          zq_last = f.zq_lo;
+          last_q = q;
           seen = true;
         }
       }
     }
   }
-  if (live && seen) {
-    const double zl = zmax;
-    if (always_exact || crossing_filter3(a, bb, zq_last, zl, c, c4)) {
-      const SolveState r = solve_crossings_ool(a, bb, n, (double)t_last, (double)tmax, c, dtt,
-                                               tau, K, row, out_t, out_a, out_x, out_z);
+  // the last interval, up to t_max
+  if (kAlwaysExact) {
+    if (live & seen) {
+      const SolveState r = solve_crossings_ool(a, bb, n, (double)ev[last_q].t, (double)tmax, c,
+                                               dtt, tau, K, row, out_t, out_a, out_x, out_z);
+      n = r.n;
+    }
+  } else {
+    const double zv = fmin(fmax(bb * inv2c, zq_last), zmax_hi);
+    const double gq = fma(fma(c, zv, -bb), zv, a);
+    if ((a > 0.0) & (gq <= kFilterEps * a)) {
+      const SolveStateZ r = solve_crossings_deferred_ends(
+          a, bb, n, zq_last, zmax_hi, ConvEnds{ev + last_q, nullptr, tmax}, c, inv2c, dtt, tau,
+          K, row, out_t, out_a, out_x, out_z);
       n = r.n;
     }
   }
   if (active) {
+    if (!kAlwaysExact) finish_spike_times(row, n, tau, out_t, out_z);
     out_n[(size_t)b * g.Xo * g.Yo * g.Co + j] = n;
     for (int k = n; k < K; ++k) out_t[row + k] = kInfF;
   }
@@ -381,19 +447,38 @@ int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const
               const float *x, const float *z, float *errors, float c, float tau, float dtt,
               int *n_eff, double *S1, double *S2);
 
-static void conv_tile(const ConvGeom &g, int &TX, int &TY) {
-  // as many positions as fit 512 lanes with all output channels, square-ish
-  int per = std::max(1, 512 / g.Co);
-  TX = 1;
-  TY = 1;
-  while (TX * TY * 2 <= per && (TX < g.Xo || TY < g.Yo)) {
-    if (TY <= TX && TY < g.Yo)
-      TY *= 2;
-    else if (TX < g.Xo)
-      TX *= 2;
-    else
-      TY *= 2;
+// Wave tiling of the conv solver. A wave reads the whole event list of its sample once (cost
+// ~ 1/64 per event) and processes the events inside its joint field (cost ~ 1 per event), so
+// the work of a layer is ~ waves x (1/64 + joint field / input plane); the tiling that
+// minimises it is taken (more positions per wave share the scan, but widen the joint field).
+static ConvTiling conv_tiling(const ConvGeom &g) {
+  ConvTiling best{};
+  double best_cost = 1e300;
+  const int g0 = (g.Co + 63) / 64;
+  for (int groups = g0; groups <= std::min(g.Co, 4 * g0); ++groups) {
+    const int CW = (g.Co + groups - 1) / groups;
+    if (CW > 64 || (groups > g0 && (g.Co + CW - 1) / CW != groups)) continue;
+    const int npos = 64 / CW;
+    for (int PX = 1; PX <= std::min(npos, g.Xo); ++PX) {
+      const int PY = std::min(npos / PX, g.Yo);
+      const int tx = (g.Xo + PX - 1) / PX, ty = (g.Yo + PY - 1) / PY;
+      const double waves = (double)groups * tx * ty;
+      const double field = (double)(PX + g.fx - 1) * (PY + g.fy - 1) / ((double)g.X * g.Y);
+      const double cost = waves * (1.0 / 64.0 + field);
+      if (cost < best_cost) {
+        best_cost = cost;
+        best.PX = PX;
+        best.PY = PY;
+        best.CW = CW;
+        best.tilesX = tx;
+        best.tilesY = ty;
+        best.cgroups = groups;
+      }
+    }
   }
+  best.inv_ci = 1.0f / (float)g.Ci;
+  best.inv_y = 1.0f / (float)g.Y;
+  return best;
 }
 
 }  // namespace bats
@@ -410,7 +495,8 @@ static bool conv_geom(const int *pre_shape, const int *filters_shape, ConvGeom &
   g.Xo = g.X - g.fx + 1;
   g.Yo = g.Y - g.fy + 1;
   return g.X > 0 && g.Y > 0 && g.Ci > 0 && g.fx > 0 && g.fy > 0 && g.Co > 0 && g.Xo > 0 &&
-         g.Yo > 0 && g.Co <= 1024;
+         g.Yo > 0 && g.Co <= 1024 && (long long)g.X * g.Y * g.Ci < (1LL << 24) &&
+         (long long)(g.fx + 64) * (g.fy + 64) * g.Ci * g.Co < INT32_MAX;
 }
 
 extern "C" size_t bats_conv_lif_forward_workspace_size(int B, const int *pre_shape, int K_pre,
@@ -432,7 +518,8 @@ extern "C" int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, 
                                      size_t workspace_bytes) {
   ConvGeom g;
   BATS_REQUIRE(pre_shape && filters_shape && conv_geom(pre_shape, filters_shape, g),
-               "bad conv shapes (pre_shape = {X, Y, C_in}, filters_shape = {fx, fy, C_out})");
+               "bad conv shapes (pre_shape = {X, Y, C_in}, filters_shape = {fx, fy, C_out}; "
+               "C_out <= 1024, X * Y * C_in < 2^24)");
   BATS_REQUIRE(B >= 0 && K_pre >= 0 && K_post >= 1, "bad sizes");
   BATS_REQUIRE(B <= 65535, "batch above 65535");
   if (B == 0) return 0;
@@ -453,24 +540,25 @@ extern "C" int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, 
   if (int r = build_event_lists(st, B, S, K_pre, pre_times, tau_s, tau, c, max_simulation,
                                 pre_exp_tau_s, pre_exp_tau, ev_ws, &el))
     return r;
-  int TX, TY;
-  conv_tile(g, TX, TY);
-  const int lanes = TX * TY * g.Co;
-  const int threads = std::min(kConvMaxThreads, (lanes + 63) / 64 * 64);
-  const int tiles = ((g.Xo + TX - 1) / TX) * ((g.Yo + TY - 1) / TY);
-  const size_t lds = sizeof(ConvEv) * threads + sizeof(int) * threads;
-  if (lds > 65536) {
-    // more than 655 output channels: one position per block needs up to 100 KB of LDS
-    BATS_REQUIRE(hipFuncSetAttribute((const void *)conv_solve_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds) == hipSuccess,
-                 "conv_solve_kernel: LDS request above the device limit");
+  const ConvTiling tl = conv_tiling(g);
+  const long long units = (long long)B * tl.tilesX * tl.tilesY * tl.cgroups;
+  BATS_REQUIRE(units <= (long long)INT32_MAX - 64, "conv layer too large");
+  const int blocks = (int)((units + kConvWaves - 1) / kConvWaves);
+  if (c > 0.f) {
+    BATS_TIMED("conv_solve_kernel", st,
+               hipLaunchKernelGGL(conv_solve_kernel<false>, dim3(blocks), dim3(64 * kConvWaves),
+                                  0, st, el.events, el.fast, el.idx, el.counts, el.Pcap, g, tl,
+                                  (int)units, WT, K_post, (double)c, (double)delta_theta_tau,
+                                  (double)tau, max_simulation, out_n, out_t, out_a, out_x,
+                                  out_z));
+  } else {
+    BATS_TIMED("conv_solve_kernel", st,
+               hipLaunchKernelGGL(conv_solve_kernel<true>, dim3(blocks), dim3(64 * kConvWaves),
+                                  0, st, el.events, el.fast, el.idx, el.counts, el.Pcap, g, tl,
+                                  (int)units, WT, K_post, (double)c, (double)delta_theta_tau,
+                                  (double)tau, max_simulation, out_n, out_t, out_a, out_x,
+                                  out_z));
   }
-  BATS_TIMED("conv_solve_kernel", st,
-             hipLaunchKernelGGL(conv_solve_kernel, dim3(tiles, B), dim3(threads), lds, st,
-                                el.events, el.counts, el.Pcap, g, TX, TY, WT, K_post, (double)c,
-                                (double)delta_theta_tau, (double)tau, max_simulation,
-                                (c > 0.f) ? 0 : 1, out_n, out_t, out_a, out_x, out_z));
   return check_launch("conv_solve_kernel");
 }
 

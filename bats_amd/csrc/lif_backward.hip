@@ -319,13 +319,6 @@ static int reduce_slices(hipStream_t st, int slices, int64_t JI, double inv_b,
 constexpr int kRecWindow = 64;
 constexpr int kTileWaves = 4;
 
-__device__ __forceinline__ int xcd_logical_block(int p, int nblk) {
-  constexpr int kXcd = 8;
-  const int q = nblk / kXcd, r = nblk % kXcd;
-  const int xcd = p % kXcd, k = p / kXcd;
-  return xcd * q + min(xcd, r) + k;
-}
-
 // LDS ordering inside one wave: its DS instructions execute in order, so a fence at
 // wavefront scope (which also keeps the compiler from moving LDS accesses across it) is all
 // that a wave-private array needs.

@@ -150,8 +150,22 @@ __device__ __forceinline__ double tau_log(double tau, double z) { return tau * l
 // far inside kZBand; the event record is read only for a root inside the band. After an
 // accepted spike the clamp filter is applied to (z, t_hi] with the reset b, so that the
 // closed form runs again only when another root can be accepted.
-__device__ __forceinline__ SolveStateZ solve_crossings_deferred(
-    double a, double b, int n, double zq_lo, double zl_hi, const Event *ev, double c,
+// The interval's end times (t_lo, t_hi) are read from `ends` only for a root inside the band:
+// EventEnds reads them from the event record, TimeEnds holds them.
+struct EventEnds {
+  const Event *ev;
+  __device__ float lo() const { return ev->t; }
+  __device__ float hi() const { return ev->tl; }
+};
+struct TimeEnds {
+  float t_lo, t_hi;
+  __device__ float lo() const { return t_lo; }
+  __device__ float hi() const { return t_hi; }
+};
+
+template <class Ends>
+__device__ __forceinline__ SolveStateZ solve_crossings_deferred_ends(
+    double a, double b, int n, double zq_lo, double zl_hi, const Ends ends, double c,
     double inv2c, double dtt, double tau, int K, size_t row, float *out_t, float *out_a,
     float *out_x, float *out_z) {
   const double z_lo = zq_lo * (1.0 / (1.0 - kFilterEps));
@@ -178,10 +192,10 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
     if (valid & ((!lo_clear & !lo_rej) | (!hi_clear & !hi_rej))) {
       const double tz = tau_log(tau, z);
       if (!lo_clear & !lo_rej) {
-        const double last = last_is_event ? (double)ev->t : tau_log(tau, zlast);
+        const double last = last_is_event ? (double)ends.lo() : tau_log(tau, zlast);
         above = tz > last;
       }
-      if (!hi_clear & !hi_rej) below = !(tz > (double)ev->tl);
+      if (!hi_clear & !hi_rej) below = !(tz > (double)ends.hi());
     }
     if (!(valid & above & below)) break;
     const size_t slot = row + n;
@@ -202,6 +216,14 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
   r.b = b;
   r.n = n;
   return r;
+}
+
+__device__ __forceinline__ SolveStateZ solve_crossings_deferred(
+    double a, double b, int n, double zq_lo, double zl_hi, const Event *ev, double c,
+    double inv2c, double dtt, double tau, int K, size_t row, float *out_t, float *out_a,
+    float *out_x, float *out_z) {
+  return solve_crossings_deferred_ends(a, b, n, zq_lo, zl_hi, EventEnds{ev}, c, inv2c, dtt, tau,
+                                       K, row, out_t, out_a, out_x, out_z);
 }
 
 // Spike times of one lane's n stashed spikes: t = tau*log(z) and z rounded to float32, as

@@ -71,6 +71,32 @@ def test_conv_forward_backward(case):
                      "conv mean gradient")
 
 
+@pytest.mark.parametrize("case", ["wide_channels", "ragged_tiles", "zero_threshold"])
+def test_conv_forward_geometries(case):
+    """Solver tilings away from the network's shapes: 70 output channels (two channel groups
+    per position), a 9x5 output cut by the position tiles at both edges, and theta = 0
+    (c = 0: no threshold quadratic, every interval solved with the closed form)."""
+    rng = np.random.default_rng(5)
+    params = (0.05, 0.15, 0.1)
+    if case == "wide_channels":
+        pre_shape, fshape = (7, 6, 3), (3, 2, 70)
+    elif case == "ragged_tiles":
+        pre_shape, fshape = (11, 8, 2), (3, 4, 6)
+    else:
+        pre_shape, fshape = (10, 9, 2), (4, 3, 5)
+        params = (0.05, 0.0, 0.1)
+    W = rng.normal(0.5, 1.0, size=(fshape[2], fshape[0], fshape[1], pre_shape[2])).astype(F32)
+    t = _inputs(rng, 3, pre_shape, 3)
+    K, tmax = 5, 0.6
+    print(f"[conv geometry] {case}", flush=True)
+    ref = oconv.conv_forward(t, W, pre_shape, *params, K, tmax=tmax)
+    (n, tt, a, x, z, _, _), _ = _gpu_conv(t, W, pre_shape, params, K, tmax)
+    torch.cuda.synchronize()
+    assert_forward_match(n.cpu().numpy(), tt.cpu().numpy(), a.cpu().numpy(), x.cpu().numpy(),
+                         z.cpu().numpy(), ref)
+    assert ref.n.sum() > 10
+
+
 def test_pooling_forward_backward():
     from bats_amd import ops
     rng = np.random.default_rng(1)

@@ -31,7 +31,7 @@ SIGNATURES = {
     "bats_conv_lif_forward_workspace_size": (_sz, [_i, _vp, _i, _vp]),
     "bats_conv_lif_forward": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _f, _f, _f, _f, _f,
                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
-    "bats_conv_lif_backward_workspace_size": (_sz, [_i, _vp, _vp, _i]),
+    "bats_conv_lif_backward_workspace_size": (_sz, [_i, _vp, _i, _vp, _i]),
     "bats_conv_lif_backward": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp,
                                     _sz]),

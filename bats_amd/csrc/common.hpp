@@ -115,6 +115,15 @@ __device__ __forceinline__ int xcd_logical_block(int p, int nblk) {
   return xcd * q + min(xcd, r) + k;
 }
 
+// LDS ordering inside one wave: its DS instructions execute in order, so a fence at
+// wavefront scope (which also keeps the compiler from moving LDS accesses across it) is all
+// that a wave-private array needs.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Fill kernel (used instead of hipMemsetAsync so that every call stays graph-capturable).
 template <typename T>
 __global__ void fill_kernel(T *__restrict__ p, int64_t n, T v) {

@@ -6,11 +6,12 @@
 // per-event arithmetic (clamp filter, deferred closed form) on them, each lane using only the
 // events inside its own field (compute_spike_times_conv.cu:155-173).
 //
-// Backward: post_prep_kernel of the dense layer prepares the per-spike suffix sums; then
-// conv_wgrad_kernel (one block per filter tap and sample, lanes over output positions, block
-// reduction) and conv_pre_err_kernel (lanes over input neurons, loop over the output neurons
-// whose field contains them) -- the gradient regrouping of lif_backward.hip applied to the
-// receptive-field structure of compute_weights_gradient_conv.cu and
+// Backward: row_prep_kernel of the dense layer runs the reset chain and forms the per-spike
+// terms d1, d2; conv_rec_kernel lists every output position's spikes (all channels) in time
+// order; conv_wgrad_kernel (lane per filter tap, merge walk of the position's list against
+// the tap's pre spikes) and conv_pre_err_kernel (lane per pre spike, walk of the lists of the
+// output positions whose field holds it) apply the gradient regrouping of lif_backward.hip to
+// the receptive-field structure of compute_weights_gradient_conv.cu and
 // propagate_errors_to_pre_spikes_conv.cu.
 //
 // Pooling: pool_forward_kernel merges the four children's ascending lists per output neuron
@@ -319,133 +320,267 @@ __global__ void pool_backward_kernel(int64_t total, const float *__restrict__ er
 }
 
 // ---------------------------------------------------------------- backward
-__device__ __forceinline__ void lookup_row(const float *__restrict__ tr,
-                                           const double *__restrict__ s1,
-                                           const double *__restrict__ s2, int nj, float tp,
-                                           double &v1, double &v2) {
-  v1 = 0.0;
-  v2 = 0.0;
-  for (int k = nj - 1; k >= 0; --k) {
-    if (tr[k] > tp) {
-      v1 = s1[k];
-      v2 = s2[k];
+// ---------------------------------------------------------------- backward
+// Post-spike records, one per (sample, output position): slot 0 holds the count T and slots
+// 1..T the position's spikes over all its channels in ascending time (ties: channel, then
+// spike order) as {t, d1, d2, channel}, with d1, d2 the per-spike weight-gradient terms of
+// row_prep_kernel. One wave per position: the channels' spike times are staged in the wave's
+// LDS in (channel, spike) order, and an entry's rank is the number of staged entries before it
+// in (time, stage order).
+__host__ __device__ inline int conv_rec_slots(int Co, int K) {
+  return (1 + Co * K + 3) / 4 * 4;
+}
+__host__ __device__ inline int conv_rec_lds_per_wave(int Co, int K) {
+  return ((Co + 1 + Co * K) * 4 + 15) / 16 * 16;
+}
+
+__global__ __launch_bounds__(64 * kConvWaves) void conv_rec_kernel(
+    int units, int Co, int K, int waves_per_block, const int *__restrict__ n_eff,
+    const float *__restrict__ post_t, const float2 *__restrict__ D, float4 *__restrict__ rec) {
+  extern __shared__ __align__(16) char smem[];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int unit = blockIdx.x * waves_per_block + wave;
+  if (unit >= units) return;  // whole wave; no workgroup barrier follows
+  const int lane = threadIdx.x & 63;
+  int *off = reinterpret_cast<int *>(smem + (size_t)wave * conv_rec_lds_per_wave(Co, K));
+  float *ts = reinterpret_cast<float *>(off + Co + 1);
+  const size_t row0 = (size_t)unit * Co;  // rows of the position's channels
+  int T = 0;
+  for (int c0 = 0; c0 < Co; c0 += 64) {
+    const int cc = c0 + lane;
+    const int nj = cc < Co ? n_eff[row0 + cc] : 0;
+    int incl = nj;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
     }
+    const int ex = T + incl - nj;
+    if (cc < Co) off[cc] = ex;
+    for (int k = 0; k < nj; ++k) ts[ex + k] = post_t[(row0 + cc) * K + k];
+    T += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) off[Co] = T;
+  wave_lds_fence();
+  float4 *R = rec + (size_t)unit * conv_rec_slots(Co, K);
+  if (lane == 0) R[0] = make_float4(__int_as_float(T), 0.f, 0.f, 0.f);
+  for (int e0 = 0; e0 < T; e0 += 64) {
+    const int e = e0 + lane;
+    if (e >= T) break;
+    // the entry's channel: the last cc with off[cc] <= e (a channel with spikes)
+    int lo = 0, hi = Co - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int k = e - off[lo];
+    const float te = ts[e];
+    int rank = 0;
+    for (int f = 0; f < T; ++f) {
+      const float tf = ts[f];
+      rank += ((tf < te) | ((tf == te) & (f < e))) ? 1 : 0;
+    }
+    const float2 d = D[(row0 + lo) * K + k];
+    R[1 + rank] = make_float4(te, d.x, d.y, __int_as_float(lo));
   }
 }
 
-// One block per (filter tap, sample); lanes over output positions.
-__global__ void conv_wgrad_kernel(ConvGeom g, int K, int K_pre, const float *__restrict__ pre_t,
-                                  const float *__restrict__ pre_es,
-                                  const float *__restrict__ pre_e,
-                                  const float *__restrict__ post_t, const int *__restrict__ n_eff,
-                                  const double *__restrict__ S1, const double *__restrict__ S2,
-                                  double *__restrict__ per_sample) {
-  __shared__ double s_red[16];
+// Weight gradient: one workgroup per (sample, chunk of output positions, group of 64 filter
+// taps (dx, dy, ci), group of CG output channels); lane = tap, and the workgroup's waves take
+// the chunk's positions in turn. For output position p the lane's pre neuron is p + (dx, dy),
+// channel ci. The position's record is walked in ascending time while the lane's pointer into
+// the pre neuron's (ascending) spikes advances, keeping P1 = sum es_m and P2 = sum e_m over the
+// pre spikes before the post spike; a post spike of channel cc then adds d1 P1 - d2 P2 to
+// acc[cc][lane], the wave's float64 accumulators in LDS (a lane only touches its own
+// addresses, so the order of its additions is fixed). At the end the waves' accumulators are
+// summed in wave order into the workgroup's partial sums.
+struct ConvWgradTiling {
+  int tgroups, CG, cgroups, nchunks, ppc;  // ppc: positions per chunk
+  int blocks;
+};
+
+__global__ __launch_bounds__(64 * kConvWaves) void conv_wgrad_kernel(
+    ConvGeom g, int K_pre, const float *__restrict__ pre_t, const float *__restrict__ pre_es,
+    const float *__restrict__ pre_e, int rslots, const float4 *__restrict__ rec,
+    ConvWgradTiling wt, double *__restrict__ partial) {
+  extern __shared__ __align__(16) char smem[];
+  double *accs = reinterpret_cast<double *>(smem);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int blk = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int cg = blk % wt.cgroups;
+  const int tg = (blk / wt.cgroups) % wt.tgroups;
+  const int ch = (blk / (wt.cgroups * wt.tgroups)) % wt.nchunks;
+  const int b = blk / (wt.cgroups * wt.tgroups * wt.nchunks);
   const int taps = g.fx * g.fy * g.Ci;
-  const int tap = blockIdx.x;  // over Co * taps, layout [Co, fx, fy, Ci]
-  const int b = blockIdx.y;
-  const int cc = tap / taps;
-  const int r = tap - cc * taps;
-  const int dx = r / (g.fy * g.Ci);
-  const int r2 = r - dx * (g.fy * g.Ci);
+  const int tap = tg * 64 + lane;
+  const bool active = tap < taps;
+  const int tc = active ? tap : 0;
+  const int dx = tc / (g.fy * g.Ci);
+  const int r2 = tc - dx * (g.fy * g.Ci);
   const int dy = r2 / g.Ci;
   const int ci = r2 - dy * g.Ci;
-  const int Npost = g.Xo * g.Yo * g.Co;
+  const int c_lo = cg * wt.CG;
+  double *acc = accs + (size_t)wave * wt.CG * 64;
+  for (int u = 0; u < wt.CG; ++u) acc[u * 64 + lane] = 0.0;
+  wave_lds_fence();
+  const int npos = g.Xo * g.Yo;
   const int Npre = g.X * g.Y * g.Ci;
-  double acc = 0.0;
-  for (int p = threadIdx.x; p < g.Xo * g.Yo; p += blockDim.x) {
+  const int p_end = min(npos, (ch + 1) * wt.ppc);
+  for (int p = ch * wt.ppc + wave; p < p_end; p += kConvWaves) {
     const int x = p / g.Yo, y = p - (p / g.Yo) * g.Yo;
-    const int j = (x * g.Yo + y) * g.Co + cc;
-    const size_t prow = (size_t)b * Npost + j;
-    const int nj = n_eff[prow];
-    if (nj == 0) continue;
-    const int i = ((x + dx) * g.Y + (y + dy)) * g.Ci + ci;
-    const size_t pbase = ((size_t)b * Npre + i) * K_pre;
-    for (int m = 0; m < K_pre; ++m) {
-      const float tp = pre_t[pbase + m];
-      if (!isfinite(tp)) break;
-      double v1, v2;
-      lookup_row(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
-      acc = fma((double)pre_es[pbase + m], v1, acc);
-      acc = fma(-(double)pre_e[pbase + m], v2, acc);
-    }
-  }
-  // block reduction in a fixed order
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) s_red[wave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) s += s_red[w];
-    per_sample[(size_t)b * gridDim.x + tap] = s;
-  }
-}
-
-__global__ void conv_wgrad_finish_kernel(int B, int taps_all, int mode,
-                                         const double *__restrict__ per_sample,
-                                         float *__restrict__ grad) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mode == 1) {
-    if (e < B * taps_all) grad[e] = (float)per_sample[e];
-    return;
-  }
-  if (e >= taps_all) return;
-  double s = 0.0;
-  for (int b = 0; b < B; ++b) s += per_sample[(size_t)b * taps_all + e];
-  grad[e] = (float)(mode == 2 ? s : s / (double)B);
-}
-
-// Lanes over the input neurons of one sample; every output neuron whose field holds the
-// lane's neuron contributes W * (V1 es/tau_s - V2 e/tau) to each of its spikes.
-__global__ void conv_pre_err_kernel(ConvGeom g, int K, int K_pre, const float *__restrict__ pre_t,
-                                    const float *__restrict__ pre_es,
-                                    const float *__restrict__ pre_e,
-                                    const float *__restrict__ W, const float *__restrict__ post_t,
-                                    const int *__restrict__ n_eff, const double *__restrict__ S1,
-                                    const double *__restrict__ S2, double tau_s, double tau,
-                                    float *__restrict__ pre_err) {
-  const int Npre = g.X * g.Y * g.Ci;
-  const int Npost = g.Xo * g.Yo * g.Co;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (i >= Npre) return;
-  const int px = i / (g.Y * g.Ci), r = i - px * (g.Y * g.Ci), py = r / g.Ci, pc = r - py * g.Ci;
-  const size_t pbase = ((size_t)b * Npre + i) * K_pre;
-  for (int m = 0; m < K_pre; ++m) {
-    const float tp = pre_t[pbase + m];
-    if (!isfinite(tp)) {
-      pre_err[pbase + m] = 0.f;
-      continue;
-    }
-    const double es = (double)pre_es[pbase + m], e = (double)pre_e[pbase + m];
-    double acc = 0.0;
-    for (int dx = 0; dx < g.fx; ++dx) {
-      const int x = px - dx;
-      if (x < 0 || x >= g.Xo) continue;
-      for (int dy = 0; dy < g.fy; ++dy) {
-        const int y = py - dy;
-        if (y < 0 || y >= g.Yo) continue;
-        for (int cc = 0; cc < g.Co; ++cc) {
-          const int j = (x * g.Yo + y) * g.Co + cc;
-          const size_t prow = (size_t)b * Npost + j;
-          const int nj = n_eff[prow];
-          if (nj == 0) continue;
-          double v1, v2;
-          lookup_row(post_t + prow * K, S1 + prow * K, S2 + prow * K, nj, tp, v1, v2);
-          const double w = (double)W[(((size_t)cc * g.fx + dx) * g.fy + dy) * g.Ci + pc];
-          acc += w * ((v1 * es) / tau_s - (v2 * e) / tau);
-        }
+    const size_t pb = ((size_t)b * Npre + ((x + dx) * g.Y + (y + dy)) * g.Ci + ci) * K_pre;
+    const float4 *R = rec + ((size_t)b * npos + p) * rslots;
+    const int T = __float_as_int(R[0].x);
+    int m = 0;
+    double P1 = 0.0, P2 = 0.0;
+    float tn = (active && K_pre > 0) ? pre_t[pb] : kInfF;
+    for (int k = 0; k < T; ++k) {
+      const float4 E = R[1 + k];
+      while (tn < E.x) {
+        P1 += (double)pre_es[pb + m];
+        P2 += (double)pre_e[pb + m];
+        ++m;
+        tn = (m < K_pre) ? pre_t[pb + m] : kInfF;
+      }
+      const int cc = __float_as_int(E.w) - c_lo;
+      if ((unsigned)cc < (unsigned)wt.CG) {
+        const double q = fma((double)E.y, P1, -((double)E.z * P2));
+        __hip_atomic_fetch_add(acc + cc * 64 + lane, q, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
     }
-    pre_err[pbase + m] = (float)acc;
+  }
+  __syncthreads();
+  double *out = partial + (size_t)blk * wt.CG * 64;
+  for (int e = threadIdx.x; e < wt.CG * 64; e += blockDim.x) {
+    double v = 0.0;
+    for (int w = 0; w < kConvWaves; ++w) v += accs[(size_t)w * wt.CG * 64 + e];
+    out[e] = v;
+  }
+}
+
+// grad[cc, tap] (per sample for mode 1, else summed over the batch, divided by B for mode 0)
+// from the workgroups' partial sums, added in (sample, chunk) order.
+__global__ void conv_wgrad_reduce_kernel(int B, int Co, int taps, ConvWgradTiling wt, int mode,
+                                         const double *__restrict__ partial,
+                                         float *__restrict__ grad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)Co * taps;
+  if (e >= (mode == 1 ? (int64_t)B * per : per)) return;
+  const int bo = (int)(e / per);
+  const int r = (int)(e - (int64_t)bo * per);
+  const int cc = r / taps, tap = r - cc * taps;
+  const int cg = cc / wt.CG, u = cc - cg * wt.CG;
+  const int tg = tap / 64, l = tap - tg * 64;
+  const int b0 = mode == 1 ? bo : 0, b1 = mode == 1 ? bo + 1 : B;
+  double s = 0.0;
+  for (int b = b0; b < b1; ++b)
+    for (int ch = 0; ch < wt.nchunks; ++ch) {
+      const int64_t blk = (((int64_t)b * wt.nchunks + ch) * wt.tgroups + tg) * wt.cgroups + cg;
+      s += partial[(blk * wt.CG + u) * 64 + l];
+    }
+  grad[e] = (float)(mode == 0 ? s / (double)B : s);
+}
+
+// First compacted pre spike of every input position of a sample (compacted spikes are in
+// neuron order, and the neurons of one position are consecutive): pos_off[b][pp] for
+// pp = 0..npos, with pos_off[b][npos] = the sample's spike count.
+__global__ void conv_spk_bounds_kernel(int npos, int Ci, int cap, const float4 *__restrict__ spk,
+                                       const int *__restrict__ count, int *__restrict__ pos_off) {
+  const int b = blockIdx.x;
+  const int cnt = count[b];
+  const float4 *sp = spk + (size_t)b * cap;
+  int *po = pos_off + (size_t)b * (npos + 1);
+  if (cnt == 0) {
+    for (int pp = threadIdx.x; pp <= npos; pp += blockDim.x) po[pp] = 0;
+    return;
+  }
+  for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
+    const int q = __float_as_int(sp[s].w) / Ci;
+    const int qp = s > 0 ? __float_as_int(sp[s - 1].w) / Ci : -1;
+    for (int pp = qp + 1; pp <= q; ++pp) po[pp] = s;
+    if (s == cnt - 1)
+      for (int pp = q + 1; pp <= npos; ++pp) po[pp] = cnt;
+  }
+}
+
+// Pre-spike errors: one wave per (sample, input position); lanes take the position's compacted
+// pre spikes (all input channels), 64 at a time. For every output position whose field holds
+// the input position (at offset (dx, dy)), the position's record is walked from its latest
+// spike down to the earliest pre spike of the chunk, and every post spike (channel cc) later
+// than the lane's pre spike adds W[cc, dx, dy, ci] (d1 es/tau_s - d2 e/tau): summed in
+// float32 per output position and in float64 over the positions.
+constexpr int kPreGroup = 4;
+
+__global__ __launch_bounds__(64 * kConvWaves) void conv_pre_err_kernel(
+    ConvGeom g, int B, int K_pre, int cap, const float4 *__restrict__ spk,
+    const int *__restrict__ slot, const int *__restrict__ pos_off, int rslots,
+    const float4 *__restrict__ rec, const float *__restrict__ W, float inv_tau_s, float inv_tau,
+    float *__restrict__ pre_errors) {
+  const int npos_in = g.X * g.Y;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int unit = xcd_logical_block(blockIdx.x, gridDim.x) * kConvWaves + wave;
+  if (unit >= B * npos_in) return;  // whole wave; no workgroup barrier follows
+  const int lane = threadIdx.x & 63;
+  const int b = unit / npos_in, pp = unit - b * npos_in;
+  const int px = pp / g.Y, py = pp - (pp / g.Y) * g.Y;
+  const int s0 = pos_off[(size_t)b * (npos_in + 1) + pp];
+  const int s1 = pos_off[(size_t)b * (npos_in + 1) + pp + 1];
+  const int x_lo = max(0, px - g.fx + 1), x_hi = min(g.Xo - 1, px);
+  const int y_lo = max(0, py - g.fy + 1), y_hi = min(g.Yo - 1, py);
+  const int taps = g.fx * g.fy * g.Ci;
+  const int npos_out = g.Xo * g.Yo;
+  const float4 *sp = spk + (size_t)b * cap;
+  for (int c0 = s0; c0 < s1; c0 += 64) {
+    const int s = c0 + lane;
+    const bool act = s < s1;
+    const float4 P = act ? sp[s] : make_float4(kInfF, 0.f, 0.f, 0.f);
+    const float tp = P.x, esd = P.y * inv_tau_s, ed = P.z * inv_tau;
+    const int ci = act ? __float_as_int(P.w) - pp * g.Ci : 0;
+    float tmin = tp;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, d, 64));
+    tmin = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tmin)));
+    double pe = 0.0;
+    for (int x = x_lo; x <= x_hi; ++x) {
+      for (int y = y_lo; y <= y_hi; ++y) {
+        const int wb = ((px - x) * g.fy + (py - y)) * g.Ci + ci;
+        const float4 *R = rec + ((size_t)b * npos_out + x * g.Yo + y) * rslots;
+        const int T = __float_as_int(R[0].x);
+        float pt = 0.f;
+        for (int k = T; k > 0; k -= kPreGroup) {
+          float4 E[kPreGroup];
+          float w[kPreGroup];
+#pragma unroll
+          for (int u = 0; u < kPreGroup; ++u)
+            E[u] = (k - u > 0) ? R[k - u] : make_float4(-kInfF, 0.f, 0.f, __int_as_float(0));
+#pragma unroll
+          for (int u = 0; u < kPreGroup; ++u) w[u] = W[(size_t)__float_as_int(E[u].w) * taps + wb];
+#pragma unroll
+          for (int u = 0; u < kPreGroup; ++u) {
+            const float q = fmaf(esd, E[u].y, -(ed * E[u].z));
+            pt = fmaf((tp < E[u].x) ? w[u] : 0.f, q, pt);
+          }
+          if (!(E[kPreGroup - 1].x > tmin)) break;
+        }
+        pe += (double)pt;
+      }
+    }
+    if (act) pre_errors[(size_t)b * cap + slot[(size_t)b * cap + s]] = (float)pe;
   }
 }
 
 // defined in lif_backward.hip
 int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const float *a,
               const float *x, const float *z, float *errors, float c, float tau, float dtt,
-              int *n_eff, double *S1, double *S2);
+              int *n_eff, float2 *D);
+int compact_pre_spikes(hipStream_t st, int B, int I, int K_pre, const float *pre_t,
+                       const float *pre_es, const float *pre_e, float4 *spk, int *slot,
+                       int *count);
 
 // Wave tiling of the conv solver. A wave reads the whole event list of its sample once (cost
 // ~ 1/64 per event) and processes the events inside its joint field (cost ~ 1 per event), so
@@ -562,17 +697,60 @@ extern "C" int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, 
   return check_launch("conv_solve_kernel");
 }
 
-extern "C" size_t bats_conv_lif_backward_workspace_size(int B, const int *pre_shape,
+// Workspace of the conv backward (byte offsets) and the weight-gradient tiling.
+struct ConvBwdLayout {
+  size_t n_eff, D, rec, partial, spk, slot, count, pos_off, total;
+  int rslots, cap;
+  ConvWgradTiling wt;
+};
+
+static ConvBwdLayout conv_bwd_layout(int B, const ConvGeom &g, int K_pre, int K_post) {
+  ConvBwdLayout L{};
+  const size_t BJ = (size_t)B * g.Xo * g.Yo * g.Co;
+  const int npos = g.Xo * g.Yo;
+  const int taps = g.fx * g.fy * g.Ci;
+  ConvWgradTiling &wt = L.wt;
+  wt.tgroups = (taps + 63) / 64;
+  wt.cgroups = (g.Co + 63) / 64;
+  wt.CG = (g.Co + wt.cgroups - 1) / wt.cgroups;
+  // about 1536 workgroups (6144 waves), each wave with a few positions
+  const long long per_chunk = (long long)std::max(B, 1) * wt.tgroups * wt.cgroups;
+  int nchunks = (int)std::min<long long>((1536 + per_chunk - 1) / per_chunk,
+                                         (npos + kConvWaves - 1) / kConvWaves);
+  nchunks = std::max(nchunks, 1);
+  wt.ppc = (npos + nchunks - 1) / nchunks;
+  wt.nchunks = (npos + wt.ppc - 1) / wt.ppc;
+  wt.blocks = (int)(per_chunk * wt.nchunks);
+  L.rslots = conv_rec_slots(g.Co, K_post);
+  L.cap = g.X * g.Y * g.Ci * K_pre;
+  size_t off = 0;
+  L.n_eff = off;
+  off = align_up(off + sizeof(int) * BJ, 256);
+  L.D = off;
+  off = align_up(off + sizeof(float2) * BJ * K_post, 256);
+  L.rec = off;
+  off = align_up(off + sizeof(float4) * (size_t)B * npos * L.rslots, 256);
+  L.partial = off;
+  off = align_up(off + sizeof(double) * (size_t)wt.blocks * wt.CG * 64, 256);
+  L.spk = off;
+  off = align_up(off + sizeof(float4) * (size_t)B * L.cap, 256);
+  L.slot = off;
+  off = align_up(off + sizeof(int) * (size_t)B * L.cap, 256);
+  L.count = off;
+  off = align_up(off + sizeof(int) * (size_t)B, 256);
+  L.pos_off = off;
+  off = align_up(off + sizeof(int) * (size_t)B * (g.X * g.Y + 1), 256);
+  L.total = off;
+  return L;
+}
+
+extern "C" size_t bats_conv_lif_backward_workspace_size(int B, const int *pre_shape, int K_pre,
                                                         const int *filters_shape, int K_post) {
   ConvGeom g;
-  if (B < 0 || !pre_shape || !filters_shape || !conv_geom(pre_shape, filters_shape, g)) return 0;
-  const size_t BJ = (size_t)B * g.Xo * g.Yo * g.Co;
-  size_t off = 0;
-  off = align_up(off + sizeof(int) * BJ, 256);
-  off = align_up(off + sizeof(double) * BJ * K_post, 256);
-  off = align_up(off + sizeof(double) * BJ * K_post, 256);
-  off = align_up(off + sizeof(double) * (size_t)B * g.Co * g.fx * g.fy * g.Ci, 256);
-  return off;
+  if (B < 0 || K_pre < 0 || K_post < 1 || !pre_shape || !filters_shape ||
+      !conv_geom(pre_shape, filters_shape, g))
+    return 0;
+  return conv_bwd_layout(B, g, K_pre, K_post).total;
 }
 
 extern "C" int bats_conv_lif_backward(void *stream, int B, const int *pre_shape, int K_pre,
@@ -590,44 +768,76 @@ extern "C" int bats_conv_lif_backward(void *stream, int B, const int *pre_shape,
   BATS_REQUIRE(B >= 0 && K_pre >= 0 && K_post >= 1, "bad sizes");
   BATS_REQUIRE(grad_mode >= 0 && grad_mode <= 2, "grad_mode must be 0, 1 or 2");
   BATS_REQUIRE(B <= 65535, "batch above 65535");
+  BATS_REQUIRE(conv_rec_lds_per_wave(g.Co, K_post) <= 64 * 1024,
+               "C_out * K_post above 16000 (post-spike record stage)");
+  BATS_REQUIRE((long long)g.X * g.Y * g.Ci * K_pre < (1LL << 31), "pre spike buffer too large");
   if (B == 0) return 0;
-  const size_t need = bats_conv_lif_backward_workspace_size(B, pre_shape, filters_shape, K_post);
-  BATS_REQUIRE(workspace && workspace_bytes >= need, "workspace too small");
+  const ConvBwdLayout L = conv_bwd_layout(B, g, K_pre, K_post);
+  BATS_REQUIRE(workspace && workspace_bytes >= L.total, "workspace too small");
   BATS_REQUIRE(n && t && a && x && z && errors && grad, "null pointer");
-  hipStream_t st = as_stream(stream);
-  const size_t BJ = (size_t)B * g.Xo * g.Yo * g.Co;
-  char *ws = static_cast<char *>(workspace);
-  size_t off = 0;
-  int *n_eff = reinterpret_cast<int *>(ws + off);
-  off = align_up(off + sizeof(int) * BJ, 256);
-  double *S1 = reinterpret_cast<double *>(ws + off);
-  off = align_up(off + sizeof(double) * BJ * K_post, 256);
-  double *S2 = reinterpret_cast<double *>(ws + off);
-  off = align_up(off + sizeof(double) * BJ * K_post, 256);
-  double *per_sample = reinterpret_cast<double *>(ws + off);
-  if (int r = post_prep(st, (int)BJ, K_post, n, t, a, x, z, errors, c, tau, delta_theta_tau,
-                        n_eff, S1, S2))
-    return r;
-  const int taps_all = g.Co * g.fx * g.fy * g.Ci;
   if (K_pre > 0) BATS_REQUIRE(pre_times && pre_exp_tau_s && pre_exp_tau, "null pre pointer");
-  const int threads = std::min(1024, std::max(64, (g.Xo * g.Yo + 63) / 64 * 64));
+  hipStream_t st = as_stream(stream);
+  char *ws = static_cast<char *>(workspace);
+  int *n_eff = reinterpret_cast<int *>(ws + L.n_eff);
+  float2 *D = reinterpret_cast<float2 *>(ws + L.D);
+  float4 *rec = reinterpret_cast<float4 *>(ws + L.rec);
+  double *partial = reinterpret_cast<double *>(ws + L.partial);
+  const int npos = g.Xo * g.Yo;
+  const size_t BJ = (size_t)B * npos * g.Co;
+  if (int r = post_prep(st, (int)BJ, K_post, n, t, a, x, z, errors, c, tau, delta_theta_tau,
+                        n_eff, D))
+    return r;
+  {
+    const int units = B * npos;
+    const int lw = conv_rec_lds_per_wave(g.Co, K_post);
+    const int wpb = std::max(1, std::min(kConvWaves, 64 * 1024 / lw));
+    BATS_TIMED("conv_rec_kernel", st,
+               hipLaunchKernelGGL(conv_rec_kernel, dim3((units + wpb - 1) / wpb),
+                                  dim3(64 * wpb), (size_t)lw * wpb, st, units, g.Co, K_post, wpb,
+                                  n_eff, t, D, rec));
+    if (int r = check_launch("conv_rec_kernel")) return r;
+  }
+  const int taps = g.fx * g.fy * g.Ci;
+  const ConvWgradTiling &wt = L.wt;
+  const size_t wlds = sizeof(double) * kConvWaves * wt.CG * 64;
+  if (wlds > 64 * 1024)
+    BATS_REQUIRE(hipFuncSetAttribute((const void *)conv_wgrad_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)wlds) == hipSuccess,
+                 "conv_wgrad_kernel: LDS request above the device limit");
   BATS_TIMED("conv_wgrad_kernel", st,
-             hipLaunchKernelGGL(conv_wgrad_kernel, dim3(taps_all, B), dim3(threads), 0, st, g,
-                                K_post, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau, t, n_eff,
-                                S1, S2, per_sample));
+             hipLaunchKernelGGL(conv_wgrad_kernel, dim3(wt.blocks), dim3(64 * kConvWaves), wlds,
+                                st, g, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau, L.rslots,
+                                rec, wt, partial));
   if (int r = check_launch("conv_wgrad_kernel")) return r;
-  const int fin = grad_mode == 1 ? B * taps_all : taps_all;
-  hipLaunchKernelGGL(conv_wgrad_finish_kernel, dim3((fin + 255) / 256), dim3(256), 0, st, B,
-                     taps_all, grad_mode, per_sample, grad);
-  if (int r = check_launch("conv_wgrad_finish_kernel")) return r;
+  const int64_t fin = (int64_t)(grad_mode == 1 ? B : 1) * g.Co * taps;
+  hipLaunchKernelGGL(conv_wgrad_reduce_kernel, dim3((unsigned)((fin + 255) / 256)), dim3(256), 0,
+                     st, B, g.Co, taps, wt, grad_mode, partial, grad);
+  if (int r = check_launch("conv_wgrad_reduce_kernel")) return r;
   if (pre_errors && K_pre > 0) {
     BATS_REQUIRE(weights, "null weights");
-    const int Npre = g.X * g.Y * g.Ci;
+    float4 *spk = reinterpret_cast<float4 *>(ws + L.spk);
+    int *slot = reinterpret_cast<int *>(ws + L.slot);
+    int *count = reinterpret_cast<int *>(ws + L.count);
+    int *pos_off = reinterpret_cast<int *>(ws + L.pos_off);
+    const int npos_in = g.X * g.Y;
+    const int64_t nerr = (int64_t)B * L.cap;
+    hipLaunchKernelGGL(fill_kernel<float>, dim3((unsigned)std::min<int64_t>((nerr + 255) / 256, 65535)),
+                       dim3(256), 0, st, pre_errors, nerr, 0.f);
+    if (int r = check_launch("fill_kernel")) return r;
+    if (int r = compact_pre_spikes(st, B, g.X * g.Y * g.Ci, K_pre, pre_times, pre_exp_tau_s,
+                                   pre_exp_tau, spk, slot, count))
+      return r;
+    hipLaunchKernelGGL(conv_spk_bounds_kernel, dim3(B), dim3(256), 0, st, npos_in, g.Ci, L.cap,
+                       spk, count, pos_off);
+    if (int r = check_launch("conv_spk_bounds_kernel")) return r;
+    const int64_t units = (int64_t)B * npos_in;
     BATS_TIMED("conv_pre_err_kernel", st,
-               hipLaunchKernelGGL(conv_pre_err_kernel, dim3((Npre + 255) / 256, B), dim3(256), 0,
-                                  st, g, K_post, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau,
-                                  weights, t, n_eff, S1, S2, (double)tau_s, (double)tau,
-                                  pre_errors));
+               hipLaunchKernelGGL(conv_pre_err_kernel,
+                                  dim3((unsigned)((units + kConvWaves - 1) / kConvWaves)),
+                                  dim3(64 * kConvWaves), 0, st, g, B, K_pre, L.cap, spk, slot,
+                                  pos_off, L.rslots, rec, weights, (float)(1.0 / (double)tau_s),
+                                  (float)(1.0 / (double)tau), pre_errors));
     if (int r = check_launch("conv_pre_err_kernel")) return r;
   }
   return 0;

@@ -33,7 +33,7 @@ constexpr int kBwdThreads = 256;
 // factors f1, f2 (compute_factors.cu:20-30, rounded to float32 as stored there) and the
 // per-spike weight-gradient terms d1 = e f1, d2 = e f2. Outputs (each optional):
 //   n_eff [B*J]      spikes before the row's first infinite time
-//   S1, S2 [B*J*K]   float64 suffix sums of d1, d2 (the conv backward reads them)
+//   D [B*J*K]        the per-spike terms {d1, d2} in float32 (read by the conv backward)
 //   rec              one record of tile_rec_slots(K) float4 per (sample, tile of kWT rows):
 //                      slot 0     x: number of spikes in the tile (int)
 //                      slots 1..  the spikes of row 0, row 1, ... as {t, d1, d2, r} with r
@@ -55,8 +55,7 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
     int B, int J, int K, const int *__restrict__ n, const float *__restrict__ t,
     const float *__restrict__ a, const float *__restrict__ x, const float *__restrict__ z,
     float *__restrict__ errors, double c, double tau, double dtt, int apply_reset,
-    int *__restrict__ n_eff, double *__restrict__ S1, double *__restrict__ S2,
-    float4 *__restrict__ rec) {
+    int *__restrict__ n_eff, float2 *__restrict__ D, float4 *__restrict__ rec) {
   const int gj = (J + kWT - 1) / kWT;
   const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t tiles = (int64_t)B * gj;  // lane < tiles * kWT for every launched lane group
@@ -154,7 +153,6 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
   if (!active) return;
   float4 *ent = tr ? tr + kRecHead + excl : nullptr;
   const float roff = __int_as_float(u * 64 * (int)sizeof(double));
-  double s1 = 0.0, s2 = 0.0;
   if (fast) {
 #pragma unroll
     for (int k = kPrepFast - 1; k >= 0; --k) {
@@ -163,12 +161,7 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
         const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
         const float f2 = (float)(tau / xk);
         const double d1 = (double)ev[k] * (double)f1, d2 = (double)ev[k] * (double)f2;
-        s1 += d1;
-        s2 += d2;
-        if (S1) {
-          S1[row + k] = s1;
-          S2[row + k] = s2;
-        }
+        if (D) D[row + k] = make_float2((float)d1, (float)d2);
         if (ent) ent[k] = make_float4(tv[k], (float)d1, (float)d2, roff);
       }
     }
@@ -192,12 +185,7 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
         const float f1 = (float)(tau / ak * (1.0 + zk * c / xk));
         const float f2 = (float)(tau / xk);
         const double d1 = (double)ev[k] * (double)f1, d2 = (double)ev[k] * (double)f2;
-        s1 += d1;
-        s2 += d2;
-        if (S1) {
-          S1[row + c0 + k] = s1;
-          S2[row + c0 + k] = s2;
-        }
+        if (D) D[row + c0 + k] = make_float2((float)d1, (float)d2);
         if (ent) ent[c0 + k] = make_float4(tv[k], (float)d1, (float)d2, roff);
       }
     }
@@ -206,15 +194,15 @@ __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
 
 static int launch_row_prep(hipStream_t st, int B, int J, int K, const int *n, const float *t,
                            const float *a, const float *x, const float *z, float *errors,
-                           double c, double tau, double dtt, int *n_eff, double *S1,
-                           double *S2, float4 *rec) {
+                           double c, double tau, double dtt, int *n_eff, float2 *D,
+                           float4 *rec) {
   const int64_t lanes = (int64_t)B * ((J + kWT - 1) / kWT) * kWT;
   if (lanes <= 0) return 0;
   // small launches (an output layer's rows) in one-wave blocks, spread over many CUs
   const int threads = lanes < 256 * 256 ? 64 : kBwdThreads;
   hipLaunchKernelGGL(row_prep_kernel, dim3((unsigned)((lanes + threads - 1) / threads)),
                      dim3(threads), 0, st, B, J, K, n, t, a, x, z, errors, c, tau, dtt, 1,
-                     n_eff, S1, S2, rec);
+                     n_eff, D, rec);
   return check_launch("row_prep_kernel");
 }
 
@@ -319,14 +307,6 @@ static int reduce_slices(hipStream_t st, int slices, int64_t JI, double inv_b,
 constexpr int kRecWindow = 64;
 constexpr int kTileWaves = 4;
 
-// LDS ordering inside one wave: its DS instructions execute in order, so a fence at
-// wavefront scope (which also keeps the compiler from moving LDS accesses across it) is all
-// that a wave-private array needs.
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <bool kSingle>
 __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
@@ -682,7 +662,7 @@ __global__ __launch_bounds__(64 * kTileWaves) void pre_err_spk_kernel(
 // ------------------------------------------------------------------ narrow layers
 // Layers with at most kST post neurons (an output layer): one block per (pre-neuron block,
 // sample). The sample's raw post rows are copied to LDS, one thread per row runs the reset
-// chain, the factors and the suffix sums (the arithmetic of post_prep_kernel), and every
+// chain, the factors and the suffix sums (the arithmetic of row_prep_kernel), and every
 // lane then forms, for each of its pre spikes, the weight-gradient terms of all post neurons
 // and -- from the same lookups -- the pre-spike error sum_j W (V1 es/tau_s - V2 e/tau).
 // The per-sample weight-gradient rows are reduced over the batch by wgrad_reduce_kernel.
@@ -921,11 +901,21 @@ static int allow_lds(const void *kernel, size_t bytes) {
   return 0;
 }
 
+// Compacted pre spikes of every sample in neuron order (compact_spikes_kernel), for the conv
+// backward.
+int compact_pre_spikes(hipStream_t st, int B, int I, int K_pre, const float *pre_t,
+                       const float *pre_es, const float *pre_e, float4 *spk, int *slot,
+                       int *count) {
+  hipLaunchKernelGGL(compact_spikes_kernel, dim3(B), dim3(kCompThreads), 0, st, I, K_pre, pre_t,
+                     pre_es, pre_e, I * K_pre, spk, slot, count);
+  return check_launch("compact_spikes_kernel");
+}
+
 int post_prep(hipStream_t st, int BJ, int K, const int *n, const float *t, const float *a,
               const float *x, const float *z, float *errors, float c, float tau, float dtt,
-              int *n_eff, double *S1, double *S2) {
+              int *n_eff, float2 *D) {
   return launch_row_prep(st, 1, BJ, K, n, t, a, x, z, errors, (double)c, (double)tau,
-                         (double)dtt, n_eff, S1, S2, nullptr);
+                         (double)dtt, n_eff, D, nullptr);
 }
 
 }  // namespace bats
@@ -978,7 +968,7 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   // the reset chain alone, written back in place
   auto reset_only = [&]() -> int {
     return launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c, (double)tau,
-                           (double)delta_theta_tau, nullptr, nullptr, nullptr, nullptr);
+                           (double)delta_theta_tau, nullptr, nullptr, nullptr);
   };
   if (JI == 0) return reset_only();  // no weights: only the reset chain of the errors
   const bool want_pre = pre_errors && K_pre > 0;
@@ -1009,7 +999,7 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   BATS_TIMED("row_prep_kernel", st,
              prep_rc = launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c,
                                        (double)tau, (double)delta_theta_tau, nullptr,
-                                       nullptr, nullptr, rec));
+                                       nullptr, rec));
   if (prep_rc) return prep_rc;
   const bool multi = K_pre > 1;
   const int cap = I * K_pre;

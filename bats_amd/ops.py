@@ -317,7 +317,7 @@ def conv_lif_backward(n, t, a, x, z, errors, pre_times, pre_shape, pre_es, pre_e
     mode = 1 if per_sample else (2 if batch_sum else 0)
     lib = _lib.load()
     ps, fs = _shape3(pre_shape), _shape3((fx, fy, Co))
-    ws_bytes = lib.bats_conv_lif_backward_workspace_size(B, ps, fs, K)
+    ws_bytes = lib.bats_conv_lif_backward_workspace_size(B, ps, Kp, fs, K)
     ws = _workspace(ws_bytes, dev)
     _lib.check(lib.bats_conv_lif_backward(
         _stream(), B, ps, Kp, fs, K, _ptr(n, I32), _ptr(t, F32), _ptr(a, F32), _ptr(x, F32),

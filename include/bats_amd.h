@@ -120,7 +120,7 @@ int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, int K_pre,
                           float *pre_exp_tau_s, float *pre_exp_tau, void *workspace,
                           size_t workspace_bytes);
 
-size_t bats_conv_lif_backward_workspace_size(int B, const int *pre_shape,
+size_t bats_conv_lif_backward_workspace_size(int B, const int *pre_shape, int K_pre,
                                              const int *filters_shape, int K_post);
 
 /*

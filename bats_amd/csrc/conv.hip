@@ -52,7 +52,8 @@ __global__ void conv_filter_transpose_kernel(const float *__restrict__ W, float 
 // checks after every event of the sample (compute_spike_times_conv.cu:155-173). The check is
 // the dense solver's: the clamp filter, then the closed form in z = exp(t/tau) space with
 // the spike times formed after the walk.
-constexpr int kConvWaves = 4;
+constexpr int kConvWaves = 4;       // waves per workgroup of the backward kernels
+constexpr int kConvSolveWaves = 8;  // waves per workgroup of the solver (they share its LDS)
 constexpr int kConvBatch = 8;
 
 struct ConvTiling {
@@ -71,6 +72,14 @@ struct ConvEnds {
   __device__ float hi() const { return hi_ev ? fminf(hi_ev->t, tmax) : tmax; }
 };
 
+// A lane value clamped to [lo, hi], written as the two instructions (fmax/fmin would first
+// quiet both operands, two more float64 operations; the bounds are never NaN).
+__device__ __forceinline__ double clamp_lane2(double x, double lo, double hi) {
+  double r;
+  asm("v_max_f64 %0, %1, %2\n\tv_min_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
 // n / d for 0 <= n < 2^24 and d >= 1 with inv = 1.0f / d: the float32 quotient is within one
 // of the exact one (exact for d = 1), and one correction step fixes it.
 __device__ __forceinline__ int div_small(int n, int d, float inv) {
@@ -83,16 +92,88 @@ __device__ __forceinline__ int div_small(int n, int d, float inv) {
 
 // kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every own interval is solved
 // with the reference's loop in t space.
-template <bool kAlwaysExact>
-__global__ __launch_bounds__(64 * kConvWaves) void conv_solve_kernel(
+// The decomposition of one pre-neuron index of the chunk: in the tile's joint field or not,
+// the uniform part ebase of its weight index and its packed offset (dx0 << 16 | dy0) from
+// the tile origin.
+struct ConvChunkEv {
+  int ebase, dpk;
+  bool inu;
+};
+
+__device__ __forceinline__ ConvChunkEv conv_decompose(int qi, const ConvGeom &g,
+                                                      const ConvTiling &tl, int x0, int y0,
+                                                      int ux, int uy) {
+  ConvChunkEv r{0, 0, false};
+  if (qi >= 0) {
+    const int rr = div_small(qi, g.Ci, tl.inv_ci);
+    const int pc = qi - rr * g.Ci;
+    const int px = div_small(rr, g.Y, tl.inv_y);
+    const int dx0 = px - x0, dy0 = rr - px * g.Y - y0;
+    r.inu = ((unsigned)dx0 < (unsigned)ux) & ((unsigned)dy0 < (unsigned)uy);
+    r.ebase = ((dx0 * g.fy + dy0) * g.Ci + pc) * g.Co;
+    r.dpk = (dx0 << 16) | dy0;
+  }
+  return r;
+}
+
+// The event data of an in-field event, as staged for the wave: es, e, the lower end zq_lo of
+// the interval that the event opens and the upper end zl_hi of the interval that it closes.
+struct alignas(16) ConvStageEv {
+  double es, e, zq, zh;
+};
+
+__device__ __forceinline__ ConvStageEv conv_lane_event(bool inu, const FastEvent *fv, int q,
+                                                       double zmax_hi) {
+  constexpr double kLoToHi = (1.0 + kFilterEps) / (1.0 - kFilterEps);
+  ConvStageEv r{0.0, 0.0, 0.0, 0.0};
+  if (inu) {
+    const FastEvent f = fv[q];
+    r.es = f.es;
+    r.e = f.e;
+    r.zq = f.zq_lo;
+    r.zh = fmin(f.zq_lo * kLoToHi, zmax_hi);
+  }
+  return r;
+}
+
+// Per-wave LDS stage of one 64-event chunk.
+struct ConvWaveStage {
+  ConvStageEv ev[64];
+  int eb[64];
+  int dp[64];
+};
+
+__host__ inline size_t conv_solve_lds_bytes(size_t wbytes, bool lds_w) {
+  return (lds_w ? (wbytes + 15) / 16 * 16 + 16 : 0) + sizeof(ConvWaveStage) * kConvSolveWaves;
+}
+
+// kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every own interval is solved
+// with the reference's loop in t space. kLdsW: the transposed filter bank is staged in the
+// workgroup's LDS, followed by one zero (the weight of an event outside a lane's field).
+// kSingle: one output position per wave, so that every in-field event is in the field of
+// every lane (no per-lane field test).
+template <bool kAlwaysExact, bool kLdsW, bool kSingle>
+__global__ __launch_bounds__(64 * kConvSolveWaves) void conv_solve_kernel(
     const Event *__restrict__ events, const FastEvent *__restrict__ fast,
     const int *__restrict__ evidx, const int *__restrict__ counts, int Pcap, ConvGeom g,
-    ConvTiling tl, int units, const float *__restrict__ WT, int K, double c, double dtt,
+    ConvTiling tl, int units, const float *__restrict__ WT_g, int K, double c, double dtt,
     double tau, float tmax, int *__restrict__ out_n, float *__restrict__ out_t,
     float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
+  extern __shared__ __align__(16) char smem[];
+  const int nw = g.fx * g.fy * g.Ci * g.Co;
+  const float *WT = WT_g;
+  ConvWaveStage *stages = reinterpret_cast<ConvWaveStage *>(smem);
+  if (kLdsW) {
+    float *sW = reinterpret_cast<float *>(smem);
+    for (int e = threadIdx.x; e <= nw; e += blockDim.x) sW[e] = e < nw ? WT_g[e] : 0.f;
+    __syncthreads();
+    WT = sW;
+    stages = reinterpret_cast<ConvWaveStage *>(smem + ((size_t)nw * 4 + 15) / 16 * 16 + 16);
+  }
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int unit = xcd_logical_block(blockIdx.x, gridDim.x) * kConvWaves + wave;
+  const int unit = xcd_logical_block(blockIdx.x, gridDim.x) * kConvSolveWaves + wave;
   if (unit >= units) return;  // whole wave; no workgroup barrier follows
+  ConvWaveStage &stg = stages[wave];
   const int lane = threadIdx.x & 63;
   const int tiles = tl.tilesX * tl.tilesY;
   const int cg = unit % tl.cgroups;
@@ -107,7 +188,8 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv_solve_kernel(
   const int j = active ? ((x0 + ox) * g.Yo + (y0 + oy)) * g.Co + cc : 0;
   const size_t row = ((size_t)b * g.Xo * g.Yo * g.Co + j) * K;
   // weight of an own event: WT[ebase - loff], ebase = ((dx0 fy + dy0) Ci + pc) Co with
-  // (dx0, dy0) the event's offset from the tile origin
+  // (dx0, dy0) the event's offset from the tile origin; an inactive lane reads weight 0
+  // (the padding zero in LDS, or WT[0] times a zero flag from global memory)
   const int loff = (ox * g.fy + oy) * g.Ci * g.Co - cc;
   const int ux = tl.PX + g.fx - 1, uy = tl.PY + g.fy - 1;  // joint field of the tile
   const int P = counts[b];
@@ -116,87 +198,102 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv_solve_kernel(
   const int *ix = evidx + (size_t)b * Pcap;
   const double inv2c = 0.5 / c;
   const double zmax_hi = fmin(exp((double)tmax / tau), 1e300) * (1.0 + kFilterEps);
-  constexpr double kLoToHi = (1.0 + kFilterEps) / (1.0 - kFilterEps);
   // an inactive lane, or one whose buffer is full, carries a = -inf (it fails a > 0); the
   // always-exact form uses `live`
   double a = (kAlwaysExact || active) ? 0.0 : -kInfD, bb = 0.0;
   double zq_last = 0.0;  // zq_lo of the lane's previous own event
   int last_q = 0, n = 0;
   bool live = active, seen = false;
-  int qn = (lane < P) ? ix[lane] : -1;
+  // weight of event (ebase eb, offset dp) for this lane, 0 outside its field
+  auto weight = [&](int eb, int dp) -> float {
+    bool own = active;
+    if (!kSingle)
+      own = own & ((unsigned)((dp >> 16) - ox) < (unsigned)g.fx) &
+            ((unsigned)((dp & 0xffff) - oy) < (unsigned)g.fy);
+    if (kLdsW) return WT[own ? eb - loff : nw];
+    const float w = WT[own ? eb - loff : 0];
+    return own ? w : 0.f;
+  };
+  // software pipeline over 64-event chunks: the next chunk's event data and the indices of
+  // the one after it are requested before the current chunk is processed
+  ConvChunkEv nx = conv_decompose((lane < P) ? ix[lane] : -1, g, tl, x0, y0, ux, uy);
+  ConvStageEv nf = conv_lane_event(nx.inu, fv, lane, zmax_hi);
+  int qn = (64 + lane < P) ? ix[64 + lane] : -1;
   for (int base = 0; base < P; base += 64) {
-    const int qi = qn;
-    if (base + 64 < P) qn = (base + 64 + lane < P) ? ix[base + 64 + lane] : -1;
-    int ebase = 0, dpk = 0;
-    bool inu = false;
-    if (qi >= 0) {
-      const int r = div_small(qi, g.Ci, tl.inv_ci);
-      const int pc = qi - r * g.Ci;
-      const int px = div_small(r, g.Y, tl.inv_y);
-      const int dx0 = px - x0, dy0 = r - px * g.Y - y0;
-      inu = ((unsigned)dx0 < (unsigned)ux) & ((unsigned)dy0 < (unsigned)uy);
-      ebase = ((dx0 * g.fy + dy0) * g.Ci + pc) * g.Co;
-      dpk = (dx0 << 16) | dy0;
+    const ConvChunkEv cu = nx;
+    wave_lds_fence();  // the previous chunk's stage reads are done
+    stg.ev[lane] = nf;
+    stg.eb[lane] = cu.ebase;
+    stg.dp[lane] = cu.dpk;
+    wave_lds_fence();
+    if (base + 64 < P) {
+      nx = conv_decompose(qn, g, tl, x0, y0, ux, uy);
+      nf = conv_lane_event(nx.inu, fv, base + 64 + lane, zmax_hi);
+      qn = (base + 128 + lane < P) ? ix[base + 128 + lane] : -1;
     }
-    unsigned long long m = __ballot(inu);
-    while (m) {
-      int ql[kConvBatch];
-#pragma unroll
-      for (int k = 0; k < kConvBatch; ++k) {
-        ql[k] = m ? (int)__builtin_ctzll(m) : -1;
+    unsigned long long m = __ballot(cu.inu);
+    if (!m) continue;
+    // the in-field events in order, each one's stage data and weight read one event ahead
+    int ql = (int)__builtin_ctzll(m);
+    m &= m - 1;
+    ConvStageEv E = stg.ev[ql];
+    int dpc = stg.dp[ql];
+    float w = weight(stg.eb[ql], dpc);
+    while (true) {
+      const bool more = m != 0;
+      int qn2 = 0, dpn = 0;
+      ConvStageEv En{};
+      float wn = 0.f;
+      if (more) {
+        qn2 = (int)__builtin_ctzll(m);
         m &= m - 1;
+        En = stg.ev[qn2];
+        dpn = stg.dp[qn2];
+        wn = weight(stg.eb[qn2], dpn);
       }
-      float w[kConvBatch];
-      bool own[kConvBatch];
-#pragma unroll
-      for (int k = 0; k < kConvBatch; ++k) {
-        own[k] = false;
-        w[k] = 0.f;
-        if (ql[k] >= 0) {
-          const int eb = __builtin_amdgcn_readlane(ebase, ql[k]);
-          const int dp = __builtin_amdgcn_readlane(dpk, ql[k]);
-          own[k] = active & ((unsigned)((dp >> 16) - ox) < (unsigned)g.fx) &
-                   ((unsigned)((dp & 0xffff) - oy) < (unsigned)g.fy);
-          if (own[k]) w[k] = WT[eb - loff];
+      const int q = base + ql;
+      bool own = active;
+      if (!kSingle)
+        own = own & ((unsigned)((dpc >> 16) - ox) < (unsigned)g.fx) &
+              ((unsigned)((dpc & 0xffff) - oy) < (unsigned)g.fy);
+      if (kAlwaysExact) {
+        if (own & live & seen) {
+          const SolveState r =
+              solve_crossings_ool(a, bb, n, (double)ev[last_q].t, (double)fminf(ev[q].t, tmax),
+                                  c, dtt, tau, K, row, out_t, out_a, out_x, out_z);
+          bb = r.b;
+          n = r.n;
+          live = r.live != 0;
+        }
+      } else {
+        const double zv = clamp_lane2(bb * inv2c, zq_last, E.zh);
+        const double gq = fma(fma(c, zv, -bb), zv, a);
+        if (own & (a > 0.0) & (gq <= kFilterEps * a)) {
+          const SolveStateZ r = solve_crossings_deferred_ends(
+              a, bb, n, zq_last, E.zh, ConvEnds{ev + last_q, ev + q, tmax}, c, inv2c, dtt, tau,
+              K, row, out_t, out_a, out_x, out_z);
+          bb = r.b;
+          n = r.n;
+          if (!r.live) a = -kInfD;
         }
       }
-#pragma unroll
-      for (int k = 0; k < kConvBatch; ++k) {
-        if (ql[k] < 0) break;
-        const int q = base + ql[k];
-        const FastEvent f = fv[q];
-        const double zl_hi = fmin(f.zq_lo * kLoToHi, zmax_hi);
-        if (kAlwaysExact) {
-          if (own[k] & live & seen) {
-            const SolveState r =
-                solve_crossings_ool(a, bb, n, (double)ev[last_q].t,
-                                    (double)fminf(ev[q].t, tmax), c, dtt, tau, K, row, out_t,
-                                    out_a, out_x, out_z);
-            bb = r.b;
-            n = r.n;
-            live = r.live != 0;
-          }
-        } else {
-          const double zv = fmin(fmax(bb * inv2c, zq_last), zl_hi);
-          const double gq = fma(fma(c, zv, -bb), zv, a);
-          if (own[k] & (a > 0.0) & (gq <= kFilterEps * a)) {
-            const SolveStateZ r = solve_crossings_deferred_ends(
-                a, bb, n, zq_last, zl_hi, ConvEnds{ev + last_q, ev + q, tmax}, c, inv2c, dtt,
-                tau, K, row, out_t, out_a, out_x, out_z);
-            bb = r.b;
-            n = r.n;
-            if (!r.live) a = -kInfD;
-          }
-        }
-        const double w64 = (double)w[k];
-        a = fma(w64, f.es, a);
-        bb = fma(w64, f.e, bb);
-        if (own[k]) {
-          zq_last = f.zq_lo;
-          last_q = q;
-          seen = true;
-        }
+      const double w64 = (double)w;
+      a = fma(w64, E.es, a);
+      bb = fma(w64, E.e, bb);
+      if (kSingle) {
+        zq_last = E.zq;
+        last_q = q;
+        seen = true;
+      } else if (own) {
+        zq_last = E.zq;
+        last_q = q;
+        seen = true;
       }
+      if (!more) break;
+      ql = qn2;
+      E = En;
+      dpc = dpn;
+      w = wn;
     }
   }
   // the last interval, up to t_max
@@ -207,7 +304,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv_solve_kernel(
       n = r.n;
     }
   } else {
-    const double zv = fmin(fmax(bb * inv2c, zq_last), zmax_hi);
+    const double zv = clamp_lane2(bb * inv2c, zq_last, zmax_hi);
     const double gq = fma(fma(c, zv, -bb), zv, a);
     if ((a > 0.0) & (gq <= kFilterEps * a)) {
       const SolveStateZ r = solve_crossings_deferred_ends(
@@ -463,26 +560,52 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv_wgrad_kernel(
   }
 }
 
-// grad[cc, tap] (per sample for mode 1, else summed over the batch, divided by B for mode 0)
-// from the workgroups' partial sums, added in (sample, chunk) order.
-__global__ void conv_wgrad_reduce_kernel(int B, int Co, int taps, ConvWgradTiling wt, int mode,
-                                         const double *__restrict__ partial,
-                                         float *__restrict__ grad) {
+// grad[cc, tap] (per sample for mode 1, else summed over the batch and divided by B for
+// mode 0) from the workgroups' partial sums, in two passes with a fixed order of additions.
+// Pass 1: block (group g = (sample for mode 1, channel, tap group), split s) of 64 x 4
+// threads; lane l = tap in the group, row r; the partials n = s*4 + r, s*4 + r + 4S, ... of
+// the group are summed per thread, then over the 4 rows, into stage[g][s][l].
+// Pass 2: one thread per output sums its S stage values.
+__global__ __launch_bounds__(256) void conv_wgrad_reduce1_kernel(
+    int B, ConvWgradTiling wt, int mode, int S, const double *__restrict__ partial,
+    double *__restrict__ stage) {
+  __shared__ double s_row[4][64];
+  const int l = threadIdx.x & 63, r = threadIdx.x >> 6;
+  const int g = blockIdx.x / S, sp = blockIdx.x - (blockIdx.x / S) * S;
+  const int tg = g % wt.tgroups;
+  const int cc = (g / wt.tgroups);  // channel, or sample * Co + channel for mode 1
+  const int Co = wt.CG * wt.cgroups;  // channel capacity of the groups
+  const int bo = mode == 1 ? cc / Co : 0;
+  const int ch_c = mode == 1 ? cc - bo * Co : cc;
+  const int cg = ch_c / wt.CG, u = ch_c - cg * wt.CG;
+  const int N = mode == 1 ? wt.nchunks : B * wt.nchunks;
+  double v = 0.0;
+  for (int n = sp * 4 + r; n < N; n += 4 * S) {
+    const int b = mode == 1 ? bo : n / wt.nchunks;
+    const int ch = mode == 1 ? n : n - (n / wt.nchunks) * wt.nchunks;
+    const int64_t blk = (((int64_t)b * wt.nchunks + ch) * wt.tgroups + tg) * wt.cgroups + cg;
+    v += partial[(blk * wt.CG + u) * 64 + l];
+  }
+  s_row[r][l] = v;
+  __syncthreads();
+  if (r == 0)
+    stage[(size_t)blockIdx.x * 64 + l] = ((s_row[0][l] + s_row[1][l]) + s_row[2][l]) + s_row[3][l];
+}
+
+__global__ void conv_wgrad_reduce2_kernel(int B, int Co_real, int taps, ConvWgradTiling wt,
+                                          int mode, int S, const double *__restrict__ stage,
+                                          float *__restrict__ grad) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)Co * taps;
+  const int64_t per = (int64_t)Co_real * taps;
   if (e >= (mode == 1 ? (int64_t)B * per : per)) return;
   const int bo = (int)(e / per);
-  const int r = (int)(e - (int64_t)bo * per);
-  const int cc = r / taps, tap = r - cc * taps;
-  const int cg = cc / wt.CG, u = cc - cg * wt.CG;
+  const int rr = (int)(e - (int64_t)bo * per);
+  const int cc = rr / taps, tap = rr - cc * taps;
   const int tg = tap / 64, l = tap - tg * 64;
-  const int b0 = mode == 1 ? bo : 0, b1 = mode == 1 ? bo + 1 : B;
+  const int Co = wt.CG * wt.cgroups;
+  const int64_t g = ((int64_t)(mode == 1 ? bo * Co + cc : cc)) * wt.tgroups + tg;
   double s = 0.0;
-  for (int b = b0; b < b1; ++b)
-    for (int ch = 0; ch < wt.nchunks; ++ch) {
-      const int64_t blk = (((int64_t)b * wt.nchunks + ch) * wt.tgroups + tg) * wt.cgroups + cg;
-      s += partial[(blk * wt.CG + u) * 64 + l];
-    }
+  for (int sp = 0; sp < S; ++sp) s += stage[((size_t)g * S + sp) * 64 + l];
   grad[e] = (float)(mode == 0 ? s / (double)B : s);
 }
 
@@ -678,29 +801,39 @@ extern "C" int bats_conv_lif_forward(void *stream, int B, const int *pre_shape, 
   const ConvTiling tl = conv_tiling(g);
   const long long units = (long long)B * tl.tilesX * tl.tilesY * tl.cgroups;
   BATS_REQUIRE(units <= (long long)INT32_MAX - 64, "conv layer too large");
-  const int blocks = (int)((units + kConvWaves - 1) / kConvWaves);
-  if (c > 0.f) {
-    BATS_TIMED("conv_solve_kernel", st,
-               hipLaunchKernelGGL(conv_solve_kernel<false>, dim3(blocks), dim3(64 * kConvWaves),
-                                  0, st, el.events, el.fast, el.idx, el.counts, el.Pcap, g, tl,
-                                  (int)units, WT, K_post, (double)c, (double)delta_theta_tau,
-                                  (double)tau, max_simulation, out_n, out_t, out_a, out_x,
-                                  out_z));
-  } else {
-    BATS_TIMED("conv_solve_kernel", st,
-               hipLaunchKernelGGL(conv_solve_kernel<true>, dim3(blocks), dim3(64 * kConvWaves),
-                                  0, st, el.events, el.fast, el.idx, el.counts, el.Pcap, g, tl,
-                                  (int)units, WT, K_post, (double)c, (double)delta_theta_tau,
-                                  (double)tau, max_simulation, out_n, out_t, out_a, out_x,
-                                  out_z));
-  }
+  const int blocks = (int)((units + kConvSolveWaves - 1) / kConvSolveWaves);
+  const size_t wbytes = sizeof(float) * (size_t)taps * g.Co;
+  const bool lds_w = conv_solve_lds_bytes(wbytes, true) <= 80 * 1024;
+  const bool single = tl.PX * tl.PY == 1;
+  const bool exact = !(c > 0.f);
+  using KernT = void (*)(const Event *, const FastEvent *, const int *, const int *, int,
+                         ConvGeom, ConvTiling, int, const float *, int, double, double, double,
+                         float, int *, float *, float *, float *, float *);
+  static const KernT kernels[8] = {
+      conv_solve_kernel<false, false, false>, conv_solve_kernel<false, false, true>,
+      conv_solve_kernel<false, true, false>,  conv_solve_kernel<false, true, true>,
+      conv_solve_kernel<true, false, false>,  conv_solve_kernel<true, false, true>,
+      conv_solve_kernel<true, true, false>,   conv_solve_kernel<true, true, true>};
+  const KernT kern = kernels[(exact ? 4 : 0) + (lds_w ? 2 : 0) + (single ? 1 : 0)];
+  const size_t lds = conv_solve_lds_bytes(wbytes, lds_w);
+  if (lds > 64 * 1024)
+    BATS_REQUIRE(hipFuncSetAttribute((const void *)kern,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds) == hipSuccess,
+                 "conv_solve_kernel: LDS request above the device limit");
+  BATS_TIMED("conv_solve_kernel", st,
+             hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kConvSolveWaves), lds, st,
+                                el.events, el.fast, el.idx, el.counts, el.Pcap, g, tl,
+                                (int)units, WT, K_post, (double)c, (double)delta_theta_tau,
+                                (double)tau, max_simulation, out_n, out_t, out_a, out_x, out_z));
   return check_launch("conv_solve_kernel");
 }
 
 // Workspace of the conv backward (byte offsets) and the weight-gradient tiling.
 struct ConvBwdLayout {
-  size_t n_eff, D, rec, partial, spk, slot, count, pos_off, total;
+  size_t n_eff, D, rec, partial, stage, spk, slot, count, pos_off, total;
   int rslots, cap;
+  int S1;  // splits of the first reduction pass
   ConvWgradTiling wt;
 };
 
@@ -732,6 +865,13 @@ static ConvBwdLayout conv_bwd_layout(int B, const ConvGeom &g, int K_pre, int K_
   off = align_up(off + sizeof(float4) * (size_t)B * npos * L.rslots, 256);
   L.partial = off;
   off = align_up(off + sizeof(double) * (size_t)wt.blocks * wt.CG * 64, 256);
+  // first reduction pass: about 16 partials per thread
+  const int Nred = wt.nchunks * std::max(B, 1);
+  L.S1 = std::max(1, std::min(256, (Nred + 63) / 64));
+  const int64_t groups = (int64_t)wt.cgroups * wt.CG * wt.tgroups;  // modes 0 and 2
+  const int64_t groups1 = groups * std::max(B, 1);                 // mode 1 (S = 1)
+  L.stage = off;
+  off = align_up(off + sizeof(double) * 64 * (size_t)std::max(groups * L.S1, groups1), 256);
   L.spk = off;
   off = align_up(off + sizeof(float4) * (size_t)B * L.cap, 256);
   L.slot = off;
@@ -810,10 +950,20 @@ extern "C" int bats_conv_lif_backward(void *stream, int B, const int *pre_shape,
                                 st, g, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau, L.rslots,
                                 rec, wt, partial));
   if (int r = check_launch("conv_wgrad_kernel")) return r;
-  const int64_t fin = (int64_t)(grad_mode == 1 ? B : 1) * g.Co * taps;
-  hipLaunchKernelGGL(conv_wgrad_reduce_kernel, dim3((unsigned)((fin + 255) / 256)), dim3(256), 0,
-                     st, B, g.Co, taps, wt, grad_mode, partial, grad);
-  if (int r = check_launch("conv_wgrad_reduce_kernel")) return r;
+  {
+    double *stage = reinterpret_cast<double *>(ws + L.stage);
+    const int S = grad_mode == 1 ? 1 : L.S1;
+    const int64_t groups =
+        (int64_t)(grad_mode == 1 ? B : 1) * wt.cgroups * wt.CG * wt.tgroups;
+    BATS_TIMED("conv_wgrad_reduce_kernel", st,
+               hipLaunchKernelGGL(conv_wgrad_reduce1_kernel, dim3((unsigned)(groups * S)),
+                                  dim3(256), 0, st, B, wt, grad_mode, S, partial, stage));
+    if (int r = check_launch("conv_wgrad_reduce1_kernel")) return r;
+    const int64_t fin = (int64_t)(grad_mode == 1 ? B : 1) * g.Co * taps;
+    hipLaunchKernelGGL(conv_wgrad_reduce2_kernel, dim3((unsigned)((fin + 255) / 256)),
+                       dim3(256), 0, st, B, g.Co, taps, wt, grad_mode, S, stage, grad);
+    if (int r = check_launch("conv_wgrad_reduce2_kernel")) return r;
+  }
   if (pre_errors && K_pre > 0) {
     BATS_REQUIRE(weights, "null weights");
     float4 *spk = reinterpret_cast<float4 *>(ws + L.spk);

@@ -5,6 +5,7 @@ for profiling one kernel family at a time:
 
 It runs only the requested operator (after one warm-up) on the same synthetic inputs as
 bench.py, plus one elementwise copy of a known size used to calibrate HBM traffic counters.
+The conv_* operators are the two conv layers of bench.py's convolutional leg (batch 128).
 """
 from __future__ import annotations
 
@@ -21,7 +22,9 @@ from ._constants import lif_constants
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--what", choices=("forward1", "forward2", "backward1", "backward2",
-                                       "forward_hh", "backward_hh", "copy"),
+                                       "forward_hh", "backward_hh", "conv_forward1",
+                                       "conv_forward2", "conv_backward1", "conv_backward2",
+                                       "copy"),
                     default="forward1")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--batch", type=int, default=256)
@@ -46,6 +49,26 @@ def main():
         w3 = torch.as_tensor(rng.normal(0, 1.0, size=(800, 800)).astype(np.float32), device=dev)
         hh = ops.lif_forward(h[1], w3, *c1, 1.0, 30)
         err3 = torch.randn(hh[1].shape, device=dev)
+    if args.what.startswith("conv_"):
+        # the two conv layers of the 28x28-c15-p-c40-p-300-10 network at batch 128, with the
+        # weights and inputs that bench.py's conv leg draws (same generator and order)
+        crng = np.random.default_rng(3)
+        cw1 = torch.as_tensor(crng.normal(0.0, 1.0 / 5.0, size=(15, 5, 5, 1)).astype(np.float32),
+                              device=dev)
+        cw2 = torch.as_tensor((8.0 * crng.normal(0.0, 1.0 / np.sqrt(375), size=(40, 5, 5, 15)))
+                              .astype(np.float32), device=dev)
+        crng.normal(size=(300, 640))
+        crng.normal(size=(10, 300))
+        cx = crng.uniform(0.0, 1.0, size=(128, 784))
+        ct = torch.as_tensor(np.where(cx > 0.1, 1.0 - cx, np.inf).astype(np.float32)[:, :, None],
+                             device=dev)
+        k1 = lif_constants(0.130, 0.04, 0.04)
+        k2 = lif_constants(0.130, 0.8, 0.8)
+        o1 = ops.conv_lif_forward(ct, (28, 28, 1), cw1, *k1, 1.0, 30)
+        p1n, p1t, _ = ops.pool_forward(o1[0], o1[1], (24, 24, 15))
+        o2 = ops.conv_lif_forward(p1t, (12, 12, 15), cw2, *k2, 1.0, 3)
+        ce1 = torch.randn(o1[1].shape, device=dev)
+        ce2 = torch.randn(o2[1].shape, device=dev)
     src = torch.empty(args.copy_mib * 2 ** 18, dtype=torch.float32, device=dev).uniform_()
 
     def run():
@@ -61,6 +84,16 @@ def main():
             ops.lif_forward(h[1], w3, *c1, 1.0, 30)
         elif args.what == "backward_hh":
             ops.lif_backward(*hh[:5], err3.clone(), h[1], hh[5], hh[6], w3, *c1)
+        elif args.what == "conv_forward1":
+            ops.conv_lif_forward(ct, (28, 28, 1), cw1, *k1, 1.0, 30)
+        elif args.what == "conv_forward2":
+            ops.conv_lif_forward(p1t, (12, 12, 15), cw2, *k2, 1.0, 3)
+        elif args.what == "conv_backward1":
+            ops.conv_lif_backward(*o1[:5], ce1.clone(), ct, (28, 28, 1), o1[5], o1[6], cw1, *k1,
+                                  pre_trainable=False)
+        elif args.what == "conv_backward2":
+            ops.conv_lif_backward(*o2[:5], ce2.clone(), p1t, (12, 12, 15), o2[5], o2[6], cw2,
+                                  *k2, pre_trainable=True)
         else:
             torch.add(src, 0.0)
 
@@ -70,8 +103,12 @@ def main():
     for _ in range(args.reps):
         run()
     torch.cuda.synchronize()
+    extra = ""
+    if args.what.startswith("conv_"):
+        extra = (f", conv1 / pool1 / conv2 spikes per sample {o1[0].sum().item() / 128:.1f} / "
+                 f"{p1n.sum().item() / 128:.1f} / {o2[0].sum().item() / 128:.1f}")
     print(f"{args.what}: {(time.perf_counter() - t0) * 1e3 / args.reps:.3f} ms/call "
-          f"(hidden spikes/sample {h[0].sum().item() / B:.1f})", flush=True)
+          f"(hidden spikes/sample {h[0].sum().item() / B:.1f}{extra})", flush=True)
 
 
 if __name__ == "__main__":

@@ -409,7 +409,6 @@ def config_legs(args):
     from bats_amd.Optimizers import AdamOptimizer
 
     dev = torch.device("cuda", 0)
-    rng = np.random.default_rng(args.seed + 1)
     out = {}
 
     def dense_net(sizes, Ks, thetas, scale):
@@ -426,7 +425,11 @@ def config_legs(args):
             prev = lay
         return net
 
+    wanted = [w.strip() for w in args.configs.split(",") if w.strip()]
+
     def run(name, net, batch, inputs, loss_fct, desc, steps=10, warmup=3):
+        if not any(name.startswith(w) for w in wanted):
+            return
         t_d, n_d = inputs
         lab = torch.as_tensor(rng.integers(0, 10, size=batch), device=dev)
         opt = AdamOptimizer(learning_rate=LR)
@@ -453,17 +456,22 @@ def config_legs(args):
         return (torch.as_tensor(t, device=dev),
                 torch.as_tensor(np.isfinite(t[:, :, 0]).astype(np.int32), device=dev))
 
+    # every leg draws its weights and inputs from its own generator, so that a leg's workload
+    # does not depend on which other legs run
     # single-spike TTFS network (784-800-10, K = 1, batch 50, softmax on first spike times)
+    rng = np.random.default_rng(args.seed + 1)
     run("ttfs_784-800-10_b50", dense_net([N_IN, N_HID, N_OUT], [1, 1], [1.0, 1.3], 1.0), 50,
         latency(50), TTFSSoftmaxCrossEntropy(tau=0.005),
         "784-800-10, K=1/1, theta 1.0/1.3, W~N(0,1), TTFS loss tau 0.005, Adam")
     # deep dense network (784-800-800-800-10, K = 30, batch 256)
+    rng = np.random.default_rng(args.seed + 2)
     run("deep_784-800x3-10_b256",
         dense_net([N_IN, 800, 800, 800, N_OUT], [30, 30, 30, 30], [1.0, 1.0, 1.0, 1.3],
                   1.0), 256, latency(256), SpikeCountClassLoss(target_true=15, target_false=3),
         "784-800-800-800-10, K=30, theta 1.0 (1.3 output), W~N(0,1), count loss 15/3, Adam")
     # convolutional network (batch 128): conv5x5x15 (K 30) -> pool -> conv5x5x40 (K 3) ->
     # pool -> 300 (K 10) -> 10 (K 30), thresholds of experiments/mnist/train_conv.py
+    rng = np.random.default_rng(args.seed + 3)
     net = Network()
     inp = ConvInputLayer(neurons_shape=np.array([28, 28, 1]), name="input")
     net.add_layer(inp, input=True)
@@ -530,6 +538,8 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--configs", default="ttfs,deep,conv",
+                    help="training-step legs to run (comma list of ttfs, deep, conv)")
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the TTFS, deep and conv training-step legs")
     args = ap.parse_args()

@@ -561,6 +561,13 @@ __global__ __launch_bounds__(kCompThreads) void compact_spikes_kernel(
 // compacted pre spikes and walks the record, adding es d1 - e d2 (t_pre < t_post) with a
 // no-return LDS atomic. Lanes of one neuron (and, at a chunk boundary, two waves) may add to
 // one address, so the float64 sums are exact up to the order of those additions.
+// 24-bit unsigned product (full rate; v_mul_lo_u32 runs at a quarter of it). Operands < 2^24.
+__device__ __forceinline__ unsigned mul_u24(unsigned a, unsigned b) {
+  unsigned r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "s"(b));
+  return r;
+}
+
 __host__ __device__ inline size_t spk_wgrad_lds_bytes(int I, int K) {
   return (size_t)kWT * I * sizeof(double) + (size_t)tile_rec_slots(K) * sizeof(float4);
 }
@@ -580,11 +587,17 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
   const int b0 = sl * b_per_slice, b1 = min(B, b0 + b_per_slice);
   const int R = tile_rec_slots(K);
   for (int e = tid; e < kWT * I; e += nth) acc[e] = 0.0;
+  // the next sample's record: the thread's entry tid (up to nth entries) in a register
+  float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (b0 < b1 && tid < R - kRecHead) rn = rec[((size_t)b0 * gj + tj) * R + kRecHead + tid];
   for (int b = b0; b < b1; ++b) {
     const float4 *grec = rec + ((size_t)b * gj + tj) * R;
     __syncthreads();  // the previous record is consumed (and acc is zeroed)
     const int cnt = __float_as_int(grec[0].x);
-    for (int e = tid; e < cnt; e += nth) rb[e] = grec[kRecHead + e];
+    if (tid < cnt) rb[tid] = rn;
+    for (int e = tid + nth; e < cnt; e += nth) rb[e] = grec[kRecHead + e];
+    if (b + 1 < b1 && tid < R - kRecHead)
+      rn = rec[((size_t)(b + 1) * gj + tj) * R + kRecHead + tid];
     __syncthreads();
     const int P = count[b];
     for (int c = wv; c * 64 < P; c += nw) {
@@ -592,15 +605,23 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
       const bool valid = sidx < P;
       const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
       double *al = acc + (valid ? __float_as_int(S.w) : 0);
-      for (int k = 0; k < cnt; ++k) {
-        const float4 E = rb[k];
-        float q = fmaf(S.y, E.y, -(S.z * E.z));
-        asm volatile("" : "+v"(q));
-        q = (S.x < E.x) ? q : 0.f;
-        if (valid)
-          __hip_atomic_fetch_add(al + (size_t)(__float_as_int(E.w) >> 9) * I, (double)q,
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // entries four at a time (their LDS reads in one round); a pair adds only where the
+      // causal mask holds (a skipped term is an exact zero)
+      auto term = [&](const float4 E) {
+        const float q = fmaf(S.y, E.y, -(S.z * E.z));
+        if (S.x < E.x)
+          __hip_atomic_fetch_add(al + mul_u24((unsigned)__float_as_int(E.w) >> 9, (unsigned)I),
+                                 (double)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      };
+      int k = 0;
+      for (; k + 4 <= cnt; k += 4) {
+        const float4 E0 = rb[k], E1 = rb[k + 1], E2 = rb[k + 2], E3 = rb[k + 3];
+        term(E0);
+        term(E1);
+        term(E2);
+        term(E3);
       }
+      for (; k < cnt; ++k) term(rb[k]);
     }
   }
   __syncthreads();
@@ -620,6 +641,8 @@ __global__ __launch_bounds__(64 * kTileWaves) void pre_err_spk_kernel(
     const int *__restrict__ slot, const int *__restrict__ count, const float *__restrict__ W,
     const float4 *__restrict__ rec, double inv_tau_s, double inv_tau,
     float *__restrict__ pre_errors) {
+  constexpr int kNth = 64 * kTileWaves;
+  constexpr int kPf = 2;  // record entries per thread prefetched for the next tile
   extern __shared__ __align__(16) char smem[];
   float *wl = reinterpret_cast<float *>(smem);  // [kTileWaves][kWT][64]
   float4 *rb = reinterpret_cast<float4 *>(wl + kTileWaves * kWT * 64);
@@ -627,7 +650,7 @@ __global__ __launch_bounds__(64 * kTileWaves) void pre_err_spk_kernel(
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int b = blockIdx.y;
   const int P = count[b];
-  const int s0 = blockIdx.x * (64 * kTileWaves);
+  const int s0 = blockIdx.x * kNth;
   if (s0 >= P) return;  // whole block
   const int sidx = s0 + tid;
   const bool valid = sidx < P;
@@ -636,24 +659,50 @@ __global__ __launch_bounds__(64 * kTileWaves) void pre_err_spk_kernel(
   const float esd = (float)((double)S.y * inv_tau_s), ed = (float)((double)S.z * inv_tau);
   float *ww = wl + wv * kWT * 64;
   const int R = tile_rec_slots(K);
+  // the next tile's record entries (tid, tid + kNth) and weights, requested one tile ahead
+  float4 rn[kPf];
+  float wn[kWT];
+  auto prefetch = [&](int tj) {
+    const float4 *grec = rec + ((size_t)b * gj + tj) * R + kRecHead;
+#pragma unroll
+    for (int p = 0; p < kPf; ++p) {
+      const int e = tid + p * kNth;
+      rn[p] = e < R - kRecHead ? grec[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const int j0 = tj * kWT;
+#pragma unroll
+    for (int u = 0; u < kWT; ++u)
+      wn[u] = (valid && j0 + u < J) ? W[(size_t)(j0 + u) * I + i] : 0.f;
+  };
+  prefetch(0);
   double pe = 0.0;
   for (int tj = 0; tj < gj; ++tj) {
     const float4 *grec = rec + ((size_t)b * gj + tj) * R;
-    const int j0 = tj * kWT;
     __syncthreads();  // the previous tile is consumed
     const int cnt = __float_as_int(grec[0].x);
-    for (int e = tid; e < cnt; e += 64 * kTileWaves) rb[e] = grec[kRecHead + e];
 #pragma unroll
-    for (int u = 0; u < kWT; ++u)
-      ww[u * 64 + lane] = (valid && j0 + u < J) ? W[(size_t)(j0 + u) * I + i] : 0.f;
+    for (int p = 0; p < kPf; ++p)
+      if (tid + p * kNth < cnt) rb[tid + p * kNth] = rn[p];
+    for (int e = tid + kPf * kNth; e < cnt; e += kNth) rb[e] = grec[kRecHead + e];
+#pragma unroll
+    for (int u = 0; u < kWT; ++u) ww[u * 64 + lane] = wn[u];
+    if (tj + 1 < gj) prefetch(tj + 1);
     __syncthreads();
     float pt = 0.f;
-    for (int k = 0; k < cnt; ++k) {
-      const float4 E = rb[k];
+    auto term = [&](const float4 E) {
       const float w = ww[(__float_as_int(E.w) >> 9) * 64 + lane];
       const float q = fmaf(esd, E.y, -(ed * E.z));
       pt = fmaf((S.x < E.x) ? w : 0.f, q, pt);
+    };
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+      const float4 E0 = rb[k], E1 = rb[k + 1], E2 = rb[k + 2], E3 = rb[k + 3];
+      term(E0);
+      term(E1);
+      term(E2);
+      term(E3);
     }
+    for (; k < cnt; ++k) term(rb[k]);
     pe += (double)pt;
   }
   if (valid) pre_errors[(size_t)b * I * K_pre + slot[(size_t)b * cap + sidx]] = (float)pe;

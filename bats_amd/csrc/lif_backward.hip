@@ -555,12 +555,68 @@ __global__ __launch_bounds__(kCompThreads) void compact_spikes_kernel(
   if (tid == 0) count[b] = s_base;
 }
 
+// The same list in rank-major order: every neuron's first spike (in neuron order), then every
+// second spike, and so on, so that the 64 spikes of a wave's chunk belong to distinct neurons
+// (the weight-gradient kernel below adds them to per-neuron LDS addresses: lanes that share an
+// address serialise). n_i is kept in LDS (I ints), and each rank is one pass of
+// wave-aggregated block scans over the neurons.
+__global__ __launch_bounds__(kCompThreads) void compact_spikes_rank_kernel(
+    int I, int K_pre, const float *__restrict__ pre_t, const float *__restrict__ pre_es,
+    const float *__restrict__ pre_e, int cap, float4 *__restrict__ spk, int *__restrict__ slot,
+    int *__restrict__ count) {
+  extern __shared__ int s_n[];  // [I]
+  __shared__ int s_wave[kCompThreads / 64];
+  __shared__ int s_max;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) s_max = 0;
+  __syncthreads();
+  int mx = 0;
+  for (int i = tid; i < I; i += kCompThreads) {
+    const int n = leading_finite(pre_t + ((size_t)b * I + i) * K_pre, K_pre);
+    s_n[i] = n;
+    mx = max(mx, n);
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, 64));
+  if (lane == 0) atomicMax(&s_max, mx);
+  __syncthreads();
+  const int maxn = s_max;
+  float4 *o = spk + (size_t)b * cap;
+  int *os = slot + (size_t)b * cap;
+  int base = 0;
+  for (int m = 0; m < maxn; ++m) {
+    for (int i0 = 0; i0 < I; i0 += kCompThreads) {
+      const int i = i0 + tid;
+      const bool flag = i < I && s_n[i] > m;
+      const unsigned long long bal = __ballot(flag);
+      if (lane == 0) s_wave[wv] = __popcll(bal);
+      __syncthreads();
+      int off = base + (int)__popcll(bal & ((1ull << lane) - 1ull)), total = 0;
+#pragma unroll
+      for (int w = 0; w < kCompThreads / 64; ++w) {
+        off += w < wv ? s_wave[w] : 0;
+        total += s_wave[w];
+      }
+      if (flag) {
+        const size_t row = ((size_t)b * I + i) * K_pre + m;
+        o[off] = make_float4(pre_t[row], pre_es[row], pre_e[row], __int_as_float(i));
+        os[off] = i * K_pre + m;
+      }
+      base += total;
+      __syncthreads();  // s_wave is read by every wave before the next chunk writes it
+    }
+  }
+  if (tid == 0) count[b] = base;
+}
+
 // Weight gradient with a lane per pre spike: one workgroup per (tile of kWT post rows, batch
 // slice) accumulates acc[row][pre neuron] in float64 in LDS. Per sample the tile's record
 // (row_prep_kernel) is copied to LDS, and each wave takes 64-spike chunks of the sample's
 // compacted pre spikes and walks the record, adding es d1 - e d2 (t_pre < t_post) with a
-// no-return LDS atomic. Lanes of one neuron (and, at a chunk boundary, two waves) may add to
-// one address, so the float64 sums are exact up to the order of those additions.
+// no-return LDS atomic. With the rank-major list the lanes of a chunk belong to distinct
+// neurons except where a rank ends inside the chunk; lanes of one neuron, and waves, may still
+// add to one address, so the float64 sums are exact up to the order of those additions.
 // 24-bit unsigned product (full rate; v_mul_lo_u32 runs at a quarter of it). Operands < 2^24.
 __device__ __forceinline__ unsigned mul_u24(unsigned a, unsigned b) {
   unsigned r;
@@ -1054,10 +1110,19 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   const int cap = I * K_pre;
   if (multi) {
     BATS_REQUIRE((int64_t)I * K_pre < (int64_t)1 << 31, "pre spike buffer too large");
-    BATS_TIMED("compact_spikes_kernel",
-               st, hipLaunchKernelGGL(compact_spikes_kernel, dim3(B), dim3(kCompThreads), 0, st,
-                                      I, K_pre, pre_times, pre_exp_tau_s, pre_exp_tau, cap, spk,
-                                      spk_slot, spk_count));
+    // rank-major order when the per-neuron counts fit in LDS (every caller here reads the
+    // list through `slot` or per spike, so the order is free)
+    const bool rank = (size_t)I * sizeof(int) <= 64 * 1024;
+    BATS_TIMED("compact_spikes_kernel", st, {
+      if (rank)
+        hipLaunchKernelGGL(compact_spikes_rank_kernel, dim3(B), dim3(kCompThreads),
+                           sizeof(int) * (size_t)I, st, I, K_pre, pre_times, pre_exp_tau_s,
+                           pre_exp_tau, cap, spk, spk_slot, spk_count);
+      else
+        hipLaunchKernelGGL(compact_spikes_kernel, dim3(B), dim3(kCompThreads), 0, st, I, K_pre,
+                           pre_times, pre_exp_tau_s, pre_exp_tau, cap, spk, spk_slot,
+                           spk_count);
+    });
     if (int r = check_launch("compact_spikes_kernel")) return r;
   }
   if (L.spk_path && grad_mode != 1) {

@@ -106,10 +106,46 @@ __device__ __forceinline__ void bitonic1024_regs(uint64_t &x0, uint64_t &x1,
   }
 }
 
+// Merge passes over sorted runs of L0 keys (N / L0 a power of two, N >= 2 * blockDim.x): each
+// pass merges run pairs into runs of twice the length, every thread producing N / blockDim.x
+// consecutive outputs from its merge-path split (a binary search on its diagonal). Keys are
+// distinct except the ~0 padding. Returns the buffer holding the sorted keys.
+__device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0) {
+  const int E = N / (int)blockDim.x;
+  for (int L = L0; L < N; L <<= 1) {
+    const int d = threadIdx.x * E;
+    const int pair = d / (2 * L), d0 = d - pair * 2 * L;
+    const uint64_t *A = src + (size_t)pair * 2 * L, *Bv = A + L;
+    int lo = max(0, d0 - L), hi = min(d0, L);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (A[mid] < Bv[d0 - 1 - mid])
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    int ia = lo, ib = d0 - lo;
+    uint64_t *out = dst + (size_t)pair * 2 * L + d0;
+    for (int e = 0; e < E; ++e) {
+      const uint64_t av = ia < L ? A[ia] : ~0ull, bv = ib < L ? Bv[ib] : ~0ull;
+      const bool take_a = (ib >= L) || (ia < L && av < bv);
+      out[e] = take_a ? av : bv;
+      ia += take_a ? 1 : 0;
+      ib += take_a ? 0 : 1;
+    }
+    __syncthreads();
+    uint64_t *tmp = src;
+    src = dst;
+    dst = tmp;
+  }
+  return src;
+}
+
 // One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
 // sample's Event list padded to a multiple of kEventPad, the event indices, the count, and
 // the per-slot float32 exps the backward reads. Up to 1024 events the sort runs in
-// registers (bitonic1024_regs); longer lists use the LDS (or global) network.
+// registers (bitonic1024_regs); longer lists that fit twice in the LDS are sorted as 1024-key
+// register runs merged by merge_runs, the rest with the LDS (or global) bitonic network.
 __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
     float tmax, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
@@ -183,6 +219,20 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     keys[2 * threadIdx.x] = x0;
     keys[2 * threadIdx.x + 1] = x1;
     __syncthreads();
+  } else if (!use_global && 2 * N <= lds_cap) {
+    // 1024-key runs sorted in registers, then merge-path passes between two LDS buffers
+    for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
+    __syncthreads();
+    const int t = threadIdx.x;
+    for (int c0 = 0; c0 < N; c0 += 1024) {
+      uint64_t x0 = keys[c0 + 2 * t], x1 = keys[c0 + 2 * t + 1];
+      bitonic1024_regs(x0, x1, keys + c0);  // exchanges through the run's own slots
+      __syncthreads();
+      keys[c0 + 2 * t] = x0;
+      keys[c0 + 2 * t + 1] = x1;
+      __syncthreads();
+    }
+    keys = merge_runs(keys, keys + N, N, 1024);
   } else {
     for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
     if (use_global) __threadfence();

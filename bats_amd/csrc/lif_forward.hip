@@ -168,23 +168,45 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
   // capacity are only counted; such a sample is compacted again into global memory.
   constexpr int kUnroll = 8;
   const int nth = blockDim.x;
-  for (int s0 = 0; s0 < S; s0 += kUnroll * nth) {
-    float tv[kUnroll];
+  auto take = [&](float tv, int s) {
+    const bool fin = isfinite(tv);
+    const unsigned long long m = __ballot(fin);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(&s_fill, __popcll(m));
+    base = __shfl(base, 0, 64);
+    const int pos = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+    if (fin && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(tv) << 32) | (uint32_t)s;
+  };
+  if ((S & 3) == 0 && (reinterpret_cast<size_t>(times) & 15) == 0) {
+    // 16-byte loads: four slots per load, kUnroll loads in flight per thread
+    const float4 *t4 = reinterpret_cast<const float4 *>(times);
+    const int S4 = S >> 2;
+    for (int s0 = 0; s0 < S4; s0 += kUnroll * nth) {
+      float4 tv[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int s = s0 + u * nth + threadIdx.x;
-      tv[u] = s < S ? times[s] : kInfF;
+      for (int u = 0; u < kUnroll; ++u) {
+        const int q = s0 + u * nth + threadIdx.x;
+        tv[u] = q < S4 ? t4[q] : make_float4(kInfF, kInfF, kInfF, kInfF);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int s = 4 * (s0 + u * nth + threadIdx.x);
+        take(tv[u].x, s);
+        take(tv[u].y, s + 1);
+        take(tv[u].z, s + 2);
+        take(tv[u].w, s + 3);
+      }
     }
+  } else {
+    for (int s0 = 0; s0 < S; s0 += kUnroll * nth) {
+      float tv[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int s = s0 + u * nth + threadIdx.x;
-      const bool fin = isfinite(tv[u]);
-      const unsigned long long m = __ballot(fin);
-      int base = 0;
-      if (lane == 0 && m) base = atomicAdd(&s_fill, __popcll(m));
-      base = __shfl(base, 0, 64);
-      const int pos = base + (int)__popcll(m & ((1ull << lane) - 1ull));
-      if (fin && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(tv[u]) << 32) | (uint32_t)s;
+      for (int u = 0; u < kUnroll; ++u) {
+        const int s = s0 + u * nth + threadIdx.x;
+        tv[u] = s < S ? times[s] : kInfF;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) take(tv[u], s0 + u * nth + threadIdx.x);
     }
   }
   __syncthreads();

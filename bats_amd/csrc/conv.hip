@@ -316,7 +316,22 @@ __global__ __launch_bounds__(64 * kConvSolveWaves) void conv_solve_kernel(
   if (active) {
     if (!kAlwaysExact) finish_spike_times(row, n, tau, out_t, out_z);
     out_n[(size_t)b * g.Xo * g.Yo * g.Co + j] = n;
-    for (int k = n; k < K; ++k) out_t[row + k] = kInfF;
+  }
+  // the +inf tails, written by the whole wave: the channels of one position are consecutive
+  // rows, so each position of the tile is one contiguous block of CW x K slots
+  wave_lds_fence();
+  stg.eb[lane] = active ? n : K;
+  wave_lds_fence();
+  const int cw = min(tl.CW, g.Co - cg * tl.CW);
+  for (int pt = 0; pt < tl.PX * tl.PY; ++pt) {
+    const int px_ = x0 + pt / tl.PY, py_ = y0 + pt % tl.PY;
+    if (px_ >= g.Xo || py_ >= g.Yo) continue;
+    float *blk = out_t + ((size_t)b * g.Xo * g.Yo * g.Co +
+                          ((size_t)px_ * g.Yo + py_) * g.Co + cg * tl.CW) * K;
+    for (int e = lane; e < cw * K; e += 64) {
+      const int r = e / K;
+      if (e - r * K >= stg.eb[pt * tl.CW + r]) blk[e] = kInfF;
+    }
   }
 }
 

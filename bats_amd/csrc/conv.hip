@@ -118,29 +118,32 @@ __device__ __forceinline__ ConvChunkEv conv_decompose(int qi, const ConvGeom &g,
 
 // The event data of an in-field event, as staged for the wave: es, e, the lower end zq_lo of
 // the interval that the event opens and the upper end zl_hi of the interval that it closes.
-struct alignas(16) ConvStageEv {
-  double es, e, zq, zh;
+struct ConvStageEv {
+  double zq, zh;
+  float es, e;  // float32 values (the event records hold them widened to float64)
 };
 
 __device__ __forceinline__ ConvStageEv conv_lane_event(bool inu, const FastEvent *fv, int q,
                                                        double zmax_hi) {
   constexpr double kLoToHi = (1.0 + kFilterEps) / (1.0 - kFilterEps);
-  ConvStageEv r{0.0, 0.0, 0.0, 0.0};
+  ConvStageEv r{0.0, 0.0, 0.f, 0.f};
   if (inu) {
     const FastEvent f = fv[q];
-    r.es = f.es;
-    r.e = f.e;
+    r.es = (float)f.es;
+    r.e = (float)f.e;
     r.zq = f.zq_lo;
     r.zh = fmin(f.zq_lo * kLoToHi, zmax_hi);
   }
   return r;
 }
 
-// Per-wave LDS stage of one 64-event chunk.
+// Per-wave LDS stage of one 64-event chunk: its in-field events in order (slot = rank among
+// the chunk's in-field events), with their weight-index base, field offset and list index.
 struct ConvWaveStage {
   ConvStageEv ev[64];
   int eb[64];
   int dp[64];
+  int q[64];
 };
 
 __host__ inline size_t conv_solve_lds_bytes(size_t wbytes, bool lds_w) {
@@ -221,37 +224,39 @@ __global__ __launch_bounds__(64 * kConvSolveWaves) void conv_solve_kernel(
   int qn = (64 + lane < P) ? ix[64 + lane] : -1;
   for (int base = 0; base < P; base += 64) {
     const ConvChunkEv cu = nx;
+    const unsigned long long m = __ballot(cu.inu);
+    const int cnt = __popcll(m);
     wave_lds_fence();  // the previous chunk's stage reads are done
-    stg.ev[lane] = nf;
-    stg.eb[lane] = cu.ebase;
-    stg.dp[lane] = cu.dpk;
+    if (cu.inu) {
+      const int slot = (int)__popcll(m & ((1ull << lane) - 1ull));
+      stg.ev[slot] = nf;
+      stg.eb[slot] = cu.ebase;
+      stg.dp[slot] = cu.dpk;
+      stg.q[slot] = base + lane;
+    }
     wave_lds_fence();
     if (base + 64 < P) {
       nx = conv_decompose(qn, g, tl, x0, y0, ux, uy);
       nf = conv_lane_event(nx.inu, fv, base + 64 + lane, zmax_hi);
       qn = (base + 128 + lane < P) ? ix[base + 128 + lane] : -1;
     }
-    unsigned long long m = __ballot(cu.inu);
-    if (!m) continue;
+    if (cnt == 0) continue;
     // the in-field events in order, each one's stage data and weight read one event ahead
-    int ql = (int)__builtin_ctzll(m);
-    m &= m - 1;
-    ConvStageEv E = stg.ev[ql];
-    int dpc = stg.dp[ql];
-    float w = weight(stg.eb[ql], dpc);
-    while (true) {
-      const bool more = m != 0;
-      int qn2 = 0, dpn = 0;
+    ConvStageEv E = stg.ev[0];
+    int dpc = stg.dp[0], qc = stg.q[0];
+    float w = weight(stg.eb[0], dpc);
+    for (int k = 0; k < cnt; ++k) {
+      const bool more = k + 1 < cnt;
       ConvStageEv En{};
+      int dpn = 0, qnx = 0;
       float wn = 0.f;
       if (more) {
-        qn2 = (int)__builtin_ctzll(m);
-        m &= m - 1;
-        En = stg.ev[qn2];
-        dpn = stg.dp[qn2];
-        wn = weight(stg.eb[qn2], dpn);
+        En = stg.ev[k + 1];
+        dpn = stg.dp[k + 1];
+        qnx = stg.q[k + 1];
+        wn = weight(stg.eb[k + 1], dpn);
       }
-      const int q = base + ql;
+      const int q = qc;
       bool own = active;
       if (!kSingle)
         own = own & ((unsigned)((dpc >> 16) - ox) < (unsigned)g.fx) &
@@ -278,8 +283,8 @@ __global__ __launch_bounds__(64 * kConvSolveWaves) void conv_solve_kernel(
         }
       }
       const double w64 = (double)w;
-      a = fma(w64, E.es, a);
-      bb = fma(w64, E.e, bb);
+      a = fma(w64, (double)E.es, a);
+      bb = fma(w64, (double)E.e, bb);
       if (kSingle) {
         zq_last = E.zq;
         last_q = q;
@@ -289,10 +294,9 @@ __global__ __launch_bounds__(64 * kConvSolveWaves) void conv_solve_kernel(
         last_q = q;
         seen = true;
       }
-      if (!more) break;
-      ql = qn2;
       E = En;
       dpc = dpn;
+      qc = qnx;
       w = wn;
     }
   }

@@ -905,13 +905,15 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
   double acc[kST];
 #pragma unroll
   for (int u = 0; u < kST; ++u) acc[u] = 0.0;
+  // the lane's next pre spike is requested while the current one is processed
+  float tpn = tp0, esn = es0, en = e0;
   for (int m = 0; m < np; ++m) {
-    float tp = tp0;
-    double es = (double)es0, e = (double)e0;
-    if (m > 0) {
-      tp = pre_t[pbase + m];
-      es = (double)pre_es[pbase + m];
-      e = (double)pre_e[pbase + m];
+    const float tp = tpn;
+    const double es = (double)esn, e = (double)en;
+    if (m + 1 < np) {
+      tpn = pre_t[pbase + m + 1];
+      esn = pre_es[pbase + m + 1];
+      en = pre_e[pbase + m + 1];
     }
     double pacc = 0.0;
 #pragma unroll

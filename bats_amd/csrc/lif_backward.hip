@@ -687,21 +687,23 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
   }
 }
 
-// Pre-spike errors with a lane per pre spike: a workgroup of kTileWaves waves per (256 pre
+// Pre-spike errors with a lane per pre spike: a workgroup of kPreSpkWaves waves per (512 pre
 // spikes, sample) walks all tile records of the sample; per tile each lane stages its 16
 // weights W[row][neuron] in the wave's LDS block, and every entry adds
 //   w[row] * (es' d1 - e' d2)  if t_pre < t_post   (float32 in the tile, float64 across).
 // pre_errors must be zero-filled beforehand (slots without a spike keep 0).
-__global__ __launch_bounds__(64 * kTileWaves) void pre_err_spk_kernel(
+constexpr int kPreSpkWaves = 8;  // waves (64 pre spikes each) sharing one tile record
+
+__global__ __launch_bounds__(64 * kPreSpkWaves) void pre_err_spk_kernel(
     int I, int K_pre, int J, int K, int gj, int cap, const float4 *__restrict__ spk,
     const int *__restrict__ slot, const int *__restrict__ count, const float *__restrict__ W,
     const float4 *__restrict__ rec, double inv_tau_s, double inv_tau,
     float *__restrict__ pre_errors) {
-  constexpr int kNth = 64 * kTileWaves;
-  constexpr int kPf = 2;  // record entries per thread prefetched for the next tile
+  constexpr int kNth = 64 * kPreSpkWaves;
+  constexpr int kPf = 1;  // record entries per thread prefetched for the next tile
   extern __shared__ __align__(16) char smem[];
-  float *wl = reinterpret_cast<float *>(smem);  // [kTileWaves][kWT][64]
-  float4 *rb = reinterpret_cast<float4 *>(wl + kTileWaves * kWT * 64);
+  float *wl = reinterpret_cast<float *>(smem);  // [kPreSpkWaves][kWT][64]
+  float4 *rb = reinterpret_cast<float4 *>(wl + kPreSpkWaves * kWT * 64);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int b = blockIdx.y;
@@ -1161,12 +1163,12 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
                        st, pre_errors, nfill, 0.f);
     if (int r = check_launch("fill pre errors")) return r;
     const int gj = (J + kWT - 1) / kWT;
-    const size_t lds = (size_t)kTileWaves * kWT * 64 * sizeof(float) +
+    const size_t lds = (size_t)kPreSpkWaves * kWT * 64 * sizeof(float) +
                        (size_t)tile_rec_slots(K_post) * sizeof(float4);
     if (int r = allow_lds((const void *)pre_err_spk_kernel, lds)) return r;
-    const int groups = (cap + 64 * kTileWaves - 1) / (64 * kTileWaves);
+    const int groups = (cap + 64 * kPreSpkWaves - 1) / (64 * kPreSpkWaves);
     BATS_TIMED("pre_err_spk_kernel", st,
-               hipLaunchKernelGGL(pre_err_spk_kernel, dim3(groups, B), dim3(64 * kTileWaves), lds,
+               hipLaunchKernelGGL(pre_err_spk_kernel, dim3(groups, B), dim3(64 * kPreSpkWaves), lds,
                                   st, I, K_pre, J, K_post, gj, cap, spk, spk_slot, spk_count,
                                   weights, rec, 1.0 / (double)tau_s, 1.0 / (double)tau,
                                   pre_errors));

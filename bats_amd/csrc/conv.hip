@@ -352,6 +352,7 @@ __global__ __launch_bounds__(kPoolThreads) void pool_forward_kernel(
     int B, int X, int Y, int C, int K, const int *__restrict__ n, const float *__restrict__ t,
     int *__restrict__ out_n, float *__restrict__ out_t, int *__restrict__ out_src) {
   __shared__ int s_cnt[kPoolThreads][5];  // finite count, then the children's counts
+  __shared__ int s_src[kPoolThreads][4];  // the children's first slot in the input
   const int Xo = X / 2, Yo = Y / 2;
   const int No = Xo * Yo * C;
   const int N = X * Y * C;
@@ -374,7 +375,10 @@ __global__ __launch_bounds__(kPoolThreads) void pool_forward_kernel(
     out_n[gid] = cnt;
     s_cnt[threadIdx.x][0] = cnt;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) s_cnt[threadIdx.x][1 + d] = nd[d];
+    for (int d = 0; d < 4; ++d) {
+      s_cnt[threadIdx.x][1 + d] = nd[d];
+      s_src[threadIdx.x][d] = (int)base[d];
+    }
     int p[4] = {0, 0, 0, 0};
     float head[4];
 #pragma unroll
@@ -409,23 +413,29 @@ __global__ __launch_bounds__(kPoolThreads) void pool_forward_kernel(
   // the +inf tails of the block's rows, coalesced
   const int rows = (int)min<int64_t>(kPoolThreads, total - g0);
   const int slots = 4 * K;
+  // (row, slot) of the thread's element, advanced by kPoolThreads elements per step
+  int rr = threadIdx.x / slots, s = threadIdx.x - (threadIdx.x / slots) * slots;
+  const int step_r = kPoolThreads / slots, step_s = kPoolThreads - step_r * slots;
+  float *ot = out_t + (size_t)g0 * slots;
+  int *os = out_src + (size_t)g0 * slots;
   for (int e = threadIdx.x; e < rows * slots; e += kPoolThreads) {
-    const int rr = e / slots, s = e - rr * slots;
     const int cnt = s_cnt[rr][0];
-    if (s < cnt) continue;
-    const int64_t g = g0 + rr;
-    const int b = (int)(g / No), jo = (int)(g - (int64_t)b * No);
-    const int x = jo / (Yo * C), r = jo - x * (Yo * C), y = r / C, c = r - y * C;
-    int q = s - cnt;
-    int d = 0;
-    while (d < 3 && q >= K - s_cnt[rr][1 + d]) {
-      q -= K - s_cnt[rr][1 + d];
-      ++d;
+    if (s >= cnt) {
+      int q = s - cnt;
+      int d = 0;
+      while (d < 3 && q >= K - s_cnt[rr][1 + d]) {
+        q -= K - s_cnt[rr][1 + d];
+        ++d;
+      }
+      ot[e] = kInfF;
+      os[e] = s_src[rr][d] + s_cnt[rr][1 + d] + q;
     }
-    const int i = ((2 * x + (d >> 1)) * Y + (2 * y + (d & 1))) * C + c;
-    const size_t o = (size_t)g * slots + s;
-    out_t[o] = kInfF;
-    out_src[o] = (int)(((size_t)b * N + i) * K + s_cnt[rr][1 + d] + q);
+    s += step_s;
+    rr += step_r;
+    if (s >= slots) {
+      s -= slots;
+      ++rr;
+    }
   }
 }
 

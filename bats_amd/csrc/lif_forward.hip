@@ -106,6 +106,29 @@ __device__ __forceinline__ void bitonic1024_regs(uint64_t &x0, uint64_t &x1,
   }
 }
 
+// Bitonic network over the 128 keys of one wave, two per lane (keys 2l and 2l+1): every
+// compare-exchange partner is reached with a lane shuffle (no LDS, no barrier). Ascending.
+__device__ __forceinline__ void bitonic128_wave(uint64_t &x0, uint64_t &x1, int l) {
+#pragma unroll
+  for (int k = 2; k <= 128; k <<= 1) {
+    const bool up = ((2 * l) & k) == 0;
+#pragma unroll
+    for (int j = k >> 1; j >= 1; j >>= 1) {
+      if (j == 1) {
+        const uint64_t lo = x0 < x1 ? x0 : x1, hi = x0 < x1 ? x1 : x0;
+        x0 = up ? lo : hi;
+        x1 = up ? hi : lo;
+        continue;
+      }
+      const int pj = j >> 1;
+      const uint64_t y0 = __shfl_xor(x0, pj, 64), y1 = __shfl_xor(x1, pj, 64);
+      const bool keep_min = ((l & pj) == 0) == up;
+      x0 = keep_min ? (x0 < y0 ? x0 : y0) : (x0 < y0 ? y0 : x0);
+      x1 = keep_min ? (x1 < y1 ? x1 : y1) : (x1 < y1 ? y1 : x1);
+    }
+  }
+}
+
 // Merge passes over sorted runs of L0 keys (N / L0 a power of two, N >= 2 * blockDim.x): each
 // pass merges run pairs into runs of twice the length, every thread producing N / blockDim.x
 // consecutive outputs from its merge-path split (a binary search on its diagonal). Keys are
@@ -143,9 +166,10 @@ __device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0) {
 
 // One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
 // sample's Event list padded to a multiple of kEventPad, the event indices, the count, and
-// the per-slot float32 exps the backward reads. Up to 1024 events the sort runs in
-// registers (bitonic1024_regs); longer lists that fit twice in the LDS are sorted as 1024-key
-// register runs merged by merge_runs, the rest with the LDS (or global) bitonic network.
+// the per-slot float32 exps the backward reads. Up to 1024 events each wave sorts 128 keys in
+// registers (bitonic128_wave) and merge_runs merges the runs; longer lists that fit twice in
+// the LDS are sorted as 1024-key register runs (bitonic1024_regs) merged by merge_runs, the
+// rest with the LDS (or global) bitonic network.
 __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
     float tmax, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
@@ -234,13 +258,25 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     __syncthreads();
   }
   if (regs) {
-    uint64_t x0 = 2 * threadIdx.x < P ? keys[2 * threadIdx.x] : ~0ull;
-    uint64_t x1 = 2 * threadIdx.x + 1 < P ? keys[2 * threadIdx.x + 1] : ~0ull;
-    if (P > 1) bitonic1024_regs(x0, x1, keys);
-    __syncthreads();
-    keys[2 * threadIdx.x] = x0;
-    keys[2 * threadIdx.x + 1] = x1;
-    __syncthreads();
+    // 128-key runs sorted by each wave in registers, then merge-path passes through LDS
+    // (runs of 128 -> Ns, Ns = max(512, N) keys padded with ~0)
+    const int Ns = N > 512 ? N : 512;
+    const int l = threadIdx.x & 63, seg = (threadIdx.x >> 6) * 128;
+    if (P > 1) {
+      uint64_t x0 = 0, x1 = 0;
+      if (seg < Ns) {
+        x0 = seg + 2 * l < P ? keys[seg + 2 * l] : ~0ull;
+        x1 = seg + 2 * l + 1 < P ? keys[seg + 2 * l + 1] : ~0ull;
+        bitonic128_wave(x0, x1, l);
+      }
+      __syncthreads();
+      if (seg < Ns) {
+        keys[seg + 2 * l] = x0;
+        keys[seg + 2 * l + 1] = x1;
+      }
+      __syncthreads();
+      keys = merge_runs(keys, keys + 1024, Ns, 128);
+    }
   } else if (!use_global && 2 * N <= lds_cap) {
     // 1024-key runs sorted in registers, then merge-path passes between two LDS buffers
     for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
@@ -554,7 +590,7 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   uint64_t *gkeys = reinterpret_cast<uint64_t *>(base + off);
   out->Pcap = Pcap;
   if (S > 0) {
-    const int lds_entries = std::max(1024, std::min(N, cap));  // the register sort's exchange
+    const int lds_entries = std::max(2048, std::min(N, cap));  // the short-list merge buffers
     BATS_TIMED("sort_events_kernel", st,
                hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
                                   sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,

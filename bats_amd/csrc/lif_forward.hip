@@ -214,11 +214,28 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
+        // the four components' ballots share one LDS atomic per wave
         const int s = 4 * (s0 + u * nth + threadIdx.x);
-        take(tv[u].x, s);
-        take(tv[u].y, s + 1);
-        take(tv[u].z, s + 2);
-        take(tv[u].w, s + 3);
+        const float4 v = tv[u];
+        const bool f0 = isfinite(v.x), f1 = isfinite(v.y), f2 = isfinite(v.z),
+                   f3 = isfinite(v.w);
+        const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1), m2 = __ballot(f2),
+                                 m3 = __ballot(f3);
+        const int c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2), c3 = __popcll(m3);
+        const int tot = c0 + c1 + c2 + c3;
+        if (tot == 0) continue;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_fill, tot);
+        base = __shfl(base, 0, 64);
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        int pos = base + (int)__popcll(m0 & lt);
+        if (f0 && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(v.x) << 32) | (uint32_t)s;
+        pos = base + c0 + (int)__popcll(m1 & lt);
+        if (f1 && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(v.y) << 32) | (uint32_t)(s + 1);
+        pos = base + c0 + c1 + (int)__popcll(m2 & lt);
+        if (f2 && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(v.z) << 32) | (uint32_t)(s + 2);
+        pos = base + c0 + c1 + c2 + (int)__popcll(m3 & lt);
+        if (f3 && pos < lds_cap) lkeys[pos] = ((uint64_t)float_key(v.w) << 32) | (uint32_t)(s + 3);
       }
     }
   } else {

@@ -150,17 +150,12 @@ __device__ __forceinline__ double tau_log(double tau, double z) { return tau * l
 // far inside kZBand; the event record is read only for a root inside the band. After an
 // accepted spike the clamp filter is applied to (z, t_hi] with the reset b, so that the
 // closed form runs again only when another root can be accepted.
-// The interval's end times (t_lo, t_hi) are read from `ends` only for a root inside the band:
-// EventEnds reads them from the event record, TimeEnds holds them.
+// The interval's end times (t_lo, t_hi) are read from `ends` (lo(), hi()) only for a root
+// inside the band: EventEnds reads them from the event record of the dense event walk.
 struct EventEnds {
   const Event *ev;
   __device__ float lo() const { return ev->t; }
   __device__ float hi() const { return ev->tl; }
-};
-struct TimeEnds {
-  float t_lo, t_hi;
-  __device__ float lo() const { return t_lo; }
-  __device__ float hi() const { return t_hi; }
 };
 
 template <class Ends>
@@ -238,18 +233,6 @@ __device__ __forceinline__ void finish_spike_times(size_t row, int n, double tau
     out_t[row + k] = (float)(tau * log(z));
     out_z[row + k] = (float)z;
   }
-}
-
-// The same filter with the interval given explicitly: lower end zq = exp(t_last/tau), upper
-// end zl = exp(min(t_next, t_max)/tau) (any value above 1e300 stands for "no upper end").
-__device__ __forceinline__ bool crossing_filter3(double a, double b, double zq, double zl,
-                                                 double c, double c4) {
-  zl = fmin(zl, 1e300);
-  const double zl1 = zl * (1.0 + kFilterEps);
-  const double g = fma(-b, zl1, a + c * zl * zl);
-  const double dd = fma(b, b * (1.0 + kFilterEps), -(c4 * a));
-  const double blo = 2.0 * c * zq * (1.0 - kFilterEps), bhi = 2.0 * c * zl1;
-  return (a > 0.0) & (b > 0.0) & ((g <= 0.0) | ((b >= blo) & (b <= bhi) & (dd >= 0.0)));
 }
 
 }  // namespace bats

@@ -97,6 +97,37 @@ def test_conv_forward_geometries(case):
     assert ref.n.sum() > 10
 
 
+def test_conv_backward_tied_times():
+    """Conv backward with pre and post spikes on one 1/32 grid (many coincide): the weight
+    gradient and the pre-spike errors count a pre spike only strictly before a post spike,
+    and the time-sorted post-spike lists keep ties in channel order."""
+    from bats_amd import ops
+    from bats_amd._constants import lif_constants
+    rng = np.random.default_rng(4)
+    pre_shape = (12, 10, 2)
+    W = rng.normal(0.5, 1.0, size=(3, 4, 5, 2)).astype(F32)
+    t = _inputs(rng, 3, pre_shape, 3)
+    t = np.where(np.isfinite(t), np.round(t.astype(np.float64) * 32) / 32, np.inf).astype(F32)
+    t = np.sort(t, axis=2)
+    params, K, tmax = (0.05, 0.15, 0.1), 5, 0.6
+    ref = oconv.conv_forward(t, W, pre_shape, *params, K, tmax=tmax)
+    ref.t = np.where(np.isfinite(ref.t), np.round(ref.t.astype(np.float64) * 32) / 32,
+                     np.inf).astype(F32)
+    assert ref.n.sum() > 10
+    assert np.isin(ref.t[np.isfinite(ref.t)], t[np.isfinite(t)]).mean() > 0.5
+    err = rng.normal(size=ref.t.shape).astype(F32)
+    e_ref, g_ref, p_ref = oconv.conv_backward(ref, err, pre_trainable=True)
+    consts = lif_constants(*params)
+    err_d = _T(err)
+    g, p = ops.conv_lif_backward(_T(ref.n), _T(ref.t), _T(ref.a), _T(ref.x), _T(ref.z), err_d,
+                                 _T(t), pre_shape, _T(ref.pre_exp_s), _T(ref.pre_exp), _T(W),
+                                 *consts, per_sample=True, pre_trainable=True)
+    torch.cuda.synchronize()
+    assert_close_rel(err_d.cpu().numpy(), e_ref, 1e-4, 1e-6, "conv reset-chain errors (ties)")
+    assert_close_rel(g.cpu().numpy(), g_ref, 1e-4, 1e-6, "conv per-sample gradient (ties)")
+    assert_close_rel(p.cpu().numpy(), p_ref, 1e-4, 1e-6, "conv pre-spike errors (ties)")
+
+
 def test_pooling_forward_backward():
     from bats_amd import ops
     rng = np.random.default_rng(1)

@@ -201,6 +201,34 @@ def test_backward_hidden_to_hidden_c3_shape():
     check_backward(h2, rng.normal(size=h2.t.shape).astype(F32), "hidden to hidden")
 
 
+def _on_grid(t, step):
+    """Spike times moved onto a grid (ascending order per row kept; +inf stays +inf)."""
+    return np.where(np.isfinite(t), np.round(np.asarray(t, np.float64) / step) * step,
+                    np.inf).astype(F32)
+
+
+def test_tied_spike_times():
+    """Many equal times. Latency inputs on a 1/32 grid tie across inputs (the event order
+    must then follow the slot order, as the oracle's stable sort does); the hidden layer's
+    spikes moved onto a 1/64 grid feed a second layer (multi-spike ties, the merge sort path);
+    and the second layer's own spike times moved onto the same grid before its backward, so
+    that pre and post spikes coincide: a pre spike counts only strictly before a post spike."""
+    rng = np.random.default_rng(12)
+    t, _ = latency_inputs(rng, 8, 784)
+    t = _on_grid(t, 1.0 / 32)
+    W1 = (2.0 * rng.normal(0, 1.0, size=(800, 784))).astype(F32)
+    h, _ = check_forward(t, W1, 0.130, 1.0, 1.0, 30, 1.0, "tied latency inputs")
+    assert h.n.sum() > 1000
+    ht = _on_grid(h.t, 1.0 / 64)
+    W2 = (1.5 * rng.normal(0, 1.0, size=(800, 800))).astype(F32)
+    h2, _ = check_forward(ht, W2, 0.130, 1.0, 1.0, 30, 1.0, "tied multi-spike inputs")
+    h2.t = _on_grid(h2.t, 1.0 / 64)
+    pre = np.isfinite(h2.pre_times)
+    post = h2.t[np.isfinite(h2.t)]
+    assert np.isin(post, h2.pre_times[pre]).mean() > 0.5  # most post spikes tie with a pre spike
+    check_backward(h2, rng.normal(size=h2.t.shape).astype(F32), "tied pre and post spikes")
+
+
 @pytest.fixture
 def solver_mode():
     from bats_amd import _lib

@@ -70,6 +70,9 @@ class _GraphBatch:
         self.stream = stream
         self.h_idx = torch.empty((B, P), dtype=torch.int32, pin_memory=True)
         self.h_t = torch.empty((B, P), dtype=torch.float32, pin_memory=True)
+        self.h_idx.fill_(-1)
+        self.h_t.fill_(float("inf"))
+        self.row_len = np.zeros(B, dtype=np.int64)  # events written in each host row
         self.idx = torch.empty((B, P), dtype=torch.int32, device=dev)
         self.t = torch.empty((B, P), dtype=torch.float32, device=dev)
         self.idx.fill_(-1)
@@ -111,7 +114,7 @@ class InferenceServer:
     def __init__(self, layers: Sequence[LayerSpec], n_inputs: int, max_batch: int = 1024,
                  max_simulation: float = 1.0, memory_budget: Optional[int] = None,
                  device=None, min_batch_bucket: int = 8, min_events_bucket: int = 64,
-                 min_spikes_bucket: int = 1):
+                 min_spikes_bucket: int = 1, n_streams: int = 2):
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
         self.n_inputs = int(n_inputs)
@@ -125,7 +128,9 @@ class InferenceServer:
         self.w_dev = [ops.as_device(L.weights, torch.float32, self.device) for L in self.layers]
         self.memory_budget = memory_budget
         self.memory_in_use = 0
-        self.streams = [torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device)]
+        # batches in flight at once (one per stream): with two, a batch is assembled and
+        # launched while the previous one computes
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(n_streams)))]
         self.graphs: Dict[Tuple[int, int, int], _GraphBatch] = {}
         self.queue: collections.deque = collections.deque()
         self.inflight: List[Tuple[int, _GraphBatch]] = []
@@ -173,11 +178,17 @@ class InferenceServer:
         Kb = _bucket(max(max(r.multiplicity for r in batch), 1), self.min_spikes_bucket)
         g = self._graph_for(Bb, Pb, Kb, stream)
         hi, ht = g.h_idx.numpy(), g.h_t.numpy()
-        hi.fill(-1)
-        ht.fill(np.inf)
-        for s, r in enumerate(batch):
-            hi[s, :len(r.indices)] = r.indices
-            ht[s, :len(r.times)] = r.times
+        # only the slots written by an earlier batch are reset (padding is index -1, time +inf)
+        for s in range(Bb):
+            L = len(batch[s].indices) if s < len(batch) else 0
+            prev = int(g.row_len[s])
+            if s < len(batch):
+                hi[s, :L] = batch[s].indices
+                ht[s, :L] = batch[s].times
+            if prev > L:
+                hi[s, L:prev] = -1
+                ht[s, L:prev] = np.inf
+            g.row_len[s] = L
         with torch.cuda.stream(stream):
             g.idx.copy_(g.h_idx, non_blocking=True)
             g.t.copy_(g.h_t, non_blocking=True)

@@ -49,6 +49,7 @@ SIGNATURES = {
     "bats_sgd_step": (_i, [_vp, _i64, _vp, _vp, _vp, _f]),
     "bats_add_deltas": (_i, [_vp, _i64, _vp, _vp]),
     "bats_events_to_per_neuron": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "bats_events_csr_to_per_neuron": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bats_latency_encode": (_i, [_vp, _i64, _vp, _i, _d, _d, _vp, _vp]),
     "bats_timing_enable": (None, [_i]),
     "bats_timing_reset": (None, []),

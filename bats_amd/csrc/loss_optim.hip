@@ -177,6 +177,7 @@ __global__ __launch_bounds__(kThreads) void add_kernel(int64_t n, float *__restr
 // bats/CudaKernels/CuFiles/Inference/convert_to_spike_per_neuron.cu:9-24. One thread per
 // sample keeps the reference's sequential per-sample order (the list is time-ordered, so
 // each neuron's buffer comes out ascending); the +inf fill is a separate coalesced pass.
+// Used only for layers wider than the LDS counters of the wave form below.
 __global__ __launch_bounds__(kThreads) void events_to_neuron_kernel(
     int B, int P, int N, int K, const int *__restrict__ idx, const float *__restrict__ times,
     float *__restrict__ out_t, int *__restrict__ out_n) {
@@ -191,6 +192,72 @@ __global__ __launch_bounds__(kThreads) void events_to_neuron_kernel(
     const int c = cnt[i];
     if (c < K) ot[(size_t)i * K + c] = times[(size_t)b * P + p];
     cnt[i] = c + 1;
+  }
+}
+
+// Largest neuron count whose per-neuron counters fit the LDS of one wave's workgroup.
+constexpr int kEventsLdsNeurons = 16384;
+
+// The same conversion with one wave per sample and the sample's counters in LDS. The wave
+// reads its list 64 events at a time. A lane's slot is the neuron's count before the chunk
+// plus the number of lower lanes of the chunk holding the same neuron (a readlane sweep), and
+// the highest such lane stores the new count, so the c-th slot of a neuron holds its c-th
+// event in list order, as in the sequential loop. A negative index ends the sample's list;
+// indices >= N are skipped. The wave first fills its sample's [N,K] block with +inf (the
+// barrier's fence orders those stores before the scattered ones), and it writes every count
+// of the sample plus, when row_max is given, the sample's largest count (events dropped
+// beyond K included). Rows are [b*P, b*P + P) of a dense [B,P] list, or
+// [row_off[b], row_off[b+1]) of a packed list when row_off is given.
+__global__ __launch_bounds__(64) void events_to_neuron_wave_kernel(
+    int N, int K, int64_t P, const int *__restrict__ row_off, const int *__restrict__ idx,
+    const float *__restrict__ times, float *__restrict__ out_t, int *__restrict__ out_n,
+    int *__restrict__ row_max) {
+  extern __shared__ int cnt[];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t start = row_off ? (int64_t)row_off[b] : (int64_t)b * P;
+  const int64_t end = row_off ? (int64_t)row_off[b + 1] : start + P;
+  float *ot = out_t + (size_t)b * N * K;
+  const int64_t nk = (int64_t)N * K;
+  for (int64_t q = lane; q < nk; q += 64) ot[q] = kInfF;
+  for (int i = lane; i < N; i += 64) cnt[i] = 0;
+  __syncthreads();
+  for (int64_t p0 = start; p0 < end; p0 += 64) {
+    const int64_t p = p0 + lane;
+    const int i = p < end ? idx[p] : -1;
+    const float t = p < end ? times[p] : 0.0f;
+    const unsigned long long neg = __ballot(i < 0);
+    const int n_valid = neg ? __builtin_ctzll(neg) : 64;
+    int before = 0;
+    bool last = true;
+    for (int s = 0; s < n_valid; ++s) {
+      const int o = __builtin_amdgcn_readlane(i, s);
+      if (o == i) {
+        before += s < lane ? 1 : 0;
+        last = last && !(s > lane);
+      }
+    }
+    if (lane < n_valid && i < N) {
+      const int c = cnt[i] + before;
+      if (c < K) ot[(size_t)i * K + c] = t;
+      if (last) cnt[i] = c + 1;
+    }
+    if (n_valid < 64) break;
+  }
+  __syncthreads();
+  int m = 0;
+  int *on = out_n + (size_t)b * N;
+  for (int i = lane; i < N; i += 64) {
+    const int c = cnt[i];
+    on[i] = c;
+    m = c > m ? c : m;
+  }
+  if (row_max) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const int o = __shfl_xor(m, off);
+      m = o > m ? o : m;
+    }
+    if (lane == 0) row_max[b] = m;
   }
 }
 
@@ -339,6 +406,11 @@ extern "C" int bats_events_to_per_neuron(void *stream, int B, int P, int N, int 
   BATS_REQUIRE(out_times && out_counts, "null output");
   BATS_REQUIRE(P == 0 || (indices && times), "null input");
   hipStream_t st = as_stream(stream);
+  if (N <= kEventsLdsNeurons) {
+    hipLaunchKernelGGL(events_to_neuron_wave_kernel, dim3(B), dim3(64), sizeof(int) * N, st, N, K,
+                       (int64_t)P, nullptr, indices, times, out_times, out_counts, nullptr);
+    return check_launch("events_to_neuron_wave_kernel");
+  }
   const int64_t nt = (int64_t)B * N * K, nc = (int64_t)B * N;
   if (nt > 0) {
     hipLaunchKernelGGL(fill_kernel<float>, dim3(std::min<unsigned>(grid_for(nt), 4096)),
@@ -354,4 +426,18 @@ extern "C" int bats_events_to_per_neuron(void *stream, int B, int P, int N, int 
     if (int r = check_launch("events_to_neuron_kernel")) return r;
   }
   return 0;
+}
+
+extern "C" int bats_events_csr_to_per_neuron(void *stream, int B, int N, int K,
+                                             const int *row_offsets, const int *indices,
+                                             const float *times, float *out_times,
+                                             int *out_counts, int *out_row_max) {
+  BATS_REQUIRE(B >= 0 && N >= 0 && K >= 0, "bad sizes");
+  if (B == 0) return 0;
+  BATS_REQUIRE(N > 0 && N <= kEventsLdsNeurons, "N must be in [1, 16384]");
+  BATS_REQUIRE(row_offsets && indices && times && out_times && out_counts, "null pointer");
+  hipLaunchKernelGGL(events_to_neuron_wave_kernel, dim3(B), dim3(64), sizeof(int) * N,
+                     as_stream(stream), N, K, (int64_t)0, row_offsets, indices, times, out_times,
+                     out_counts, out_row_max);
+  return check_launch("events_to_neuron_wave_kernel");
 }

@@ -240,6 +240,35 @@ def events_to_per_neuron(indices, times, n_neurons: int, max_per_neuron: Optiona
     return out_t, out_n
 
 
+def events_csr_to_per_neuron(row_offsets, indices, times, n_neurons: int, max_per_neuron: int,
+                             out=None):
+    """Packed event lists -> per-neuron buffers: sample b's time-ordered events are
+    ``indices/times[row_offsets[b]:row_offsets[b+1]]`` (same result as
+    :func:`events_to_per_neuron` on the padded [B,P] form; reference:
+    bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33).
+    Returns (times [B,N,K], counts [B,N], row_max [B]); row_max is each sample's largest
+    per-neuron count, events dropped beyond K included. ``out`` may hold those three tensors
+    (written in place, for graph capture)."""
+    _require_gpu()
+    off = as_device(row_offsets, I32)
+    idx = as_device(indices, I32, off.device)
+    tim = as_device(times, F32, off.device)
+    B = int(off.shape[0]) - 1
+    N, K = int(n_neurons), int(max_per_neuron)
+    if out is None:
+        out = (torch.empty((B, N, K), dtype=F32, device=off.device),
+               torch.empty((B, N), dtype=I32, device=off.device),
+               torch.empty((B,), dtype=I32, device=off.device))
+    out_t, out_n, row_max = out
+    assert tuple(out_t.shape) == (B, N, K) and tuple(out_n.shape) == (B, N)
+    assert tuple(row_max.shape) == (B,)
+    lib = _lib.load()
+    _lib.check(lib.bats_events_csr_to_per_neuron(_stream(), B, N, K, _ptr(off), _ptr(idx),
+                                                 _ptr(tim), _ptr(out_t), _ptr(out_n),
+                                                 _ptr(row_max)))
+    return out_t, out_n, row_max
+
+
 # ---------------------------------------------------------------- input encoding
 def latency_encode(pixels, time_window: float = 100e-3, max_value: float = 255.0):
     """Latency code of a batch of images on the device (experiments/mnist/Dataset.py:66-71):

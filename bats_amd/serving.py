@@ -7,15 +7,21 @@ while one batch computes, the next is assembled from whatever requests have arri
 admitted, and launched as soon as a stream is free. A request is admitted when a batch slot is
 free and its device footprint fits the memory budget; it is released when its result has been
 copied out. Each batch replays a HIP graph captured for its padded shape (batch bucket,
-events-per-request bucket, spikes-per-neuron bucket), so the host cost per batch is a few
-copies and one graph launch.
+events-per-request bucket, spikes-per-neuron bucket), so the host cost per batch is one packed
+copy of the batch's event lists and one graph launch.
+
+The host does no per-event work: a batch's event lists are concatenated into one packed list
+with row offsets, and the device converts it to per-neuron buffers. The spikes-per-input-neuron
+bucket is the server's minimum bucket unless a request is known to need more; the device
+reports every sample's largest per-neuron count, and a request whose events did not all fit
+is served again in a batch with a bucket that holds them (its first result is discarded).
 
 The inference of a request is independent of the other requests of its batch (each sample is
 sorted and solved on its own), so a request's spike trains are the same alone and in any mix.
 
 Reference counterparts: bats/Network.py:32-36 (forward), bats/Losses/SpikeCountLoss.py:17-18
 (predict), bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33 (event
-list to per-neuron buffers). The reference has no server; this is new (SURVEY.md §2.1).
+list to per-neuron buffers). The reference has no server; this is new.
 """
 from __future__ import annotations
 
@@ -38,7 +44,8 @@ class Request:
     indices: np.ndarray  # int32 [P], sorted by time
     times: np.ndarray    # float32 [P], ascending
     arrival: float       # host clock (seconds)
-    multiplicity: int = 0
+    multiplicity: int = 0  # largest per-neuron count, when known (0: not known)
+    held_bytes: int = 0
     done: float = 0.0
     pred: int = -1
     counts: Optional[np.ndarray] = None
@@ -62,21 +69,23 @@ def _bucket(v: int, minimum: int) -> int:
 
 
 class _GraphBatch:
-    """Static buffers + captured graph for one padded batch shape."""
+    """Static buffers + captured graph for one padded batch shape: room for B requests of up
+    to P events each (packed back to back), K spikes per input neuron."""
 
     def __init__(self, server: "InferenceServer", B: int, P: int, K_in: int, stream):
         self.B, self.P, self.K_in = B, P, K_in
         dev = server.device
         self.stream = stream
-        self.h_idx = torch.empty((B, P), dtype=torch.int32, pin_memory=True)
-        self.h_t = torch.empty((B, P), dtype=torch.float32, pin_memory=True)
-        self.h_idx.fill_(-1)
-        self.h_t.fill_(float("inf"))
-        self.row_len = np.zeros(B, dtype=np.int64)  # events written in each host row
-        self.idx = torch.empty((B, P), dtype=torch.int32, device=dev)
-        self.t = torch.empty((B, P), dtype=torch.float32, device=dev)
-        self.idx.fill_(-1)
-        self.t.fill_(float("inf"))
+        self.h_off = torch.zeros((B + 1,), dtype=torch.int32, pin_memory=True)
+        self.h_idx = torch.full((B * P,), -1, dtype=torch.int32, pin_memory=True)
+        self.h_t = torch.full((B * P,), float("inf"), dtype=torch.float32, pin_memory=True)
+        self.off = torch.zeros((B + 1,), dtype=torch.int32, device=dev)
+        self.idx = torch.full((B * P,), -1, dtype=torch.int32, device=dev)
+        self.t = torch.full((B * P,), float("inf"), dtype=torch.float32, device=dev)
+        N = server.n_inputs
+        self.per = (torch.empty((B, N, K_in), dtype=torch.float32, device=dev),
+                    torch.empty((B, N), dtype=torch.int32, device=dev),
+                    torch.empty((B,), dtype=torch.int32, device=dev))
         n_out = server.layers[-1].weights.shape[0]
         self.zero_targets = torch.zeros((B, n_out), dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
@@ -89,11 +98,14 @@ class _GraphBatch:
         self.h_n = torch.empty(self.outputs[0].shape, dtype=torch.int32, pin_memory=True)
         self.h_st = torch.empty(self.outputs[1].shape, dtype=torch.float32, pin_memory=True)
         self.h_pred = torch.empty(self.outputs[2].shape, dtype=torch.int64, pin_memory=True)
+        self.h_rowmax = torch.empty((B,), dtype=torch.int32, pin_memory=True)
         self.done = torch.cuda.Event()
         self.requests: List[Request] = []
 
     def _run(self, server):
-        per_t, _ = ops.events_to_per_neuron(self.idx, self.t, server.n_inputs, self.K_in)
+        per_t, _, row_max = ops.events_csr_to_per_neuron(self.off, self.idx, self.t,
+                                                         server.n_inputs, self.K_in,
+                                                         out=self.per)
         x = per_t
         n = None
         for L, w in zip(server.layers, server.w_dev):
@@ -101,7 +113,7 @@ class _GraphBatch:
                                        server.max_simulation, L.max_n_spike)
         _, _, pred = ops.count_loss(n, x.shape[2], targets=self.zero_targets, want_loss=False,
                                     want_errors=False, want_pred=True)
-        return n, x, pred
+        return n, x, pred, row_max
 
 
 class InferenceServer:
@@ -114,7 +126,7 @@ class InferenceServer:
     def __init__(self, layers: Sequence[LayerSpec], n_inputs: int, max_batch: int = 1024,
                  max_simulation: float = 1.0, memory_budget: Optional[int] = None,
                  device=None, min_batch_bucket: int = 8, min_events_bucket: int = 64,
-                 min_spikes_bucket: int = 1, n_streams: int = 2):
+                 min_spikes_bucket: int = 8, n_streams: int = 2):
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
         self.n_inputs = int(n_inputs)
@@ -122,7 +134,9 @@ class InferenceServer:
         self.max_simulation = float(max_simulation)
         self.min_batch_bucket = int(min_batch_bucket)
         # larger minimum buckets mean fewer distinct padded shapes (graphs) at the cost of
-        # padding work; a server with a known request mix sets them to its upper ends
+        # padding work; a server with a known request mix sets them to its upper ends. A
+        # request with more spikes on one input neuron than the spikes bucket is served twice
+        # (the second time in a bucket that holds them).
         self.min_events_bucket = int(min_events_bucket)
         self.min_spikes_bucket = int(min_spikes_bucket)
         self.w_dev = [ops.as_device(L.weights, torch.float32, self.device) for L in self.layers]
@@ -138,8 +152,8 @@ class InferenceServer:
 
     # -------------------------------------------------------------- accounting
     def request_bytes(self, r: Request) -> int:
-        """Device bytes a request holds while admitted: its rows of the event buffers, the
-        per-neuron input buffer and every layer's outputs and workspace rows."""
+        """Device bytes a request holds while admitted: its share of the packed event
+        buffers, the per-neuron input buffer and every layer's outputs and workspace rows."""
         P = _bucket(len(r.indices), self.min_events_bucket)
         K_in = _bucket(max(r.multiplicity, 1), self.min_spikes_bucket)
         b = 8 * P + 4 * self.n_inputs * K_in * 3 + 64 * self.n_inputs * K_in
@@ -148,7 +162,6 @@ class InferenceServer:
         return b
 
     def submit(self, r: Request) -> None:
-        r.multiplicity = int(np.bincount(r.indices, minlength=self.n_inputs).max()) if len(r.indices) else 0
         self.queue.append(r)
 
     # -------------------------------------------------------------- batching
@@ -157,10 +170,14 @@ class InferenceServer:
         while self.queue and len(batch) < self.max_batch:
             r = self.queue[0]
             need = self.request_bytes(r)
+            if self.memory_budget is not None and need > self.memory_budget:
+                raise ValueError(f"request {r.rid} needs {need} bytes, more than the whole "
+                                 f"memory budget ({self.memory_budget})")
             if self.memory_budget is not None and self.memory_in_use + need > self.memory_budget:
                 break
             self.queue.popleft()
             self.memory_in_use += need
+            r.held_bytes = need
             batch.append(r)
         return batch
 
@@ -174,42 +191,59 @@ class InferenceServer:
 
     def _launch(self, batch: List[Request], stream) -> _GraphBatch:
         Bb = min(_bucket(len(batch), self.min_batch_bucket), _bucket(self.max_batch, 1))
-        Pb = _bucket(max(len(r.indices) for r in batch), self.min_events_bucket)
+        lens = [len(r.indices) for r in batch]
+        Pb = _bucket(max(lens), self.min_events_bucket)
         Kb = _bucket(max(max(r.multiplicity for r in batch), 1), self.min_spikes_bucket)
         g = self._graph_for(Bb, Pb, Kb, stream)
-        hi, ht = g.h_idx.numpy(), g.h_t.numpy()
-        # only the slots written by an earlier batch are reset (padding is index -1, time +inf)
-        for s in range(Bb):
-            L = len(batch[s].indices) if s < len(batch) else 0
-            prev = int(g.row_len[s])
-            if s < len(batch):
-                hi[s, :L] = batch[s].indices
-                ht[s, :L] = batch[s].times
-            if prev > L:
-                hi[s, L:prev] = -1
-                ht[s, L:prev] = np.inf
-            g.row_len[s] = L
+        off = g.h_off.numpy()
+        off[0] = 0
+        np.cumsum(lens, out=off[1:len(batch) + 1])
+        total = int(off[len(batch)])
+        off[len(batch) + 1:] = total  # unused slots: empty lists
+        if total:
+            np.concatenate([r.indices for r in batch], out=g.h_idx.numpy()[:total])
+            np.concatenate([r.times for r in batch], out=g.h_t.numpy()[:total])
         with torch.cuda.stream(stream):
-            g.idx.copy_(g.h_idx, non_blocking=True)
-            g.t.copy_(g.h_t, non_blocking=True)
+            g.off.copy_(g.h_off, non_blocking=True)
+            if total:
+                g.idx[:total].copy_(g.h_idx[:total], non_blocking=True)
+                g.t[:total].copy_(g.h_t[:total], non_blocking=True)
             g.graph.replay()
             g.h_n.copy_(g.outputs[0], non_blocking=True)
             g.h_st.copy_(g.outputs[1], non_blocking=True)
             g.h_pred.copy_(g.outputs[2], non_blocking=True)
+            g.h_rowmax.copy_(g.outputs[3], non_blocking=True)
             g.done.record(stream)
         g.requests = batch
         return g
 
     def _retire(self, g: _GraphBatch) -> None:
         now = time.perf_counter()
-        n, st, pred = g.h_n.numpy(), g.h_st.numpy(), g.h_pred.numpy()
+        nb = len(g.requests)
+        n = g.h_n.numpy()[:nb].copy()
+        st = g.h_st.numpy()[:nb].copy()
+        pred = g.h_pred.numpy()[:nb].tolist()
+        row_max = g.h_rowmax.numpy()[:nb]
+        over = np.nonzero(row_max > g.K_in)[0]
+        redo = []
+        for s in over.tolist():
+            # the request's events did not all fit its bucket: serve it again with one that
+            # holds them (its slot's results above are discarded)
+            r = g.requests[s]
+            r.multiplicity = int(row_max[s])
+            redo.append(r)
+        skip = set(over.tolist())
         for s, r in enumerate(g.requests):
+            self.memory_in_use -= r.held_bytes
+            r.held_bytes = 0
+            if s in skip:
+                continue
             r.done = now
-            r.pred = int(pred[s])
-            r.counts = n[s].copy()
-            r.spike_times = st[s].copy()
-            self.memory_in_use -= self.request_bytes(r)
+            r.pred = pred[s]
+            r.counts = n[s]
+            r.spike_times = st[s]
             self.completed.append(r)
+        self.queue.extendleft(reversed(redo))
         g.requests = []
 
     def poll(self) -> None:

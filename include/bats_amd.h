@@ -240,6 +240,19 @@ int bats_events_to_per_neuron(void *stream, int B, int P, int N, int K,
                               const int *indices, const float *times,
                               float *out_times, int *out_counts);
 
+/*
+ * The same conversion from a packed list: sample b's events are
+ * indices/times[row_offsets[b] .. row_offsets[b+1]) (row_offsets [B+1] int32, ascending),
+ * in time order; a negative index also ends a sample's list. Replaces the same reference
+ * wrapper (bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33) for
+ * callers that hold requests of uneven lengths back to back (no [B,P] padding to build).
+ * out_row_max [B] (may be NULL) receives each sample's largest per-neuron count, dropped
+ * events included, so a caller can tell whether K held every event. 1 <= N <= 16384.
+ */
+int bats_events_csr_to_per_neuron(void *stream, int B, int N, int K, const int *row_offsets,
+                                  const int *indices, const float *times, float *out_times,
+                                  int *out_counts, int *out_row_max);
+
 /* ------------------------------------------------------------------ observability */
 
 /*

@@ -68,3 +68,31 @@ def test_memory_budget_admission():
         assert srv.memory_in_use <= budget
     assert len(srv.completed) == 30
     assert srv.memory_in_use == 0
+
+
+def test_spikes_bucket_overflow_is_served_again():
+    # with a spikes-per-input-neuron bucket of 1, every request with two events on one input
+    # neuron overflows on the device and is served again in a bucket that holds it; results
+    # equal those of a server whose bucket holds every request at once
+    from bats_amd.serving import InferenceServer, make_requests
+    layers = _layers()
+    wide = InferenceServer(layers, 784, max_batch=32, min_spikes_bucket=16)
+    for r in make_requests(48, p_min=50, p_max=900, seed=6):
+        wide.submit(r)
+    wide.drain()
+    ref = {r.rid: (r.counts, r.spike_times, r.pred) for r in wide.completed}
+    assert all(r.multiplicity == 0 for r in wide.completed)
+    tight = InferenceServer(layers, 784, max_batch=32, min_spikes_bucket=1)
+    for r in make_requests(48, p_min=50, p_max=900, seed=6):
+        tight.submit(r)
+    tight.drain()
+    assert len(tight.completed) == 48 and tight.memory_in_use == 0
+    redone = 0
+    for r in tight.completed:
+        true_mult = int(np.bincount(r.indices, minlength=784).max())
+        if true_mult > 1:
+            assert r.multiplicity == true_mult
+            redone += 1
+        n, t, p = ref[r.rid]
+        assert np.array_equal(r.counts, n) and np.array_equal(r.spike_times, t) and r.pred == p
+    assert redone > 10

@@ -567,6 +567,22 @@ static int lds_capacity_entries() {
   return cached[dev];
 }
 
+// Compute units of the current device (256 on MI355X), read once per device.
+static int cu_count() {
+  static int cached[64];
+  static std::once_flag once[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  std::call_once(once[dev], [dev]() {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cached[dev] = n;
+  });
+  return cached[dev];
+}
+
 static int event_capacity(int S) {
   return std::max(kEventPad, (S + kEventPad - 1) / kEventPad * kEventPad);
 }
@@ -679,8 +695,12 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
                                   out_n, out_t, out_a, out_x, out_z));
     return check_launch("lif_solve_wave_kernel");
   }
-  // blocks per sample, and lanes per block rounded up to whole waves
-  const int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
+  // blocks per sample, and lanes per block rounded up to whole waves; a batch with fewer
+  // samples than the device has CUs spreads each sample's neurons over more workgroups (of
+  // at least one wave), so that the launch reaches every CU
+  int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
+  const int cus = cu_count();
+  if ((long long)B * nblk < cus) nblk = std::max(nblk, std::min((cus + B - 1) / B, (J + 63) / 64));
   const int per = (J + nblk - 1) / nblk;
   const int threads = (per + 63) / 64 * 64;
   dim3 sg((J + threads - 1) / threads, B);

@@ -195,24 +195,28 @@ __global__ __launch_bounds__(kThreads) void events_to_neuron_kernel(
   }
 }
 
-// Largest neuron count whose per-neuron counters fit the LDS of one wave's workgroup.
-constexpr int kEventsLdsNeurons = 16384;
+// Largest neuron count whose per-neuron counters and lane masks fit the LDS of one wave's
+// workgroup (12 bytes per neuron).
+constexpr int kEventsLdsNeurons = 4096;
 
 // The same conversion with one wave per sample and the sample's counters in LDS. The wave
-// reads its list 64 events at a time. A lane's slot is the neuron's count before the chunk
-// plus the number of lower lanes of the chunk holding the same neuron (a readlane sweep), and
-// the highest such lane stores the new count, so the c-th slot of a neuron holds its c-th
-// event in list order, as in the sequential loop. A negative index ends the sample's list;
-// indices >= N are skipped. The wave first fills its sample's [N,K] block with +inf (the
-// barrier's fence orders those stores before the scattered ones), and it writes every count
-// of the sample plus, when row_max is given, the sample's largest count (events dropped
-// beyond K included). Rows are [b*P, b*P + P) of a dense [B,P] list, or
-// [row_off[b], row_off[b+1]) of a packed list when row_off is given.
+// reads its list 64 events at a time. Each lane sets its bit in its neuron's 64-bit lane mask
+// (an LDS atomic OR; a wave's LDS operations complete in order), so a lane's slot is the
+// neuron's count before the chunk plus the number of lower lanes in the mask, and the highest
+// lane in the mask stores the new count: the c-th slot of a neuron holds its c-th event in
+// list order, as in the sequential loop. A negative index ends the sample's list; indices
+// >= N are skipped. The wave first fills its sample's [N,K] block with +inf (the barrier's
+// fence orders those stores before the scattered ones), and it writes every count of the
+// sample plus, when row_max is given, the sample's largest count (events dropped beyond K
+// included). Rows are [b*P, b*P + P) of a dense [B,P] list, or [row_off[b], row_off[b+1])
+// of a packed list when row_off is given.
 __global__ __launch_bounds__(64) void events_to_neuron_wave_kernel(
     int N, int K, int64_t P, const int *__restrict__ row_off, const int *__restrict__ idx,
     const float *__restrict__ times, float *__restrict__ out_t, int *__restrict__ out_n,
     int *__restrict__ row_max) {
-  extern __shared__ int cnt[];
+  extern __shared__ unsigned long long lds_masks[];
+  unsigned long long *mask = lds_masks;                     // [N]
+  int *cnt = reinterpret_cast<int *>(lds_masks + N);        // [N]
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t start = row_off ? (int64_t)row_off[b] : (int64_t)b * P;
@@ -220,44 +224,50 @@ __global__ __launch_bounds__(64) void events_to_neuron_wave_kernel(
   float *ot = out_t + (size_t)b * N * K;
   const int64_t nk = (int64_t)N * K;
   for (int64_t q = lane; q < nk; q += 64) ot[q] = kInfF;
-  for (int i = lane; i < N; i += 64) cnt[i] = 0;
+  for (int i = lane; i < N; i += 64) {
+    cnt[i] = 0;
+    mask[i] = 0ull;
+  }
   __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int64_t p0 = start; p0 < end; p0 += 64) {
     const int64_t p = p0 + lane;
     const int i = p < end ? idx[p] : -1;
     const float t = p < end ? times[p] : 0.0f;
     const unsigned long long neg = __ballot(i < 0);
     const int n_valid = neg ? __builtin_ctzll(neg) : 64;
-    int before = 0;
-    bool last = true;
-    for (int s = 0; s < n_valid; ++s) {
-      const int o = __builtin_amdgcn_readlane(i, s);
-      if (o == i) {
-        before += s < lane ? 1 : 0;
-        last = last && !(s > lane);
-      }
-    }
-    if (lane < n_valid && i < N) {
-      const int c = cnt[i] + before;
+    const bool live = lane < n_valid && i < N;
+    if (live) atomicOr(&mask[i], 1ull << lane);
+    __syncthreads();
+    unsigned long long m = 0ull;
+    int c = 0;
+    if (live) {
+      m = mask[i];
+      c = cnt[i] + __popcll(m & below);
       if (c < K) ot[(size_t)i * K + c] = t;
-      if (last) cnt[i] = c + 1;
     }
+    __syncthreads();
+    if (live && (m >> lane) == 1ull) {  // highest lane of the neuron in this chunk
+      cnt[i] = c + 1;
+      mask[i] = 0ull;
+    }
+    __syncthreads();
     if (n_valid < 64) break;
   }
   __syncthreads();
-  int m = 0;
+  int mx = 0;
   int *on = out_n + (size_t)b * N;
   for (int i = lane; i < N; i += 64) {
     const int c = cnt[i];
     on[i] = c;
-    m = c > m ? c : m;
+    mx = c > mx ? c : mx;
   }
   if (row_max) {
     for (int off = 32; off > 0; off >>= 1) {
-      const int o = __shfl_xor(m, off);
-      m = o > m ? o : m;
+      const int o = __shfl_xor(mx, off);
+      mx = o > mx ? o : mx;
     }
-    if (lane == 0) row_max[b] = m;
+    if (lane == 0) row_max[b] = mx;
   }
 }
 
@@ -407,7 +417,7 @@ extern "C" int bats_events_to_per_neuron(void *stream, int B, int P, int N, int 
   BATS_REQUIRE(P == 0 || (indices && times), "null input");
   hipStream_t st = as_stream(stream);
   if (N <= kEventsLdsNeurons) {
-    hipLaunchKernelGGL(events_to_neuron_wave_kernel, dim3(B), dim3(64), sizeof(int) * N, st, N, K,
+    hipLaunchKernelGGL(events_to_neuron_wave_kernel, dim3(B), dim3(64), (size_t)12 * N, st, N, K,
                        (int64_t)P, nullptr, indices, times, out_times, out_counts, nullptr);
     return check_launch("events_to_neuron_wave_kernel");
   }
@@ -434,9 +444,9 @@ extern "C" int bats_events_csr_to_per_neuron(void *stream, int B, int N, int K,
                                              int *out_counts, int *out_row_max) {
   BATS_REQUIRE(B >= 0 && N >= 0 && K >= 0, "bad sizes");
   if (B == 0) return 0;
-  BATS_REQUIRE(N > 0 && N <= kEventsLdsNeurons, "N must be in [1, 16384]");
+  BATS_REQUIRE(N > 0 && N <= kEventsLdsNeurons, "N must be in [1, 4096]");
   BATS_REQUIRE(row_offsets && indices && times && out_times && out_counts, "null pointer");
-  hipLaunchKernelGGL(events_to_neuron_wave_kernel, dim3(B), dim3(64), sizeof(int) * N,
+  hipLaunchKernelGGL(events_to_neuron_wave_kernel, dim3(B), dim3(64), (size_t)12 * N,
                      as_stream(stream), N, K, (int64_t)0, row_offsets, indices, times, out_times,
                      out_counts, out_row_max);
   return check_launch("events_to_neuron_wave_kernel");

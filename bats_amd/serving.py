@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import gc
 import time
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -101,6 +102,26 @@ class _GraphBatch:
         self.h_rowmax = torch.empty((B,), dtype=torch.int32, pin_memory=True)
         self.done = torch.cuda.Event()
         self.requests: List[Request] = []
+        # the first use of a graph and of its host buffers in copies carries one-time set-up
+        # costs (milliseconds); one full round trip on empty lists takes them here
+        with torch.cuda.stream(stream):
+            self._round_trip()
+        stream.synchronize()
+
+    def _round_trip(self, total: Optional[int] = None) -> None:
+        """Copies the packed lists in, replays the graph, copies the results out and records
+        ``done`` (on the current stream)."""
+        total = self.B * self.P if total is None else total
+        self.off.copy_(self.h_off, non_blocking=True)
+        if total:
+            self.idx[:total].copy_(self.h_idx[:total], non_blocking=True)
+            self.t[:total].copy_(self.h_t[:total], non_blocking=True)
+        self.graph.replay()
+        self.h_n.copy_(self.outputs[0], non_blocking=True)
+        self.h_st.copy_(self.outputs[1], non_blocking=True)
+        self.h_pred.copy_(self.outputs[2], non_blocking=True)
+        self.h_rowmax.copy_(self.outputs[3], non_blocking=True)
+        self.done.record()
 
     def _run(self, server):
         per_t, _, row_max = ops.events_csr_to_per_neuron(self.off, self.idx, self.t,
@@ -204,16 +225,7 @@ class InferenceServer:
             np.concatenate([r.indices for r in batch], out=g.h_idx.numpy()[:total])
             np.concatenate([r.times for r in batch], out=g.h_t.numpy()[:total])
         with torch.cuda.stream(stream):
-            g.off.copy_(g.h_off, non_blocking=True)
-            if total:
-                g.idx[:total].copy_(g.h_idx[:total], non_blocking=True)
-                g.t[:total].copy_(g.h_t[:total], non_blocking=True)
-            g.graph.replay()
-            g.h_n.copy_(g.outputs[0], non_blocking=True)
-            g.h_st.copy_(g.outputs[1], non_blocking=True)
-            g.h_pred.copy_(g.outputs[2], non_blocking=True)
-            g.h_rowmax.copy_(g.outputs[3], non_blocking=True)
-            g.done.record(stream)
+            g._round_trip(total)
         g.requests = batch
         return g
 
@@ -283,9 +295,11 @@ def run_poisson(server: InferenceServer, requests: Sequence[Request], rate: floa
     copied to the host)."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=len(requests))
+    # earlier results are released (and their memory collected) before the clock starts
+    server.completed = []
+    gc.collect()
     t0 = time.perf_counter()
     arrivals = t0 + np.cumsum(gaps)
-    server.completed = []
     k = 0
     while k < len(requests) or server.queue or server.inflight:
         now = time.perf_counter()

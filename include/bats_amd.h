@@ -247,7 +247,7 @@ int bats_events_to_per_neuron(void *stream, int B, int P, int N, int K,
  * wrapper (bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33) for
  * callers that hold requests of uneven lengths back to back (no [B,P] padding to build).
  * out_row_max [B] (may be NULL) receives each sample's largest per-neuron count, dropped
- * events included, so a caller can tell whether K held every event. 1 <= N <= 16384.
+ * events included, so a caller can tell whether K held every event. 1 <= N <= 4096.
  */
 int bats_events_csr_to_per_neuron(void *stream, int B, int N, int K, const int *row_offsets,
                                   const int *indices, const float *times, float *out_times,

@@ -111,10 +111,28 @@ def test_packed_all_empty_and_wide_layer():
     assert np.all(np.isinf(t)) and not n.any() and not m.any()
     # a layer wider than the LDS counters takes the sequential form of the dense entry point
     rng = np.random.default_rng(5)
-    N = 20000
+    N = 6000
     rows = _random_rows(rng, 6, N, 4000, grid=1e-3)
     P = max(len(r[0]) for r in rows)
     idx, tim = _dense_from_rows(rows, P)
     ref_t, ref_n = oev.events_to_per_neuron(idx, tim, N, max_per_neuron=3)
     t, n = _dense_gpu(idx, tim, N, 3)
     assert np.array_equal(n, ref_n) and np.array_equal(t, ref_t)
+
+
+def test_long_runs_of_one_neuron():
+    # whole 64-event chunks on one neuron, the K boundary falling inside a chunk
+    N, K = 50, 150
+    rows = []
+    for b, run in enumerate((200, 64, 130, 1)):
+        ii = np.concatenate([np.full(run, 5), np.arange(N), np.full(70, 7)]).astype(np.int32)
+        tt = np.sort(np.random.default_rng(b).uniform(0, 1, size=ii.size)).astype(F32)
+        rows.append((ii, tt))
+    P = max(len(r[0]) for r in rows)
+    idx, tim = _dense_from_rows(rows, P)
+    ref_t, ref_n = oev.events_to_per_neuron(idx, tim, N, max_per_neuron=K)
+    t, n = _dense_gpu(idx, tim, N, K)
+    assert np.array_equal(n, ref_n) and np.array_equal(t, ref_t)
+    t2, n2, m2 = _packed_gpu(*_packed(rows), N, K)
+    assert np.array_equal(n2, ref_n) and np.array_equal(t2, ref_t)
+    assert np.array_equal(m2, ref_n.max(axis=1))

@@ -383,7 +383,10 @@ constexpr int kSolveMaxThreads = 1024;
 constexpr int kGroup = kEventPad;
 
 // kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every interval is solved.
-template <bool kAlwaysExact>
+// kStaged: the event records come from a wave-private LDS stage filled one 64-event chunk
+// ahead (for launches with few waves per SIMD, whose per-event scalar loads would otherwise
+// be exposed); otherwise each record is a scalar load issued one event ahead.
+template <bool kAlwaysExact, bool kStaged>
 __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
     const Event *__restrict__ events, const FastEvent *__restrict__ fast,
     const int *__restrict__ evidx,
@@ -410,41 +413,67 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
   double a = (kAlwaysExact || active) ? 0.0 : -kInfD, bb = 0.0;
   int n = 0;
   bool live = active;
-  float wn[kGroup];
+  float wn[kGroup];  // weights of the next group
   if (Ppad > 0) {
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[u] * J];
   }
-  for (int q0 = 0; q0 < Ppad; q0 += kGroup) {
-    float w[kGroup];
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u) w[u] = wn[u];
-    if (q0 + kGroup < Ppad) {
-#pragma unroll
-      for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[q0 + kGroup + u] * J];
+  // staged form: each wave keeps the records of the next 64 events in its own LDS slice,
+  // filled with one coalesced load per chunk issued a whole chunk ahead
+  __shared__ FastEvent s_stage[kStaged ? kSolveMaxThreads / 64 : 1][64];
+  FastEvent *stage = s_stage[kStaged ? (threadIdx.x >> 6) : 0];
+  const int lane = threadIdx.x & 63;
+  FastEvent reg{};
+  if (kStaged && lane < Ppad) reg = fv[lane];
+  // the record of the event being processed; the next one is always requested during the
+  // current event's arithmetic (across group boundaries too)
+  FastEvent cur{};
+  if (!kStaged && Ppad > 0) cur = fv[0];
+  for (int c0 = 0; c0 < Ppad; c0 += 64) {
+    const int cend = min(c0 + 64, Ppad);
+    if (kStaged) {
+      stage[lane] = reg;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (c0 + 64 + lane < Ppad) reg = fv[c0 + 64 + lane];
+      cur = stage[0];
     }
-    // event data one event ahead, so that its scalar loads are in flight during the
-    // previous event's arithmetic
-    FastEvent cur = fv[q0];
+    const int qend = kStaged ? cend : Ppad;
+    for (int q0 = c0; q0 < cend; q0 += kGroup) {
+      float w[kGroup];
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) {
-      FastEvent nxt;
-      if (u + 1 < kGroup) nxt = fv[q0 + u + 1];
-      const double w64 = (double)w[u];
-      a = fma(w64, cur.es, a);
-      bb = fma(w64, cur.e, bb);
-      if (kAlwaysExact ? live : clamp_filter(a, bb, cur, c, inv2c)) {
-        const SolveStateZ r = solve_crossings_deferred(a, bb, n, cur.zq_lo, cur.zl_hi,
-                                                       ev + q0 + u, c, inv2c, dtt, tau, K, row,
-                                                       out_t, out_a, out_x, out_z);
-        bb = r.b;
-        n = r.n;
-        if (kAlwaysExact)
-          live = r.live != 0;
-        else if (!r.live)
-          a = -kInfD;
+      for (int u = 0; u < kGroup; ++u) w[u] = wn[u];
+      if (q0 + kGroup < Ppad) {
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[q0 + kGroup + u] * J];
       }
-      if (u + 1 < kGroup) cur = nxt;
+      const bool more = q0 + kGroup < qend;
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        FastEvent nxt;
+        if (u + 1 < kGroup || more) nxt = kStaged ? stage[q0 - c0 + u + 1] : fv[q0 + u + 1];
+        const double w64 = (double)w[u];
+        a = fma(w64, cur.es, a);
+        bb = fma(w64, cur.e, bb);
+        if (kAlwaysExact ? live : clamp_filter(a, bb, cur, c, inv2c)) {
+          const SolveStateZ r = solve_crossings_deferred(a, bb, n, cur.zq_lo, cur.zl_hi,
+                                                         ev + q0 + u, c, inv2c, dtt, tau, K,
+                                                         row, out_t, out_a, out_x, out_z);
+          bb = r.b;
+          n = r.n;
+          if (kAlwaysExact)
+            live = r.live != 0;
+          else if (!r.live)
+            a = -kInfD;
+        }
+        if (u + 1 < kGroup || more) cur = nxt;
+      }
+    }
+    if (kStaged) {
+      // this chunk's stage reads come before the next chunk's write
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
   }
   if (active) finish_spike_times(row, n, tau, out_t, out_z);
@@ -540,7 +569,10 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
 }
 
 // ------------------------------------------------------------------ host side
-static int g_solver_mode = 0;  // 0 auto, 1 lane per neuron, 2 wave per neuron
+// 0 auto, 1 lane per neuron, 2 wave per neuron, 3 / 4 lane per neuron with staged / scalar
+// event records
+static int g_solver_mode = 0;
+static int g_solver_blocks = 0;  // workgroups per sample of the lane solver, 0 auto
 
 // Lane-per-neuron keeps the event walk sequential in each lane and needs many (sample,
 // neuron) pairs to fill 256 CUs; below this many pairs the wave-per-neuron form is used.
@@ -701,10 +733,19 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
   const int cus = cu_count();
   if ((long long)B * nblk < cus) nblk = std::max(nblk, std::min((cus + B - 1) / B, (J + 63) / 64));
+  if (g_solver_blocks > 0) nblk = std::max(1, std::min(g_solver_blocks, (J + 63) / 64));
   const int per = (J + nblk - 1) / nblk;
   const int threads = (per + 63) / 64 * 64;
   dim3 sg((J + threads - 1) / threads, B);
-  const auto seq = always_exact ? lif_solve_seq_kernel<true> : lif_solve_seq_kernel<false>;
+  // staged event records when the launch has fewer than 3 waves per SIMD (4 SIMDs per CU):
+  // with more, the other waves hide the per-event scalar loads and staging costs a little
+  const long long waves = (long long)B * sg.x * (threads / 64);
+  const bool staged = g_solver_mode == 3 ||
+                      (g_solver_mode != 4 && waves < 3LL * 4 * (long long)cus);
+  const auto seq = always_exact ? (staged ? lif_solve_seq_kernel<true, true>
+                                          : lif_solve_seq_kernel<true, false>)
+                                : (staged ? lif_solve_seq_kernel<false, true>
+                                          : lif_solve_seq_kernel<false, false>);
   BATS_TIMED("lif_solve_seq_kernel", st,
              hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
                                 el.counts, el.Pcap, WT, J, K_post, (double)c,
@@ -713,8 +754,16 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   return check_launch("lif_solve_seq_kernel");
 }
 
+extern "C" int bats_set_solver_blocks(int per_sample) {
+  BATS_REQUIRE(per_sample >= 0, "per_sample must be >= 0 (0: automatic)");
+  g_solver_blocks = per_sample;
+  return 0;
+}
+
 extern "C" int bats_set_solver_mode(int mode) {
-  BATS_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0 (auto), 1 (lane per neuron) or 2 (wave per neuron)");
+  BATS_REQUIRE(mode >= 0 && mode <= 4,
+               "mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron), 3 (lane per "
+               "neuron, staged event records) or 4 (lane per neuron, scalar event records)");
   g_solver_mode = mode;
   return 0;
 }

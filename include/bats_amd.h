@@ -63,10 +63,19 @@ int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
 /*
  * Solver form used by bats_lif_forward: 0 = automatic (default: one lane per (sample, post
  * neuron) when there are at least 32768 such pairs, else one wave per pair), 1 = always one
- * lane per pair, 2 = always one wave per pair. Both forms give the same spike counts; the
- * setting exists for testing and tuning. No counterpart in the reference.
+ * lane per pair, 2 = always one wave per pair, 3 / 4 = one lane per pair with the event
+ * records staged in LDS / read with scalar loads (1 chooses by the launch's waves per SIMD).
+ * All forms give the same results; the setting exists for testing and tuning. No
+ * counterpart in the reference.
  */
 int bats_set_solver_mode(int mode);
+
+/*
+ * Workgroups per sample of the lane-per-neuron solver (0, the default: one per 1024 neurons,
+ * more when the batch has fewer samples than the device has CUs). A tuning knob with no
+ * counterpart in the reference; results do not depend on it.
+ */
+int bats_set_solver_blocks(int per_sample);
 
 /* ------------------------------------------------------------------ dense LIF backward */
 

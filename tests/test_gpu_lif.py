@@ -237,9 +237,11 @@ def solver_mode():
     _lib.check(lib.bats_set_solver_mode(0))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_forward_both_solver_forms(solver_mode, mode):
-    # the lane-per-neuron and wave-per-neuron solvers on the same hidden and output shapes
+    # the lane-per-neuron (automatic, staged and scalar event records) and wave-per-neuron
+    # solvers on the same hidden and output shapes, and with theta = 0 (no threshold
+    # quadratic: every interval takes the closed form)
     solver_mode(mode)
     rng = np.random.default_rng(11)
     t, _ = latency_inputs(rng, 16, 784)
@@ -250,6 +252,8 @@ def test_forward_both_solver_forms(solver_mode, mode):
     tb, _ = multi_spike_inputs(rng, 8, 100, 5, t_max=1.0)
     W3 = np.abs(rng.normal(0, 3.0, size=(64, 100))).astype(F32)
     check_forward(tb, W3, 0.05, 0.2, 0.2, 3, 0.6, f"buffer full mode {mode}")
+    W4 = rng.normal(0.2, 1.0, size=(96, 100)).astype(F32)
+    check_forward(tb, W4, 0.05, 0.0, 0.3, 4, 0.6, f"theta 0 mode {mode}")
 
 
 # ------------------------------------------------------------------ input encoding

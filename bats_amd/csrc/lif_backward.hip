@@ -661,13 +661,27 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
       const bool valid = sidx < P;
       const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
       double *al = acc + (valid ? __float_as_int(S.w) : 0);
-      // entries four at a time (their LDS reads in one round); a pair adds only where the
-      // causal mask holds (a skipped term is an exact zero)
+      // entries four at a time (their LDS reads in one round). The record lists the tile's
+      // spikes row by row and every lane reads the same entry, so a row change is uniform:
+      // a lane sums its terms of one row in a register (a pair adds only where the causal mask
+      // holds; a skipped term is an exact zero) and adds the row's sum to acc with one
+      // no-return LDS atomic when the row ends, if it is not zero
+      double racc = 0.0;
+      int crow = __builtin_amdgcn_readfirstlane(cnt > 0 ? __float_as_int(rb[0].w) >> 9 : 0);
+      auto flush = [&]() {
+        if (racc != 0.0)
+          __hip_atomic_fetch_add(al + mul_u24((unsigned)crow, (unsigned)I), racc,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        racc = 0.0;
+      };
       auto term = [&](const float4 E) {
+        const int r = __builtin_amdgcn_readfirstlane(__float_as_int(E.w) >> 9);
+        if (r != crow) {
+          flush();
+          crow = r;
+        }
         const float q = fmaf(S.y, E.y, -(S.z * E.z));
-        if (S.x < E.x)
-          __hip_atomic_fetch_add(al + mul_u24((unsigned)__float_as_int(E.w) >> 9, (unsigned)I),
-                                 (double)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (S.x < E.x) racc += (double)q;
       };
       int k = 0;
       for (; k + 4 <= cnt; k += 4) {
@@ -678,6 +692,7 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
         term(E3);
       }
       for (; k < cnt; ++k) term(rb[k]);
+      flush();
     }
   }
   __syncthreads();

@@ -285,9 +285,20 @@ def serve_bench(args):
                       "served_per_s": round(res["samples_per_s"], 1),
                       "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
                       "requests": res["n"]})
+    # saturated throughput against the HBM roofline: algorithmic bytes of one request are its
+    # event list (index + time), the API-visible spike trains of both layers (times and
+    # counts) and its prediction; the weights count once per batch of max_batch requests
+    p_mean = float(np.mean([len(r.indices) for r in reqs]))
+    per_req = (8.0 * p_mean + 4.0 * (N_HID * K_HID + N_HID) + 4.0 * (N_OUT * K_OUT + N_OUT)
+               + 8.0)
+    weights = 4.0 * (N_IN * N_HID + N_HID * N_OUT)
+    achieved = (per_req * sat + weights * sat / args.serve_batch) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 5),
+            "bytes_per_request": round(per_req, 1), "mean_events_per_request": round(p_mean, 1)}
     return {"saturated_samples_per_s": round(sat, 1), "max_batch": args.serve_batch,
             "requests": "P ~ U{50..2000} events, idx ~ U{0..783} with replacement, t ~ U[0,1]",
-            "count_agreement_with_plain": agreement, "loads": loads}
+            "roofline": roof, "count_agreement_with_plain": agreement, "loads": loads}
 
 
 def plain_count_agreement(args, layers, reqs, srv):

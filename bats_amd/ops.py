@@ -240,6 +240,11 @@ def events_to_per_neuron(indices, times, n_neurons: int, max_per_neuron: Optiona
     return out_t, out_n
 
 
+# largest input layer of the packed conversion (its per-neuron counters and lane masks live in
+# one wave's LDS)
+PACKED_EVENTS_MAX_NEURONS = 4096
+
+
 def events_csr_to_per_neuron(row_offsets, indices, times, n_neurons: int, max_per_neuron: int,
                              out=None):
     """Packed event lists -> per-neuron buffers: sample b's time-ordered events are

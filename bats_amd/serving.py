@@ -151,6 +151,9 @@ class InferenceServer:
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
         self.n_inputs = int(n_inputs)
+        if not 1 <= self.n_inputs <= ops.PACKED_EVENTS_MAX_NEURONS:
+            raise ValueError(f"n_inputs must be in [1, {ops.PACKED_EVENTS_MAX_NEURONS}] (the "
+                             f"device conversion of packed event lists), got {self.n_inputs}")
         self.max_batch = int(max_batch)
         self.max_simulation = float(max_simulation)
         self.min_batch_bucket = int(min_batch_bucket)

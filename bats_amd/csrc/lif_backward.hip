@@ -834,7 +834,12 @@ __device__ __forceinline__ int count_le(const float *__restrict__ row, int nj, f
   return kk;
 }
 
-template <bool kPre>
+// kList (the whole sample's pre neurons in one block): the sample's pre spikes are numbered
+// by a block scan of the per-neuron counts and each thread takes spikes tid, tid + nth, ...
+// (about one each), adding its weight-gradient terms to float64 accumulators [J][I] in LDS
+// with no-return atomics; without it a thread takes every spike of its own pre neuron, and
+// the busiest pre neuron of the sample sets the block's length.
+template <bool kPre, bool kList>
 __global__ __launch_bounds__(1024) void small_bwd_kernel(
     int I, int K_pre, int J, int K, const int *__restrict__ n, const float *__restrict__ t,
     const float *__restrict__ a, const float *__restrict__ x, const float *__restrict__ z,
@@ -916,6 +921,67 @@ __global__ __launch_bounds__(1024) void small_bwd_kernel(
       s2[k] = a2;
     }
     st.ne[r] = ne;
+  }
+  if (kList) {
+    // 3'. spikes numbered by a block scan of the per-neuron counts; one spike per thread pass
+    double *accl = reinterpret_cast<double *>(smem + ((dense_stage_bytes(J, K) + 15) & ~(size_t)15));
+    int *sbase = reinterpret_cast<int *>(accl + (size_t)J * I);  // [I + 1]
+    int *swave = sbase + I + 1;                                  // [32]
+    for (int e = tid; e < J * I; e += nth) accl[e] = 0.0;
+    const int lane = tid & 63, wv = tid >> 6, nwv = nth >> 6;
+    int incl = np;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) swave[wv] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wv; ++w) woff += swave[w];
+    int total = 0;
+    for (int w = 0; w < nwv; ++w) total += swave[w];
+    if (active) sbase[i] = woff + incl - np;
+    if (tid == 0) sbase[I] = total;
+    __syncthreads();
+    for (int e = tid; e < total; e += nth) {
+      // the spike's neuron: the last i with sbase[i] <= e (neurons without spikes repeat a
+      // base, the search lands on the one whose range holds e)
+      int lo = 0, hi = I - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sbase[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+      const int si = lo, m = e - sbase[si];
+      const size_t sb = ((size_t)b * I + si) * K_pre + m;
+      const float tp = pre_t[sb];
+      const double es = (double)pre_es[sb], ee = (double)pre_e[sb];
+      float ws[kST];
+#pragma unroll
+      for (int u = 0; u < kST; ++u) ws[u] = (kPre && u < J) ? W[(size_t)u * I + si] : 0.f;
+      double pacc = 0.0;
+#pragma unroll
+      for (int u = 0; u < kST; ++u) {
+        if (u < J) {
+          const int nj = __builtin_amdgcn_readfirstlane(st.ne[u]);
+          if (nj > 0) {
+            const int kk = count_le(st.t + u * KT, nj, tp);
+            const double v1 = st.s1[(size_t)u * (K + 1) + kk];
+            const double v2 = st.s2[(size_t)u * (K + 1) + kk];
+            __hip_atomic_fetch_add(accl + (size_t)u * I + si, fma(-ee, v2, es * v1),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (kPre) pacc += (double)ws[u] * (v1 * es * inv_tau_s - v2 * ee * inv_tau);
+          }
+        }
+      }
+      if (kPre) pre_errors[sb] = (float)pacc;
+    }
+    __syncthreads();
+    for (int e = tid; e < J * I; e += nth) {
+      const int u = e / I, ii = e - u * I;
+      out[((size_t)b * J + u) * I + ii] = (float)accl[e];
+    }
+    return;
   }
   __syncthreads();
   // 3. lanes
@@ -1102,8 +1168,16 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
 
   if (L.small) {
     float *out = grad_mode == 1 ? grad : partial;
-    const size_t lds = dense_stage_bytes(J, K_post);
-    auto kern = want_pre ? small_bwd_kernel<true> : small_bwd_kernel<false>;
+    // one block per sample: number the sample's pre spikes and accumulate in LDS when the
+    // float64 accumulators fit
+    const size_t stage = dense_stage_bytes(J, K_post);
+    const size_t lds_list = ((stage + 15) & ~(size_t)15) + (size_t)J * I * sizeof(double) +
+                            (size_t)(I + 1 + 32) * sizeof(int);
+    const bool list = L.gi == 1 && lds_list <= kMaxWorkgroupLds;
+    const size_t lds = list ? lds_list : stage;
+    auto kern = list ? (want_pre ? small_bwd_kernel<true, true> : small_bwd_kernel<false, true>)
+                     : (want_pre ? small_bwd_kernel<true, false> : small_bwd_kernel<false, false>);
+    if (int r = allow_lds((const void *)kern, lds)) return r;
     BATS_TIMED("small_bwd_kernel", st,
                hipLaunchKernelGGL(kern, dim3(L.gi, B), dim3(L.lanes), lds, st, I, K_pre, J,
                                   K_post, n, t, a, x, z, errors, L.gi == 1 ? 1 : 0, (double)c,

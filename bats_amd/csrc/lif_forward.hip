@@ -507,7 +507,8 @@ __device__ __forceinline__ double wave_scan_incl(double v, int lane) {
 }
 
 __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
-    const Event *__restrict__ events, const int *__restrict__ counts, int Pcap,
+    const Event *__restrict__ events, const int *__restrict__ evidx,
+    const int *__restrict__ counts, int Pcap,
     const float *__restrict__ W, int I, int J, int BJ, int K, double c, double dtt, double tau,
     int always_exact, int *__restrict__ out_n, float *__restrict__ out_t,
     float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
@@ -518,20 +519,36 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
   const int j = wave - b * J;
   const int P = counts[b];
   const Event *ev = events + (size_t)b * Pcap;
+  const int *ix = evidx + (size_t)b * Pcap;
   const float *wrow = W + (size_t)j * I;
   const size_t row = ((size_t)b * J + j) * K;
   const double c4 = 4.0 * c;
   double carry_a = 0.0, carry_b = 0.0;
   int n = 0;
   bool live = true;
+  // loads in flight ahead of the chunk being solved: the event record and the weight of the
+  // next chunk, the input index of the one after (its weight gather needs it)
+  Event en{};
+  float wn = 0.f;
+  int inn = 0;
+  if (lane < P) {
+    en = ev[lane];
+    wn = wrow[ix[lane]];
+  }
+  if (64 + lane < P) inn = ix[64 + lane];
   for (int q0 = 0; q0 < P && live; q0 += 64) {
     const int q = q0 + lane;
     const bool valid = q < P;
-    Event e{};
+    const Event e = en;
+    const float wq = wn;
+    if (q + 64 < P) {
+      en = ev[q + 64];
+      wn = wrow[inn];
+    }
+    if (q + 128 < P) inn = ix[q + 128];
     double pa = 0.0, pb = 0.0;
     if (valid) {
-      e = ev[q];
-      const double w = (double)wrow[e.idx];
+      const double w = (double)wq;
       pa = w * e.es;
       pb = w * e.e;
     }
@@ -722,7 +739,8 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
     const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));
     BATS_TIMED("lif_solve_wave_kernel", st,
                hipLaunchKernelGGL(lif_solve_wave_kernel, dim3(blocks), dim3(kWaveThreads), 0, st,
-                                  el.events, el.counts, el.Pcap, weights, I, J, (int)BJ, K_post,
+                                  el.events, el.idx, el.counts, el.Pcap, weights, I, J, (int)BJ,
+                                  K_post,
                                   (double)c, (double)delta_theta_tau, (double)tau, always_exact,
                                   out_n, out_t, out_a, out_x, out_z));
     return check_launch("lif_solve_wave_kernel");

@@ -370,8 +370,24 @@ __global__ __launch_bounds__(64 * kTileWaves) void wgrad_tile_kernel(
         double *a = reinterpret_cast<double *>(acc_lane + __float_as_int(E.w));
         __hip_atomic_fetch_add(a, (double)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       };
-#pragma unroll 2
-      for (int k = 0; k < nw; ++k) term(win[kRecHead + k]);
+      // window entries eight (then four) at a time: their LDS reads in one round
+      int k = 0;
+      for (; k + 8 <= nw; k += 8) {
+        float4 E[8];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) E[g] = win[kRecHead + k + g];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) term(E[g]);
+      }
+      for (; k + 4 <= nw; k += 4) {
+        const float4 E0 = win[kRecHead + k], E1 = win[kRecHead + k + 1];
+        const float4 E2 = win[kRecHead + k + 2], E3 = win[kRecHead + k + 3];
+        term(E0);
+        term(E1);
+        term(E2);
+        term(E3);
+      }
+      for (; k < nw; ++k) term(win[kRecHead + k]);
       for (int k = nw; k < cnt; ++k) term(grec[kRecHead + k]);
     };
     if (kSingle) {

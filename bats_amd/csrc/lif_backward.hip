@@ -677,11 +677,11 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
       const bool valid = sidx < P;
       const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
       double *al = acc + (valid ? __float_as_int(S.w) : 0);
-      // entries four at a time (their LDS reads in one round). The record lists the tile's
-      // spikes row by row and every lane reads the same entry, so a row change is uniform:
-      // a lane sums its terms of one row in a register (a pair adds only where the causal mask
-      // holds; a skipped term is an exact zero) and adds the row's sum to acc with one
-      // no-return LDS atomic when the row ends, if it is not zero
+      // entries eight, then four at a time (their LDS reads in one round). The record lists
+      // the tile's spikes row by row and every lane reads the same entry, so a row change is
+      // uniform: a lane sums its terms of one row in a register (a pair adds only where the
+      // causal mask holds; a skipped term is an exact zero) and adds the row's sum to acc
+      // with one no-return LDS atomic when the row ends, if it is not zero
       double racc = 0.0;
       int crow = __builtin_amdgcn_readfirstlane(cnt > 0 ? __float_as_int(rb[0].w) >> 9 : 0);
       auto flush = [&]() {
@@ -700,6 +700,13 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
         if (S.x < E.x) racc += (double)q;
       };
       int k = 0;
+      for (; k + 8 <= cnt; k += 8) {
+        float4 E[8];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) E[g] = rb[k + g];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) term(E[g]);
+      }
       for (; k + 4 <= cnt; k += 4) {
         const float4 E0 = rb[k], E1 = rb[k + 1], E2 = rb[k + 2], E3 = rb[k + 3];
         term(E0);

@@ -7,7 +7,15 @@ from typing import List, Optional, Tuple
 import numpy as np
 import torch
 
+from . import ops
 from .AbstractLayer import AbstractLayer
+
+
+def _plain_add_methods():
+    """The add_deltas methods that are a plain weights += delta (batched in apply_deltas)."""
+    from .Layers.ConvLIFLayer import ConvLIFLayer
+    from .Layers.LIFLayer import LIFLayer
+    return (LIFLayer.add_deltas, ConvLIFLayer.add_deltas)
 
 
 class Network:
@@ -51,10 +59,21 @@ class Network:
         return gradient
 
     def apply_deltas(self, deltas: List[Optional[torch.Tensor]]) -> None:
+        """Adds each layer's delta to its weights (bats/Network.py:50-54). The LIF and conv
+        LIF layers' additions run in one launch; other layers use their own add_deltas."""
+        plain = _plain_add_methods()
+        ws, ds = [], []
         for layer, delta in zip(self.__layers, deltas):
             if delta is None:
                 continue
-            layer.add_deltas(delta)
+            if type(layer).add_deltas in plain and isinstance(delta, torch.Tensor) \
+                    and delta.is_cuda and delta.dtype == torch.float32:
+                ws.append(layer.weights)
+                ds.append(delta)
+            else:
+                layer.add_deltas(delta)
+        if ws:
+            ops.add_deltas_multi(ws, ds)
 
     def store(self, dir_path: Path) -> None:
         dir_path = Path(dir_path)

@@ -100,15 +100,17 @@ class AdamOptimizer(AbstractOptimizer):
             self._t_dev.add_(1)
             step_dev = self._t_dev
         b1, b2 = self._bias()
-        out = []
-        for k, g in enumerate(gradient):
-            if g is None:
-                out.append(None)
-                continue
-            g = ops.as_device(g, torch.float32, g.device if isinstance(g, torch.Tensor) else None)
-            w = None if weights is None else weights[k]
-            out.append(ops.adam_step(g, self._m[k], self._v[k], self._learning_rate,
-                                     self._beta_1, self._one_minus_beta_1, self._beta_2,
-                                     self._one_minus_beta_2, self._epsilon, b1, b2,
-                                     nadam=self._nadam, weights=w, step_dev=step_dev))
+        # every layer's update in one launch
+        keys = [k for k, g in enumerate(gradient) if g is not None]
+        gs = [ops.as_device(gradient[k], torch.float32,
+                            gradient[k].device if isinstance(gradient[k], torch.Tensor) else None)
+              for k in keys]
+        ws = None if weights is None else [weights[k] for k in keys]
+        deltas = ops.adam_step_multi(gs, [self._m[k] for k in keys], [self._v[k] for k in keys],
+                                     self._learning_rate, self._beta_1, self._one_minus_beta_1,
+                                     self._beta_2, self._one_minus_beta_2, self._epsilon, b1,
+                                     b2, nadam=self._nadam, weights=ws, step_dev=step_dev)
+        out: List[Optional[torch.Tensor]] = [None] * len(gradient)
+        for k, d in zip(keys, deltas):
+            out[k] = d
         return out

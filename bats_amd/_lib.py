@@ -49,6 +49,9 @@ SIGNATURES = {
                                 _vp, _i]),
     "bats_sgd_step": (_i, [_vp, _i64, _vp, _vp, _vp, _f]),
     "bats_add_deltas": (_i, [_vp, _i64, _vp, _vp]),
+    "bats_adam_step_multi": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f,
+                                  _d, _d, _vp, _i]),
+    "bats_add_deltas_multi": (_i, [_vp, _i, _vp, _vp, _vp]),
     "bats_events_to_per_neuron": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "bats_events_csr_to_per_neuron": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bats_latency_encode": (_i, [_vp, _i64, _vp, _i, _d, _d, _vp, _vp]),

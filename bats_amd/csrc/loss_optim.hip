@@ -155,6 +155,69 @@ __global__ __launch_bounds__(kThreads) void adam_kernel(
   }
 }
 
+// Several tensors in one launch (one optimizer step over every layer): the tensors' pointers
+// and sizes travel by value in the kernel arguments; blockIdx.y selects the tensor and the
+// blocks of a row stride over it. The element update is adam_kernel's.
+constexpr int kMultiMax = 16;
+struct MultiTensors {
+  int64_t n[kMultiMax];
+  const float *g[kMultiMax];
+  float *m[kMultiMax];
+  float *v[kMultiMax];
+  float *delta[kMultiMax];
+  float *w[kMultiMax];
+};
+
+__global__ __launch_bounds__(kThreads) void adam_multi_kernel(
+    MultiTensors T, double lr, double b1, double omb1, double b2, double omb2, double eps,
+    double bias1, double bias2, const int64_t *__restrict__ step, int nadam) {
+  if (step) {
+    __shared__ double s_bias[2];
+    if (threadIdx.x == 0) {
+      const double t = (double)*step;
+      s_bias[0] = 1.0 - pow(b1, t);
+      s_bias[1] = 1.0 - pow(b2, t);
+    }
+    __syncthreads();
+    bias1 = s_bias[0];
+    bias2 = s_bias[1];
+  }
+  const int k = blockIdx.y;
+  const int64_t n = T.n[k];
+  const float *__restrict__ g = T.g[k];
+  float *__restrict__ m = T.m[k];
+  float *__restrict__ v = T.v[k];
+  float *__restrict__ delta = T.delta[k];
+  float *__restrict__ w = T.w[k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double gi = (double)g[i];
+    const float mi = (float)(b1 * (double)m[i] + omb1 * gi);
+    const float vi = (float)(b2 * (double)v[i] + omb2 * gi * gi);
+    m[i] = mi;
+    v[i] = vi;
+    const double mh = (double)(float)((double)mi / bias1);
+    const double vh = (double)(float)((double)vi / bias2);
+    float d;
+    if (nadam)
+      d = (float)(-(lr / (sqrt(vh) + eps)) * (b1 * mh + omb1 * gi / bias1));
+    else
+      d = (float)(-(lr * mh / (sqrt(vh) + eps)));
+    if (delta) delta[i] = d;
+    if (w) w[i] = w[i] + d;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void add_multi_kernel(MultiTensors T) {
+  const int k = blockIdx.y;
+  const int64_t n = T.n[k];
+  float *__restrict__ w = T.w[k];
+  const float *__restrict__ d = T.g[k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = w[i] + d[i];
+}
+
 __global__ __launch_bounds__(kThreads) void sgd_kernel(int64_t n, const float *__restrict__ g,
                                                         float *__restrict__ delta,
                                                         float *__restrict__ w, double lr) {
@@ -387,6 +450,60 @@ extern "C" int bats_nadam_step(void *stream, int64_t n, const float *grad, float
                                float epsilon, double bias1, double bias2) {
   return adam_like(stream, n, grad, m, v, delta, weights, lr, beta_1, one_minus_beta_1, beta_2,
                    one_minus_beta_2, epsilon, bias1, bias2, 1);
+}
+
+extern "C" int bats_adam_step_multi(void *stream, int count, const int64_t *sizes,
+                                    const float *const *grads, float *const *m,
+                                    float *const *v, float *const *deltas,
+                                    float *const *weights, float lr, float beta_1,
+                                    float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                                    float epsilon, double bias1, double bias2,
+                                    const int64_t *step, int nadam) {
+  BATS_REQUIRE(count >= 0 && count <= kMultiMax, "count must be in [0, 16]");
+  if (count == 0) return 0;
+  BATS_REQUIRE(sizes && grads && m && v, "null array");
+  MultiTensors T{};
+  int64_t nmax = 0;
+  for (int k = 0; k < count; ++k) {
+    BATS_REQUIRE(sizes[k] >= 0, "negative size");
+    BATS_REQUIRE(sizes[k] == 0 || (grads[k] && m[k] && v[k] &&
+                                   ((deltas && deltas[k]) || (weights && weights[k]))),
+                 "null pointer");
+    T.n[k] = sizes[k];
+    T.g[k] = grads[k];
+    T.m[k] = m[k];
+    T.v[k] = v[k];
+    T.delta[k] = deltas ? deltas[k] : nullptr;
+    T.w[k] = weights ? weights[k] : nullptr;
+    nmax = std::max(nmax, sizes[k]);
+  }
+  if (nmax == 0) return 0;
+  hipLaunchKernelGGL(adam_multi_kernel, dim3(std::min<unsigned>(grid_for(nmax), 4096), count),
+                     dim3(kThreads), 0, as_stream(stream), T, (double)lr, (double)beta_1,
+                     (double)one_minus_beta_1, (double)beta_2, (double)one_minus_beta_2,
+                     (double)epsilon, bias1, bias2, step, nadam ? 1 : 0);
+  return check_launch("adam_multi_kernel");
+}
+
+extern "C" int bats_add_deltas_multi(void *stream, int count, const int64_t *sizes,
+                                     float *const *weights, const float *const *deltas) {
+  BATS_REQUIRE(count >= 0 && count <= kMultiMax, "count must be in [0, 16]");
+  if (count == 0) return 0;
+  BATS_REQUIRE(sizes && weights && deltas, "null array");
+  MultiTensors T{};
+  int64_t nmax = 0;
+  for (int k = 0; k < count; ++k) {
+    BATS_REQUIRE(sizes[k] >= 0, "negative size");
+    BATS_REQUIRE(sizes[k] == 0 || (weights[k] && deltas[k]), "null pointer");
+    T.n[k] = sizes[k];
+    T.w[k] = weights[k];
+    T.g[k] = deltas[k];
+    nmax = std::max(nmax, sizes[k]);
+  }
+  if (nmax == 0) return 0;
+  hipLaunchKernelGGL(add_multi_kernel, dim3(std::min<unsigned>(grid_for(nmax), 4096), count),
+                     dim3(kThreads), 0, as_stream(stream), T);
+  return check_launch("add_multi_kernel");
 }
 
 extern "C" int bats_sgd_step(void *stream, int64_t n, const float *grad, float *delta,

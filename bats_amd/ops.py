@@ -201,6 +201,59 @@ def sgd_step(grad, lr, weights=None, want_delta=True):
     return delta
 
 
+def _ptr_array(tensors):
+    import ctypes
+    return (ctypes.c_void_p * len(tensors))(*[None if t is None else t.data_ptr() for t in tensors])
+
+
+MULTI_MAX = 16  # tensors per multi-tensor launch
+
+
+def adam_step_multi(grads, ms, vs, lr, beta_1, omb1, beta_2, omb2, eps, bias1, bias2,
+                    nadam=False, weights=None, want_delta=True,
+                    step_dev: Optional[torch.Tensor] = None):
+    """:func:`adam_step` over several tensors in one launch per 16 tensors (the per-layer
+    loop of bats/Optimizers/AdamOptimizer.py:35-55). Returns the list of deltas (None
+    entries when ``want_delta`` is False)."""
+    import ctypes
+    _require_gpu()
+    lib = _lib.load()
+    for t in list(grads) + list(ms) + list(vs) + ([] if weights is None else list(weights)):
+        _ptr(t, F32)
+    deltas = [torch.empty_like(g) if want_delta else None for g in grads]
+    for k0 in range(0, len(grads), MULTI_MAX):
+        sl = slice(k0, k0 + MULTI_MAX)
+        g, m, v = grads[sl], ms[sl], vs[sl]
+        sizes = (ctypes.c_int64 * len(g))(*[t.numel() for t in g])
+        _lib.check(lib.bats_adam_step_multi(
+            _stream(), len(g), sizes, _ptr_array(g), _ptr_array(m), _ptr_array(v),
+            _ptr_array(deltas[sl]) if want_delta else None,
+            _ptr_array(weights[sl]) if weights is not None else None, float(lr), float(beta_1),
+            float(omb1), float(beta_2), float(omb2), float(eps), float(bias1), float(bias2),
+            _ptr(step_dev) if step_dev is not None else None, 1 if nadam else 0))
+    return deltas
+
+
+def add_deltas_multi(weights, deltas) -> None:
+    """:func:`add_deltas` over several (weights, delta) pairs in one launch per 16 pairs."""
+    import ctypes
+    _require_gpu()
+    lib = _lib.load()
+    ds = []
+    for w, d in zip(weights, deltas):
+        if d.numel() != w.numel():
+            raise ValueError("delta and weights differ in size")
+        _ptr(w, F32)
+        dc = d.reshape(w.shape).contiguous()
+        _ptr(dc, F32)
+        ds.append(dc)
+    for k0 in range(0, len(weights), MULTI_MAX):
+        w, d = weights[k0:k0 + MULTI_MAX], ds[k0:k0 + MULTI_MAX]
+        sizes = (ctypes.c_int64 * len(w))(*[t.numel() for t in w])
+        _lib.check(lib.bats_add_deltas_multi(_stream(), len(w), sizes, _ptr_array(w),
+                                             _ptr_array(d)))
+
+
 def add_deltas(weights: torch.Tensor, delta: torch.Tensor) -> None:
     _require_gpu()
     if delta.numel() != weights.numel():

@@ -224,6 +224,28 @@ int bats_sgd_step(void *stream, int64_t n, const float *grad, float *delta,
 /* weights += delta. Replaces bats/Layers/LIFLayer.py:112-113 (add_deltas). */
 int bats_add_deltas(void *stream, int64_t n, float *weights, const float *delta);
 
+/*
+ * One optimizer step of every layer in one launch: Adam (nadam = 0) or NAdam (nadam = 1) over
+ * count <= 16 tensors, tensor k of sizes[k] elements at grads[k], m[k], v[k], writing
+ * deltas[k] and/or adding to weights[k] (either array, or an entry of it, may be NULL where the
+ * other is given). The pointer arrays are host memory, read before the call returns; step as
+ * in bats_adam_step_dev (NULL: bias1/bias2 are used). Replaces the per-layer loop of
+ * bats/Optimizers/AdamOptimizer.py:35-55 (and NAdamOptimizer.py:36-58) with one kernel.
+ */
+int bats_adam_step_multi(void *stream, int count, const int64_t *sizes,
+                         const float *const *grads, float *const *m, float *const *v,
+                         float *const *deltas, float *const *weights, float lr, float beta_1,
+                         float one_minus_beta_1, float beta_2, float one_minus_beta_2,
+                         float epsilon, double bias1, double bias2, const int64_t *step,
+                         int nadam);
+
+/*
+ * weights[k] += deltas[k] for count <= 16 tensors in one launch (host pointer arrays). Replaces
+ * the per-layer loop of bats/Network.py:50-54 (apply_deltas).
+ */
+int bats_add_deltas_multi(void *stream, int count, const int64_t *sizes, float *const *weights,
+                          const float *const *deltas);
+
 /* ------------------------------------------------------------------ input encoding */
 
 /*

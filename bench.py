@@ -561,9 +561,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the batch would shard over GPUs with one gradient all-reduce per step, "
-              "see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the batch would "
+              "shard over GPUs with one gradient all-reduce per step); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     c = None if args.skip_cpu else cpu_baseline(args)  # before the GPU is initialised

@@ -95,8 +95,9 @@ class AdamOptimizer(AbstractOptimizer):
         step_dev = None
         if self._capturable:
             if self._t_dev is None:
+                # starts from the host count, so that a restored state resumes at t + 1
                 dev = next(g.device for g in gradient if g is not None)
-                self._t_dev = torch.zeros((), dtype=torch.int64, device=dev)
+                self._t_dev = torch.full((), self._t - 1, dtype=torch.int64, device=dev)
             self._t_dev.add_(1)
             step_dev = self._t_dev
         b1, b2 = self._bias()

@@ -42,7 +42,11 @@ def _network(w1, w2):
 
 
 @pytest.mark.gpu
-def test_resume_is_exact(tmp_path):
+@pytest.mark.parametrize("capturable", [False, True])
+def test_resume_is_exact(tmp_path, capturable):
+    """Two steps, a checkpoint, two more steps from the restored state equal four steps in a
+    row, bit for bit (weights and the step counter), with the step counter on the host or on
+    the device."""
     from bats_amd.Losses import SpikeCountClassLoss
     rng = np.random.default_rng(5)
     B, I = 16, 784
@@ -61,16 +65,21 @@ def test_resume_is_exact(tmp_path):
             grads = net.backward(err)
             net.apply_deltas(opt.step([None if g is None else g[0] for g in grads]))
 
-    net_a, opt_a = _network(w1, w2), AdamOptimizer(learning_rate=0.01)
+    net_a, opt_a = _network(w1, w2), AdamOptimizer(learning_rate=0.01, capturable=capturable)
     steps(net_a, opt_a, 4)
-    net_b, opt_b = _network(w1, w2), AdamOptimizer(learning_rate=0.01)
+    net_b, opt_b = _network(w1, w2), AdamOptimizer(learning_rate=0.01, capturable=capturable)
     steps(net_b, opt_b, 2)
     checkpoint.save(tmp_path, net_b, opt_b)
     assert (tmp_path / "hidden_weights.npy").exists() and (tmp_path / "adam_state.npz").exists()
-    net_c, opt_c = _network(np.zeros_like(w1), np.zeros_like(w2)), AdamOptimizer(learning_rate=0.5)
+    net_c = _network(np.zeros_like(w1), np.zeros_like(w2))
+    opt_c = AdamOptimizer(learning_rate=0.5, capturable=capturable)
     checkpoint.load(tmp_path, net_c, opt_c)
     steps(net_c, opt_c, 2)
     torch.cuda.synchronize()
     for la, lc in zip(net_a.layers[1:], net_c.layers[1:]):
         assert torch.equal(la.weights, lc.weights), la.name
     assert opt_c.state[2] == 4
+    for ma, mc in zip(opt_a.state[0] + opt_a.state[1], opt_c.state[0] + opt_c.state[1]):
+        assert (ma is None) == (mc is None)
+        if ma is not None:
+            assert torch.equal(ma, mc)

@@ -1,7 +1,7 @@
 """The deep dense network (784-800-800-800-10, K = 30 everywhere, thresholds 1.0 / 1.3,
 W ~ N(0, 1), latency-coded inputs, spike-count loss with targets 15 / 3) end to end against
-the FP64 oracle, layer by layer, through the Network API and through chained autograd
-Functions.
+the FP64 oracle at batch 64, layer by layer, through the Network API and through chained
+autograd Functions.
 
 Each layer's backward receives the errors the GPU propagated from the layer above (not the
 oracle's), so the comparison covers the error carried through the whole chain. Bar, for every
@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 SIZES = [784, 800, 800, 800, 10]
 THETAS = [1.0, 1.0, 1.0, 1.3]
-TAU_S, K, T_MAX, B = 0.130, 30, 1.0, 16
+TAU_S, K, T_MAX, B = 0.130, 30, 1.0, 64
 
 
 def _problem():
@@ -28,6 +28,17 @@ def _problem():
     Ws = [rng.normal(0.0, 1.0, size=(SIZES[k + 1], SIZES[k])).astype(F32) for k in range(4)]
     labels = rng.integers(0, 10, size=B)
     return t, n, Ws, labels
+
+
+_CACHE = {}
+
+
+def _reference():
+    """The problem and its oracle forward, computed once for both tests (about 10 s)."""
+    if not _CACHE:
+        t, n, Ws, labels = _problem()
+        _CACHE["v"] = (t, n, Ws, labels) + _oracle(t, Ws, labels)
+    return _CACHE["v"]
 
 
 def _oracle(t, Ws, labels):
@@ -58,8 +69,7 @@ def test_deep_network_per_layer_against_oracle():
     from bats_amd.Layers import InputLayer, LIFLayer
     from bats_amd.Losses import SpikeCountClassLoss
 
-    t, n, Ws, labels = _problem()
-    states, err_out = _oracle(t, Ws, labels)
+    t, n, Ws, labels, states, err_out = _reference()
     # oracle backward, errors carried through the chain in the oracle itself
     ref_grads, err = [None] * 4, err_out
     for k in range(3, -1, -1):
@@ -102,8 +112,7 @@ def test_deep_network_per_layer_against_oracle():
 def test_deep_network_autograd_chain():
     from bats_amd.autograd import lif
 
-    t, _, Ws, labels = _problem()
-    states, err_out = _oracle(t, Ws, labels)
+    t, _, Ws, labels, states, err_out = _reference()
     ref_grads, err = [None] * 4, err_out
     for k in range(3, -1, -1):
         _, g, pe = olif.lif_backward(states[k], err, pre_trainable=k > 0)

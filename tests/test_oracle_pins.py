@@ -398,3 +398,98 @@ def test_events_to_per_neuron_reference_hand_arrays():
     t, n = oev.events_to_per_neuron(idx, tim, 3)
     assert np.array_equal(n, true_n)
     np.testing.assert_allclose(t, true_t)
+
+
+def test_nadam_closed_form_and_convergence():
+    """NAdam (bats/Optimizers/NAdamOptimizer.py:36-58). With a constant gradient g the bias
+    corrections make m_hat = g and v_hat = g^2 at every step, so the update reduces to
+    delta_t = -lr * g / (|g| + eps) * (beta_1 + (1 - beta_1) / (1 - beta_1^t)):
+    t = 1 gives -lr (1 + beta_1) g / (|g| + eps), t = 2 gives the factor beta_1 + 1/(1 + beta_1)."""
+    rng = np.random.default_rng(17)
+    g = rng.normal(size=(6, 5)).astype(F32)
+    lr, b1 = 0.05, 0.9
+    opt = oopt.NAdam(lr)
+    base = -lr * g.astype(F64) / (np.abs(g.astype(F64)) + 1e-8)
+    d1 = opt.step([None, g])
+    assert d1[0] is None
+    np.testing.assert_allclose(d1[1], base * (1.0 + b1), rtol=1e-5)
+    d2 = opt.step([None, g])
+    np.testing.assert_allclose(d2[1], base * (b1 + 1.0 / (1.0 + b1)), rtol=1e-5)
+    d3 = opt.step([None, g])
+    np.testing.assert_allclose(d3[1], base * (b1 + (1 - b1) / (1 - b1 ** 3)), rtol=1e-5)
+    # the convergence test of tests/Optimizers/AdamOptimizer.py:8-33 with this optimizer
+    w = [None, rng.normal(size=(5, 10)).astype(F32), rng.normal(size=(2, 5)).astype(F32)]
+    opt = oopt.NAdam(0.05)
+    for _ in range(1000):
+        for k, dd in enumerate(opt.step(w)):
+            if dd is not None:
+                w[k] = (w[k] + dd).astype(F32)
+    for ww in w[1:]:
+        assert np.allclose(ww, 0.0, atol=1e-6)
+
+
+def test_gd_closed_form_and_convergence():
+    """GradientDescentOptimizer (bats/Optimizers/GradientDescentOptimizer.py:8-9): delta =
+    -lr * g, None passes through; tests/Optimizers/GradientDescentOptimizer.py:8-33 drives
+    0.5*|w|^2 to zero (w_t = 0.95^t w_0 exactly in real arithmetic)."""
+    rng = np.random.default_rng(18)
+    g = rng.normal(size=(4, 3)).astype(F32)
+    d = oopt.gd_step([None, g, None], 0.05)
+    assert d[0] is None and d[2] is None
+    assert d[1].dtype == F32
+    np.testing.assert_array_equal(d[1], (F64(F32(0.05)) * -g.astype(F64)).astype(F32))
+    w0 = [None, rng.normal(size=(5, 10)).astype(F32), rng.normal(size=(2, 5)).astype(F32)]
+    w = [None if x is None else x.copy() for x in w0]
+    for step in range(1000):
+        for k, dd in enumerate(oopt.gd_step(w, 0.05)):
+            if dd is not None:
+                w[k] = (w[k] + dd).astype(F32)
+        if step == 9:
+            for a, b in zip(w[1:], w0[1:]):
+                np.testing.assert_allclose(a, b * (1 - F64(F32(0.05))) ** 10, rtol=1e-5)
+    for ww in w[1:]:
+        assert np.allclose(ww, 0.0)
+
+
+def test_spike_time_loss_hand_arrays():
+    """SpikeTimeLoss (bats/Losses/SpikeTimeLoss.py:38-47) on hand arrays: an infinite target
+    or first spike counts as 1.0 in the loss, while the errors subtract the sanitised targets
+    from the raw times (an absent spike keeps an infinite error, SpikeTimeLoss.py:45)."""
+    inf = np.inf
+    t = np.array([[[0.1, inf], [inf, inf], [0.3, 0.5]],
+                  [[0.7, 0.8], [0.2, inf], [inf, inf]]], dtype=F32)
+    tg = np.array([[[0.2, inf], [0.4, inf], [inf, inf]],
+                   [[0.5, 0.9], [inf, inf], [0.6, 0.1]]], dtype=F32)
+    loss, err = oloss.spike_time_loss_and_errors(t, tg)
+    want_loss = [0.5 * ((0.2 - 0.1) ** 2 + (0.4 - 1.0) ** 2 + (1.0 - 0.3) ** 2),
+                 0.5 * ((0.5 - 0.7) ** 2 + (1.0 - 0.2) ** 2 + (0.6 - 1.0) ** 2)]
+    np.testing.assert_allclose(loss, want_loss, rtol=1e-6)
+    want_err = np.array([[[-0.1, inf], [inf, inf], [-0.7, -0.5]],
+                         [[0.2, -0.1], [-0.8, inf], [inf, inf]]])
+    fin = np.isfinite(want_err)
+    assert np.array_equal(np.isinf(err), ~fin) and np.all(err[~fin] > 0)
+    np.testing.assert_allclose(err[fin], want_err[fin], rtol=1e-6, atol=1e-7)
+    # the caller's targets are not modified by the oracle (it works on a copy)
+    assert np.isinf(tg[0, 0, 1])
+
+
+def test_lif_layer_descends_to_target_times():
+    """The reference's end-to-end layer test (tests/Layers/LIFLayer_tests.py:91-132) on the
+    oracle: 7 inputs -> 5 LIF neurons, all weights 1, errors = first spike time - target,
+    gradient descent at lr 3e3 on the batch-mean gradient for 200 epochs; the first spike
+    times reach the targets within 5e-4. A wrong sign or a dropped term in the backward does
+    not descend."""
+    tau_s, theta, K, inf = 10e-3, 0.01, 8, np.inf
+    t_in = np.array([[[0.0, inf], [3e-3, inf], [6e-3, inf], [8e-3, inf], [9e-3, inf],
+                      [11e-3, inf], [17e-3, inf]]], dtype=F32)
+    targets = np.array([[4e-3, 6e-3, 8e-3, 12e-3, 16e-3]], dtype=F32)
+    W = np.full((5, 7), 1.0, dtype=F32)
+    for _ in range(200):
+        st = olif.lif_forward(t_in, W, tau_s, theta, theta, K, tmax=0.02)
+        err = np.zeros(st.t.shape, dtype=F32)
+        err[:, :, 0] = st.t[:, :, 0] - targets
+        _, g, pre = olif.lif_backward(st, err, pre_trainable=False)
+        assert pre is None
+        W = (W.astype(F64) + (-F64(F32(3e3)) * olif.batch_mean(g).astype(F64))).astype(F32)
+    st = olif.lif_forward(t_in, W, tau_s, theta, theta, K, tmax=0.02)
+    assert np.allclose(st.t[:, :, 0], targets, atol=5e-4), st.t[:, :, 0]

@@ -24,8 +24,8 @@ SIGNATURES = {
     "bats_lif_forward_workspace_size": (_sz, [_i, _i, _i, _i]),
     "bats_lif_forward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
-    "bats_set_solver_mode": (_i, [_i]),
-    "bats_set_solver_blocks": (_i, [_i]),
+    "bats_lif_forward_solver": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _sz]),
     "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i, _i]),
     "bats_lif_backward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp, _sz]),

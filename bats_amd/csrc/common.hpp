@@ -26,15 +26,17 @@ bool timing_enabled();
 void timing_begin(const char *name, hipStream_t st, hipEvent_t *e0, hipEvent_t *e1);
 void timing_end(const char *name, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
 
-// Launch wrapper: records start/stop events around the launch when timing is enabled.
-#define BATS_TIMED(name, st, ...)                               \
+// Launch wrapper: records start/stop events around the launch when timing is enabled (and
+// `allow`, for launches that must not create events, is true).
+#define BATS_TIMED_IF(allow, name, st, ...)                     \
   do {                                                          \
     hipEvent_t _e0 = nullptr, _e1 = nullptr;                    \
-    const bool _t = ::bats::timing_enabled();                   \
+    const bool _t = (allow) && ::bats::timing_enabled();        \
     if (_t) ::bats::timing_begin(name, st, &_e0, &_e1);         \
     __VA_ARGS__;                                                \
     if (_t) ::bats::timing_end(name, st, _e0, _e1);             \
   } while (0)
+#define BATS_TIMED(name, st, ...) BATS_TIMED_IF(true, name, st, __VA_ARGS__)
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -104,7 +106,7 @@ struct EventLists {
 size_t event_lists_bytes(int B, int S);
 int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,
                       float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
-                      void *ws, EventLists *out);
+                      void *ws, EventLists *out, bool timed = true);
 
 // Workgroups are dealt to the 8 XCDs round robin; this maps dispatch index p of nblk to a
 // logical index such that consecutive logical blocks run on one XCD and share its L2.

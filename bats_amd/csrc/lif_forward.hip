@@ -586,10 +586,8 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
 }
 
 // ------------------------------------------------------------------ host side
-// 0 auto, 1 lane per neuron, 2 wave per neuron, 3 / 4 lane per neuron with staged / scalar
-// event records
-static int g_solver_mode = 0;
-static int g_solver_blocks = 0;  // workgroups per sample of the lane solver, 0 auto
+// Solver forms (per call): 0 auto, 1 lane per neuron, 2 wave per neuron, 3 / 4 lane per
+// neuron with staged / scalar event records.
 
 // Lane-per-neuron keeps the event walk sequential in each lane and needs many (sample,
 // neuron) pairs to fill 256 CUs; below this many pairs the wave-per-neuron form is used.
@@ -653,7 +651,7 @@ size_t event_lists_bytes(int B, int S) {
 
 int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,
                       float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
-                      void *ws, EventLists *out) {
+                      void *ws, EventLists *out, bool timed) {
   const int cap = lds_capacity_entries();
   int N = 1;
   while (N < S) N <<= 1;
@@ -673,7 +671,7 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   out->Pcap = Pcap;
   if (S > 0) {
     const int lds_entries = std::max(2048, std::min(N, cap));  // the short-list merge buffers
-    BATS_TIMED("sort_events_kernel", st,
+    BATS_TIMED_IF(timed, "sort_events_kernel", st,
                hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
                                   sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,
                                   tau, c, tmax, Pcap, out->events, out->fast, out->idx, out->counts, pre_es,
@@ -698,16 +696,23 @@ extern "C" size_t bats_lif_forward_workspace_size(int B, int I, int K_pre, int J
   return fwd_bytes(B, I, K_pre, J);
 }
 
-extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
-                                const float *pre_times, const float *weights, float tau_s,
-                                float tau, float c, float delta_theta_tau,
-                                float max_simulation, int *out_n, float *out_t, float *out_a,
-                                float *out_x, float *out_z, float *pre_exp_tau_s,
-                                float *pre_exp_tau, void *workspace, size_t workspace_bytes) {
+// The forward of one dense layer. solver_mode / solver_blocks choose the solver form and the
+// lane solver's workgroups per sample (0: automatic); `timed` allows library timing events.
+static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_post,
+                            const float *pre_times, const float *weights, float tau_s,
+                            float tau, float c, float delta_theta_tau, float max_simulation,
+                            int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                            float *pre_exp_tau_s, float *pre_exp_tau, int solver_mode,
+                            int solver_blocks, bool timed, void *workspace,
+                            size_t workspace_bytes) {
   BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0, "negative size");
   BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
   BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
   BATS_REQUIRE(B <= 65535, "batch above 65535");
+  BATS_REQUIRE(solver_mode >= 0 && solver_mode <= 4,
+               "solver_mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron), 3 (lane "
+               "per neuron, staged event records) or 4 (lane per neuron, scalar event records)");
+  BATS_REQUIRE(solver_blocks >= 0, "solver_blocks must be >= 0 (0: automatic)");
   if (B == 0 || J == 0) return 0;
   BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
   BATS_REQUIRE(tau_s > 0.f && tau > 0.f, "tau_s and tau must be positive");
@@ -721,7 +726,7 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   void *ev_ws = ws + align_up(sizeof(float) * (size_t)I * J, 256);
   const int always_exact = (c > 0.f) ? 0 : 1;
   const long long BJ = (long long)B * J;
-  const bool wave_mode = g_solver_mode == 2 || (g_solver_mode == 0 && BJ < kSeqMinLanes);
+  const bool wave_mode = solver_mode == 2 || (solver_mode == 0 && BJ < kSeqMinLanes);
   if (S > 0) {
     BATS_REQUIRE(pre_times && weights && pre_exp_tau_s && pre_exp_tau, "null input pointer");
     if (!wave_mode) {  // the wave-per-neuron solver reads W's rows directly
@@ -732,12 +737,12 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   }
   EventLists el;
   if (int r = build_event_lists(st, B, S, K_pre, pre_times, tau_s, tau, c, max_simulation,
-                                pre_exp_tau_s, pre_exp_tau, ev_ws, &el))
+                                pre_exp_tau_s, pre_exp_tau, ev_ws, &el, timed))
     return r;
   if (wave_mode && S > 0) {
     BATS_REQUIRE(BJ < 2147483647LL, "too many (sample, neuron) pairs");
     const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));
-    BATS_TIMED("lif_solve_wave_kernel", st,
+    BATS_TIMED_IF(timed, "lif_solve_wave_kernel", st,
                hipLaunchKernelGGL(lif_solve_wave_kernel, dim3(blocks), dim3(kWaveThreads), 0, st,
                                   el.events, el.idx, el.counts, el.Pcap, weights, I, J, (int)BJ,
                                   K_post,
@@ -751,20 +756,20 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
   const int cus = cu_count();
   if ((long long)B * nblk < cus) nblk = std::max(nblk, std::min((cus + B - 1) / B, (J + 63) / 64));
-  if (g_solver_blocks > 0) nblk = std::max(1, std::min(g_solver_blocks, (J + 63) / 64));
+  if (solver_blocks > 0) nblk = std::max(1, std::min(solver_blocks, (J + 63) / 64));
   const int per = (J + nblk - 1) / nblk;
   const int threads = (per + 63) / 64 * 64;
   dim3 sg((J + threads - 1) / threads, B);
   // staged event records when the launch has fewer than 3 waves per SIMD (4 SIMDs per CU):
   // with more, the other waves hide the per-event scalar loads and staging costs a little
   const long long waves = (long long)B * sg.x * (threads / 64);
-  const bool staged = g_solver_mode == 3 ||
-                      (g_solver_mode != 4 && waves < 3LL * 4 * (long long)cus);
+  const bool staged = solver_mode == 3 ||
+                      (solver_mode != 4 && waves < 3LL * 4 * (long long)cus);
   const auto seq = always_exact ? (staged ? lif_solve_seq_kernel<true, true>
                                           : lif_solve_seq_kernel<true, false>)
                                 : (staged ? lif_solve_seq_kernel<false, true>
                                           : lif_solve_seq_kernel<false, false>);
-  BATS_TIMED("lif_solve_seq_kernel", st,
+  BATS_TIMED_IF(timed, "lif_solve_seq_kernel", st,
              hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
                                 el.counts, el.Pcap, WT, J, K_post, (double)c,
                                 (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
@@ -772,16 +777,28 @@ extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, in
   return check_launch("lif_solve_seq_kernel");
 }
 
-extern "C" int bats_set_solver_blocks(int per_sample) {
-  BATS_REQUIRE(per_sample >= 0, "per_sample must be >= 0 (0: automatic)");
-  g_solver_blocks = per_sample;
-  return 0;
+extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
+                                const float *pre_times, const float *weights, float tau_s,
+                                float tau, float c, float delta_theta_tau,
+                                float max_simulation, int *out_n, float *out_t, float *out_a,
+                                float *out_x, float *out_z, float *pre_exp_tau_s,
+                                float *pre_exp_tau, void *workspace, size_t workspace_bytes) {
+  return lif_forward_impl(stream, B, I, K_pre, J, K_post, pre_times, weights, tau_s, tau, c,
+                          delta_theta_tau, max_simulation, out_n, out_t, out_a, out_x, out_z,
+                          pre_exp_tau_s, pre_exp_tau, 0, 0, true, workspace, workspace_bytes);
 }
 
-extern "C" int bats_set_solver_mode(int mode) {
-  BATS_REQUIRE(mode >= 0 && mode <= 4,
-               "mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron), 3 (lane per "
-               "neuron, staged event records) or 4 (lane per neuron, scalar event records)");
-  g_solver_mode = mode;
-  return 0;
+extern "C" int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, int J, int K_post,
+                                       const float *pre_times, const float *weights,
+                                       float tau_s, float tau, float c, float delta_theta_tau,
+                                       float max_simulation, int *out_n, float *out_t,
+                                       float *out_a, float *out_x, float *out_z,
+                                       float *pre_exp_tau_s, float *pre_exp_tau,
+                                       int solver_mode, int solver_blocks, void *workspace,
+                                       size_t workspace_bytes) {
+  return lif_forward_impl(stream, B, I, K_pre, J, K_post, pre_times, weights, tau_s, tau, c,
+                          delta_theta_tau, max_simulation, out_n, out_t, out_a, out_x, out_z,
+                          pre_exp_tau_s, pre_exp_tau, solver_mode, solver_blocks, true,
+                          workspace, workspace_bytes);
 }
+

@@ -55,11 +55,14 @@ def _workspace(nbytes: int, device) -> torch.Tensor:
 # ---------------------------------------------------------------- dense LIF
 def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float32,
                 tau: np.float32, c: np.float32, delta_theta_tau: np.float32,
-                max_simulation: float, max_n_post: int):
+                max_simulation: float, max_n_post: int, solver_mode: int = 0,
+                solver_blocks: int = 0):
     """Dense LIF forward (bats/Layers/LIFLayer.py:63-91).
 
     Returns (n [B,J] i32, t [B,J,K] f32 +inf padded, a, x, z [B,J,K], pre_exp_tau_s,
-    pre_exp_tau [B,I,K_pre]).
+    pre_exp_tau [B,I,K_pre]). ``solver_mode`` / ``solver_blocks`` force a solver form and the
+    lane solver's workgroups per sample for this call (0: automatic; see
+    bats_lif_forward_solver); every form gives the same results.
     """
     _require_gpu()
     B, I, Kp = pre_times.shape
@@ -78,10 +81,11 @@ def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float3
     lib = _lib.load()
     ws_bytes = lib.bats_lif_forward_workspace_size(B, I, Kp, J)
     ws = _workspace(ws_bytes, dev)
-    _lib.check(lib.bats_lif_forward(
+    _lib.check(lib.bats_lif_forward_solver(
         _stream(), B, I, Kp, J, K, _ptr(pre_times, F32), _ptr(weights, F32),
         float(tau_s), float(tau), float(c), float(delta_theta_tau), float(np.float32(max_simulation)),
-        _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), _ptr(es), _ptr(e), _ptr(ws), ws_bytes))
+        _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), _ptr(es), _ptr(e), int(solver_mode),
+        int(solver_blocks), _ptr(ws), ws_bytes))
     return n, t, a, x, z, es, e
 
 

@@ -6,7 +6,9 @@
  * pointer; NULL is the default stream) and returns 0 on success or a nonzero status,
  * with the message available from bats_last_error(). Outputs and workspaces are
  * allocated by the caller; the library allocates no device memory, never synchronises
- * the stream and issues no memset, so every call can be captured in a HIP graph.
+ * the stream and issues no memset, so every call can be captured in a HIP graph. Every
+ * choice that changes which kernels run is a per-call argument; the only process-wide
+ * switch is the opt-in kernel timing at the end of this file, which changes no result.
  *
  * Layouts (row-major, float32 unless stated):
  *   spike times  [B, N, K]  ascending per neuron, +inf padded ("no spike")
@@ -61,21 +63,24 @@ int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
                      void *workspace, size_t workspace_bytes);
 
 /*
- * Solver form used by bats_lif_forward: 0 = automatic (default: one lane per (sample, post
- * neuron) when there are at least 32768 such pairs, else one wave per pair), 1 = always one
- * lane per pair, 2 = always one wave per pair, 3 / 4 = one lane per pair with the event
- * records staged in LDS / read with scalar loads (1 chooses by the launch's waves per SIMD).
- * All forms give the same results; the setting exists for testing and tuning. No
- * counterpart in the reference.
+ * bats_lif_forward with the solver form chosen by the caller for this call only (the library
+ * keeps no tuning state): solver_mode 0 = automatic (what bats_lif_forward does: one lane per
+ * (sample, post neuron) when there are at least 32768 such pairs, else one wave per pair),
+ * 1 = always one lane per pair, 2 = always one wave per pair, 3 / 4 = one lane per pair with
+ * the event records staged in LDS / read with scalar loads (1 chooses by the launch's waves
+ * per SIMD); solver_blocks = workgroups per sample of the lane-per-pair solver (0: one per
+ * 1024 neurons, more when the batch has fewer samples than the device has CUs). Every form
+ * gives the same results; the arguments exist for testing and tuning. Same reference
+ * interfaces as bats_lif_forward (bats/Layers/LIFLayer.py:63-91).
  */
-int bats_set_solver_mode(int mode);
-
-/*
- * Workgroups per sample of the lane-per-neuron solver (0, the default: one per 1024 neurons,
- * more when the batch has fewer samples than the device has CUs). A tuning knob with no
- * counterpart in the reference; results do not depend on it.
- */
-int bats_set_solver_blocks(int per_sample);
+int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, int J, int K_post,
+                            const float *pre_times, const float *weights,
+                            float tau_s, float tau, float c, float delta_theta_tau,
+                            float max_simulation,
+                            int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                            float *pre_exp_tau_s, float *pre_exp_tau,
+                            int solver_mode, int solver_blocks,
+                            void *workspace, size_t workspace_bytes);
 
 /* ------------------------------------------------------------------ dense LIF backward */
 
@@ -289,8 +294,9 @@ int bats_events_csr_to_per_neuron(void *stream, int B, int N, int K, const int *
 /*
  * Per-kernel HIP-event timing of the library's main launches (sort_events_kernel,
  * lif_solve_seq_kernel, lif_solve_wave_kernel, wgrad_kernel). No counterpart in the
- * reference (it has no profiler). Off by default; enabling it adds two event records per
- * timed launch and must not be used while a stream is being captured.
+ * reference (it has no profiler). Off by default; a process-wide observability switch (like
+ * a profiler's): enabling it adds two event records per timed launch of every caller, changes
+ * no result, and must not be used while a stream is being captured.
  */
 void bats_timing_enable(int on);
 void bats_timing_reset(void);

@@ -48,8 +48,9 @@ def test_errors_are_reported_without_a_gpu():
                               None, None, None, None, None, None, None, None, 0)
     assert rc != 0
     assert b"K_post" in lib.bats_last_error()
-    assert lib.bats_set_solver_mode(7) != 0
-    assert lib.bats_set_solver_mode(0) == 0
+    rc = lib.bats_lif_forward_solver(None, 1, 1, 1, 1, 1, None, None, 0.1, 0.2, 1.0, 1.0, 1.0,
+                                     None, None, None, None, None, None, None, 7, 0, None, 0)
+    assert rc != 0 and b"solver_mode" in lib.bats_last_error()
 
 
 def test_workspace_sizes_are_positive():

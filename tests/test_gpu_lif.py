@@ -20,21 +20,21 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 
 
-def gpu_forward(pre_t, W, tau_s, theta, dth, K, tmax):
+def gpu_forward(pre_t, W, tau_s, theta, dth, K, tmax, solver_mode=0):
     from bats_amd import ops
     from bats_amd._constants import lif_constants
     ts, tau, c, dtt = lif_constants(tau_s, theta, dth)
     pt = torch.as_tensor(pre_t, device="cuda")
     Wd = torch.as_tensor(W, device="cuda")
-    out = ops.lif_forward(pt, Wd, ts, tau, c, dtt, tmax, K)
+    out = ops.lif_forward(pt, Wd, ts, tau, c, dtt, tmax, K, solver_mode=solver_mode)
     torch.cuda.synchronize()
     return [o.cpu().numpy() for o in out]
 
 
-def check_forward(pre_t, W, tau_s, theta, dth, K, tmax, label):
+def check_forward(pre_t, W, tau_s, theta, dth, K, tmax, label, solver_mode=0):
     print(f"[forward] {label}: pre {pre_t.shape} W {W.shape} K={K}", flush=True)
     ref = olif.lif_forward(pre_t, W, tau_s, theta, dth, K, tmax=tmax)
-    n, t, a, x, z, es, e = gpu_forward(pre_t, W, tau_s, theta, dth, K, tmax)
+    n, t, a, x, z, es, e = gpu_forward(pre_t, W, tau_s, theta, dth, K, tmax, solver_mode)
     assert_forward_match(n, t, a, x, z, ref)
     fin = np.isfinite(pre_t)
     assert np.array_equal(es[fin], ref.pre_exp_s[fin])
@@ -229,31 +229,22 @@ def test_tied_spike_times():
     check_backward(h2, rng.normal(size=h2.t.shape).astype(F32), "tied pre and post spikes")
 
 
-@pytest.fixture
-def solver_mode():
-    from bats_amd import _lib
-    lib = _lib.load()
-    yield lambda m: _lib.check(lib.bats_set_solver_mode(m))
-    _lib.check(lib.bats_set_solver_mode(0))
-
-
 @pytest.mark.parametrize("mode", [1, 2, 3, 4])
-def test_forward_both_solver_forms(solver_mode, mode):
+def test_forward_both_solver_forms(mode):
     # the lane-per-neuron (automatic, staged and scalar event records) and wave-per-neuron
     # solvers on the same hidden and output shapes, and with theta = 0 (no threshold
-    # quadratic: every interval takes the closed form)
-    solver_mode(mode)
+    # quadratic: every interval takes the closed form); the form is chosen per call
     rng = np.random.default_rng(11)
     t, _ = latency_inputs(rng, 16, 784)
     W1 = (2.0 * rng.normal(0, 1.0, size=(800, 784))).astype(F32)
-    h, _ = check_forward(t, W1, 0.130, 1.0, 1.0, 30, 1.0, f"hidden mode {mode}")
+    h, _ = check_forward(t, W1, 0.130, 1.0, 1.0, 30, 1.0, f"hidden mode {mode}", mode)
     W2 = rng.normal(0, 1.0, size=(10, 800)).astype(F32)
-    check_forward(h.t, W2, 0.130, 1.3, 1.3, 30, 1.0, f"output mode {mode}")
+    check_forward(h.t, W2, 0.130, 1.3, 1.3, 30, 1.0, f"output mode {mode}", mode)
     tb, _ = multi_spike_inputs(rng, 8, 100, 5, t_max=1.0)
     W3 = np.abs(rng.normal(0, 3.0, size=(64, 100))).astype(F32)
-    check_forward(tb, W3, 0.05, 0.2, 0.2, 3, 0.6, f"buffer full mode {mode}")
+    check_forward(tb, W3, 0.05, 0.2, 0.2, 3, 0.6, f"buffer full mode {mode}", mode)
     W4 = rng.normal(0.2, 1.0, size=(96, 100)).astype(F32)
-    check_forward(tb, W4, 0.05, 0.0, 0.3, 4, 0.6, f"theta 0 mode {mode}")
+    check_forward(tb, W4, 0.05, 0.0, 0.3, 4, 0.6, f"theta 0 mode {mode}", mode)
 
 
 # ------------------------------------------------------------------ input encoding

@@ -26,6 +26,9 @@ SIGNATURES = {
                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
     "bats_lif_forward_solver": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _sz]),
+    "bats_lif_forward_noalloc_max_bytes": (_sz, []),
+    "bats_lif_forward_noalloc": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i, _i]),
     "bats_lif_backward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _i, _vp, _vp, _sz]),

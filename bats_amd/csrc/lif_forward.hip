@@ -802,3 +802,59 @@ extern "C" int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, in
                           workspace, workspace_bytes);
 }
 
+// ------------------------------------------------------------------ forward without allocation
+// The forward's workspace (W^T and the per-sample event records) in a file-scope device array
+// of a fixed size, for callers that cannot allocate. Every GPU the library is loaded on holds
+// its own copy until the process ends; calls on one GPU share it, so calls that overlap on one
+// GPU (two streams, two host threads) race. Nothing in it needs to start at zero: the sort
+// writes every record the solver reads, on every call.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices];
+int g_scratch_status[kMaxDevices];
+std::once_flag g_scratch_once[kMaxDevices];
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per device with that
+// device current (the host-side name of the array is not a device address).
+static int scratch_address(void **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail("bats_lif_forward_noalloc: no usable current device");
+  std::call_once(g_scratch_once[dev], [dev]() {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = p;
+    g_scratch_status[dev] = (e == hipSuccess && p != nullptr) ? 0 : (e == hipSuccess ? -1 : (int)e);
+  });
+  if (g_scratch_status[dev] != 0)
+    return fail("bats_lif_forward_noalloc: hipGetSymbolAddress of the scratch array failed (" +
+                std::to_string(g_scratch_status[dev]) + ")");
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" size_t bats_lif_forward_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int bats_lif_forward_noalloc(void *stream, int B, int I, int K_pre, int J, int K_post,
+                                        const float *pre_times, const float *weights,
+                                        float tau_s, float tau, float c, float delta_theta_tau,
+                                        float max_simulation, int *out_n, float *out_t,
+                                        float *out_a, float *out_x, float *out_z,
+                                        float *pre_exp_tau_s, float *pre_exp_tau) {
+  BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0, "negative size");
+  const size_t need = fwd_bytes(B, I, K_pre, J);
+  BATS_REQUIRE(need <= kNoAllocScratchBytes,
+               "these sizes need " + std::to_string(need) + " bytes of workspace, above the " +
+                   std::to_string(kNoAllocScratchBytes) +
+                   "-byte scratch array of the no-allocation entry point (use bats_lif_forward "
+                   "with a caller-allocated workspace)");
+  void *ws = nullptr;
+  if (int r = scratch_address(&ws)) return r;
+  return lif_forward_impl(stream, B, I, K_pre, J, K_post, pre_times, weights, tau_s, tau, c,
+                          delta_theta_tau, max_simulation, out_n, out_t, out_a, out_x, out_z,
+                          pre_exp_tau_s, pre_exp_tau, 0, 0, false, ws, kNoAllocScratchBytes);
+}

@@ -82,6 +82,25 @@ int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, int J, int K_
                             int solver_mode, int solver_blocks,
                             void *workspace, size_t workspace_bytes);
 
+/*
+ * bats_lif_forward without a workspace argument and without any allocation: the workspace is
+ * a file-scope device array of bats_lif_forward_noalloc_max_bytes() bytes (64 MiB) in the
+ * library, one per GPU, held until the process ends. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Sizes whose workspace
+ * (bats_lif_forward_workspace_size) exceeds the array are refused with an error before any
+ * launch. Calls on one GPU share the array, so calls that overlap on one GPU (several streams
+ * or host threads) race; calls in stream order do not. Library timing does not apply to it.
+ * Same arguments, outputs and results as bats_lif_forward otherwise.
+ */
+size_t bats_lif_forward_noalloc_max_bytes(void);
+int bats_lif_forward_noalloc(void *stream, int B, int I, int K_pre, int J, int K_post,
+                             const float *pre_times, const float *weights,
+                             float tau_s, float tau, float c, float delta_theta_tau,
+                             float max_simulation,
+                             int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                             float *pre_exp_tau_s, float *pre_exp_tau);
+
 /* ------------------------------------------------------------------ dense LIF backward */
 
 size_t bats_lif_backward_workspace_size(int B, int I, int K_pre, int J, int K_post);

@@ -11,8 +11,11 @@ N(0, 1/sqrt(fan_in)) initialisation leaves the hidden layer silent, see --varian
 The timed step is the whole step (forward of both layers, loss, backward of both layers,
 Adam with an on-device step counter, weight update); inputs, labels and weights are resident
 in HBM. It is timed both as stream launches and as replays of one captured HIP graph, K steps
-each after W warm-up steps; `value` is the faster of the two (named in config.launch).
-Prints one JSON line.
+each after W warm-up steps from the initial weights; `value` is the faster of the two (named in
+config.launch), from the wall clock around exactly K steps. Every step is also bracketed by HIP
+events on its stream (`step_ms_hip_events`: median, p10, p90, first and last step): training
+raises the hidden layer's activity, so later steps cost more (`spikes_per_sample` gives the
+activity of the first and of the last timed step). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -147,16 +150,23 @@ def gpu_bench(args):
         kern[name] = (ms / cnt if cnt else None, cnt)
     ops.timing(reset=True)
 
-    # stream launches: W warm-up steps, then exactly K timed steps
+    # stream launches: W warm-up steps, then exactly K timed steps; each step is also bracketed
+    # by HIP events on the launch stream (the library launches on torch's current stream)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
     restart(opt)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
+        ev[k][0].record()
         step()
+        ev[k][1].record()
     torch.cuda.synchronize()
     eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    eager_steps = [a.elapsed_time(b) for a, b in ev]
+    hid_spikes_last = float(hid.spike_trains[1].sum().item()) / args.batch
 
     # graph capture of the whole step
     opt = opt_graph
@@ -174,12 +184,62 @@ def gpu_bench(args):
         graph.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
+        ev[k][0].record()
         graph.replay()
+        ev[k][1].record()
     torch.cuda.synchronize()
     graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    graph_steps = [a.elapsed_time(b) for a, b in ev]
     return dict(graph_ms=graph_ms, eager_ms=eager_ms, kernels=kern, hid_spikes=hid_spikes,
-                out_spikes=out_spikes)
+                out_spikes=out_spikes, hid_spikes_last=hid_spikes_last,
+                eager_steps=eager_steps, graph_steps=graph_steps)
+
+
+def literal_bench(args):
+    """The same step on the literal initialisation N(0, 1/sqrt(fan_in)) (same seed and inputs),
+    stream launches, W warm-up + K timed steps: the hidden layer is silent there, so this times
+    the path with empty solver walks and an all-zero backward."""
+    import torch
+    from bats_amd.Losses import SpikeCountClassLoss
+    from bats_amd.Optimizers import AdamOptimizer
+    dev = torch.device("cuda", 0)
+    t, n, labels, w1, w2 = make_inputs(args.seed, args.batch, "literal")
+    net, hid, out = build_network(w1, w2)
+    t_d, n_d = torch.as_tensor(t, device=dev), torch.as_tensor(n, device=dev)
+    lab_d = torch.as_tensor(labels, device=dev)
+    loss_fct = SpikeCountClassLoss(target_true=TARGET_TRUE, target_false=TARGET_FALSE)
+    opt = AdamOptimizer(learning_rate=LR)
+
+    def step():
+        net.reset()
+        net.forward(t_d, n_d, max_simulation=T_MAX, training=True)
+        out_t, out_n = net.output_spike_trains
+        _, errors = loss_fct.compute_loss_and_errors(out_t, out_n, lab_d)
+        grads = net.backward(errors)
+        net.apply_deltas(opt.step([None if g is None else g[0] for g in grads]))
+
+    step()
+    spikes = [float(hid.spike_trains[1].sum().item()) / args.batch,
+              float(out.spike_trains[1].sum().item()) / args.batch]
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    return {"workload": "same step, W~N(0,1/sqrt(fan_in)) literal initialisation",
+            "ms_per_step": round(ms, 4), "samples_per_s": round(args.batch / (ms * 1e-3), 1),
+            "spikes_per_sample": {"hidden": round(spikes[0], 1), "output": round(spikes[1], 2)}}
+
+
+def _step_stats(ms_list):
+    a = np.asarray(ms_list, dtype=np.float64)
+    return {"median": round(float(np.median(a)), 4), "p10": round(float(np.percentile(a, 10)), 4),
+            "p90": round(float(np.percentile(a, 90)), 4), "first": round(float(a[0]), 4),
+            "last": round(float(a[-1]), 4)}
 
 
 # ---------------------------------------------------------------- plain form (baseline)
@@ -350,20 +410,22 @@ def first_forward_counts(args):
 
 # ---------------------------------------------------------------- CPU baseline (oracle)
 def _oracle_steps(job):
-    """One CPU worker: the FP64 NumPy oracle's training step on its shard, BLAS on one
-    thread; returns (samples, seconds)."""
-    seed, batch, variant, lo, hi, min_seconds = job
+    """One CPU worker: the FP64 NumPy oracle's training step on its shard with BLAS on one
+    thread, one warm-up step then `reps` timed steps; returns (samples per step, per-step
+    seconds)."""
+    seed, batch, variant, lo, hi, reps = job
     from threadpoolctl import threadpool_limits
     from oracle import lif as olif
     from oracle import losses as oloss
     from oracle import optim as ooptim
 
     t, n, labels, w1, w2 = make_inputs(seed, batch, variant)
-    t, labels = t[lo:hi], labels[lo:hi]
+    idx = np.arange(lo, hi) % batch
+    t, labels = t[idx], labels[idx]
+    times = []
     with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        reps = 0
-        while True:
+        for _ in range(1 + reps):
+            t0 = time.perf_counter()
             h = olif.lif_forward(t, w1, TAU_S, THETA_H, THETA_H, K_HID, tmax=T_MAX)
             o = olif.lif_forward(h.t, w2, TAU_S, THETA_O, THETA_O, K_OUT, tmax=T_MAX)
             tg = oloss.count_targets(o.n, labels, TARGET_TRUE, TARGET_FALSE)
@@ -372,31 +434,58 @@ def _oracle_steps(job):
             _, g1, _ = olif.lif_backward(h, pe, pre_trainable=False)
             opt = ooptim.Adam(LR)
             opt.step([olif.batch_mean(g1), olif.batch_mean(g2)])
-            reps += 1
-            el = time.perf_counter() - t0
-            if el >= min_seconds or reps >= 3:
-                break
-    return (hi - lo) * reps, el
+            times.append(time.perf_counter() - t0)
+    return hi - lo, times[1:]
+
+
+def cpu_share():
+    """(os.cpu_count(), cores in this process's affinity set, the cgroup CPU quota in cores or
+    None): the host cores the CPU baseline may use."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    for path in ("/sys/fs/cgroup/cpu.max",):
+        try:
+            with open(path) as f:
+                q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+        except (OSError, ValueError):
+            pass
+    if quota is None:
+        try:
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+                q = float(f.read())
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                period = float(f.read())
+            if q > 0:
+                quota = q / period
+        except (OSError, ValueError):
+            pass
+    return os.cpu_count(), aff, quota
 
 
 def cpu_baseline(args):
-    """The oracle's training step on bounded samples of the same workload: one core on
-    args.cpu_sample samples, then every worker core (at most 16) on its own shard of the
-    batch at once. Runs before the GPU is initialised (the workers are forked)."""
+    """The oracle's training step timed as BASELINE.md's CPU baseline asks: one process per
+    usable host core (the affinity set, capped by a cgroup CPU quota when there is one), each
+    on its own shard of the batch with BLAS/OpenMP on one thread, one warm-up step then the
+    median of `args.cpu_reps` steps; and the same on one core alone. Runs before the GPU is
+    initialised (the workers are forked)."""
     import multiprocessing as mp
-    one_n, one_s = _oracle_steps((args.seed, args.batch, args.variant, 0, args.cpu_sample,
-                                  args.cpu_min_seconds))
-    cores = max(1, min(16, os.cpu_count() or 1))
-    per = max(1, min(args.cpu_sample // 2, args.batch // cores))
-    jobs = [(args.seed, args.batch, args.variant, (k * per) % args.batch,
-             (k * per) % args.batch + per, args.cpu_min_seconds / 2) for k in range(cores)]
-    with mp.get_context("fork").Pool(cores) as pool:
+    n_cpu, aff, quota = cpu_share()
+    procs = aff if quota is None else max(1, min(aff, int(quota)))
+    reps = args.cpu_reps
+    one_n, one_t = _oracle_steps((args.seed, args.batch, args.variant, 0, args.cpu_sample, reps))
+    one = one_n / float(np.median(one_t))
+    per = args.cpu_shard
+    jobs = [(args.seed, args.batch, args.variant, k * per, k * per + per, reps)
+            for k in range(procs)]
+    with mp.get_context("fork").Pool(procs) as pool:
         t0 = time.perf_counter()
         res = pool.map(_oracle_steps, jobs)
         wall = time.perf_counter() - t0
-    all_n = sum(r[0] for r in res)
-    return dict(one=one_n / one_s, one_seconds=one_s, one_samples=one_n, all=all_n / wall,
-                all_seconds=wall, cores=cores, per=per)
+    all_rate = sum(nn / float(np.median(tt)) for nn, tt in res)
+    return dict(one=one, one_samples=one_n, all=all_rate, all_seconds=wall, cores=procs,
+                per=per, cpu_count=n_cpu, affinity=aff, quota=quota, reps=reps)
 
 
 # ---------------------------------------------------------------- the other configurations
@@ -523,15 +612,36 @@ def config_legs(args):
     return out
 
 
+# Kernel sources whose text fixes lif_solve_seq_kernel's memory traffic: the measured figure in
+# profiles/traffic.json is used only while their hash equals the one stored with it.
+TRAFFIC_SOURCES = ("bats_amd/csrc/lif_forward.hip", "bats_amd/csrc/solver.hpp",
+                   "bats_amd/csrc/common.hpp")
+
+
+def traffic_source_hash():
+    import hashlib
+    root = os.path.dirname(os.path.abspath(__file__))
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(root, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` at this workload, from the calibrated FETCH_SIZE /
-    WRITE_SIZE counters recorded in profiles/traffic.json (None when absent)."""
+    """(HBM bytes per launch, note) of `kernel` at this workload, from the calibrated
+    FETCH_SIZE / WRITE_SIZE counters recorded in profiles/traffic.json; the bytes are None when
+    the record is absent or was measured on other kernel sources than the current ones."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)[kernel]["bytes"]
+            rec = json.load(f)[kernel]
     except (OSError, KeyError, ValueError):
-        return None
+        return None, "no traffic record"
+    cur = traffic_source_hash()
+    if rec.get("source_hash") != cur:
+        return None, (f"stale: measured on sources {rec.get('source_hash')}, current {cur}")
+    return rec["bytes"], f"profiles/traffic.json, sources {cur}"
 
 
 def main():
@@ -544,8 +654,14 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--variant", choices=("active", "literal"), default="active")
-    ap.add_argument("--cpu-sample", type=int, default=64)
-    ap.add_argument("--cpu-min-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-sample", type=int, default=32,
+                    help="samples of the one-core CPU baseline")
+    ap.add_argument("--cpu-shard", type=int, default=16,
+                    help="samples per process of the all-core CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="timed CPU steps per process (after one warm-up step)")
+    ap.add_argument("--skip-literal", action="store_true",
+                    help="skip the literal-initialisation variant")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="timed steps of the plain form (default: --steps)")
@@ -577,12 +693,13 @@ def main():
     # roofline of the dominant kernel: the hidden layer's lane-per-neuron solver
     solve_ms, solve_cnt = r["kernels"]["lif_solve_seq_kernel"]
     alg_bytes = 4 * N_IN * N_HID + 4 * (B * N_HID * K_HID + B * N_HID)
+    traffic, traffic_note = measured_traffic("lif_solve_seq_kernel")
     roof = None
     if solve_ms:
         achieved = alg_bytes / (solve_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic("lif_solve_seq_kernel") if B == 256 else None,
+                "traffic": traffic if B == 256 else None, "traffic_source": traffic_note,
                 "kernel": "lif_solve_seq_kernel (784->800 layer)",
                 "kernel_avg_us": round(solve_ms * 1e3, 2), "algorithmic_bytes": alg_bytes}
 
@@ -597,6 +714,7 @@ def main():
                  "hidden_count_agreement": float(np.mean(opt_counts[0] == pr["counts"][0])),
                  "output_count_agreement": float(np.mean(opt_counts[1] == pr["counts"][1]))}
 
+    literal = None if (args.skip_literal or args.variant == "literal") else literal_bench(args)
     serve = None if args.skip_serve else serve_bench(args)
     legs = None if args.skip_configs else config_legs(args)
 
@@ -604,12 +722,15 @@ def main():
     if c is not None:
         cpu = {"value": round(c["all"], 3), "unit": "samples/s", "cores": c["cores"],
                "kind": "port",
-               "sample": f"full training step of the FP64 NumPy oracle (BLAS on 1 thread per "
-                         f"process) on shards of {c['per']} samples of the same batch, "
-                         f"{c['cores']} forked processes at once, {c['all_seconds']:.1f} s wall; "
-                         f"1 core: {c['one']:.3f} samples/s on {args.cpu_sample} samples "
-                         f"({c['one_seconds']:.1f} s)",
-               "value_1_core": round(c["one"], 3)}
+               "sample": f"full training step of the FP64 NumPy oracle (BLAS/OpenMP on 1 thread "
+                         f"per process), {c['cores']} forked processes at once, each on its own "
+                         f"shard of {c['per']} samples of the batch, median of {c['reps']} steps "
+                         f"after 1 warm-up per process ({c['all_seconds']:.1f} s wall); 1 core: "
+                         f"{c['one']:.3f} samples/s on {c['one_samples']} samples (median of "
+                         f"{c['reps']} after 1 warm-up)",
+               "value_1_core": round(c["one"], 3), "os_cpu_count": c["cpu_count"],
+               "affinity_cores": c["affinity"],
+               "cgroup_cpu_quota": None if c["quota"] is None else round(c["quota"], 2)}
 
     step_bytes = 176e6 * B / 256.0
     line = {
@@ -624,8 +745,12 @@ def main():
         "other_configs": legs,
         "stream_ms_per_step": round(r["eager_ms"], 4),
         "graph_ms_per_step": round(r["graph_ms"], 4),
+        "step_ms_hip_events": {"stream": _step_stats(r["eager_steps"]),
+                               "graph": _step_stats(r["graph_steps"])},
+        "literal_variant": literal,
         "step_hbm_fraction": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-        "spikes_per_sample": {"hidden": round(r["hid_spikes"], 1), "output": round(r["out_spikes"], 2)},
+        "spikes_per_sample": {"hidden": round(r["hid_spikes"], 1), "output": round(r["out_spikes"], 2),
+                              "hidden_last_timed_step": round(r["hid_spikes_last"], 1)},
         "kernel_avg_us": {k: (None if v[0] is None else round(v[0] * 1e3, 2))
                           for k, v in r["kernels"].items()},
     }

@@ -30,19 +30,21 @@
 namespace bats {
 
 // ------------------------------------------------------------------ transpose
+// W^T rows have Jp >= J entries (a multiple of 4); the Jp - J padding entries are written as 0.
 __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict__ W,
-                                                        float *__restrict__ WT, int J, int I) {
+                                                        float *__restrict__ WT, int J, int I,
+                                                        int Jp) {
   __shared__ float tile[32][33];
   const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int r = ty; r < 32; r += 8) {
     const int j = j0 + r, i = i0 + tx;
-    if (j < J && i < I) tile[r][tx] = W[(size_t)j * I + i];
+    tile[r][tx] = (j < J && i < I) ? W[(size_t)j * I + i] : 0.f;
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + r, j = j0 + tx;
-    if (j < J && i < I) WT[(size_t)i * J + j] = tile[tx][r];
+    if (j < Jp && i < I) WT[(size_t)i * Jp + j] = tile[tx][r];
   }
 }
 
@@ -373,25 +375,37 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
 }
 
 // ------------------------------------------------------------------ solver, lane per neuron
-// A workgroup is one sample and up to 1024 consecutive post neurons (one lane each; all of
-// them when J <= 1024, so that every wave of the sample shares one CU's caches). The 32-byte
-// FastEvent records are wave-uniform and read with scalar loads; the weights of the next
-// kGroup events are requested while the current group is processed. Per event: a and b
-// accumulate in float64, the clamp filter runs, and only lanes whose filter passes enter the
-// closed-form loop; the spike times of a lane are computed after its walk.
+// A workgroup is one sample and up to 1024 consecutive post neurons (one lane each; all of them
+// when J <= 1024, so that every wave of the sample shares one CU's caches). The 32-byte
+// FastEvent records are wave-uniform and read with scalar loads one event ahead; the weights
+// of the next kGroup events are requested while the current group is processed, through a
+// buffer resource over W^T (the lane's column is a VGPR offset, the event's row offset a
+// scalar one: no 64-bit address arithmetic per load), with the group's input indices loaded a
+// group earlier still. Per event: a and b accumulate in float64 and the clamp filter runs;
+// the branch into the closed-form loop is wave-uniform (a ballot of the lanes whose filter
+// passes), so the common event costs 11 VALU instructions, two waits, one scalar load and an
+// untaken branch. The spike times of a lane are computed after its walk.
 constexpr int kSolveMaxThreads = 1024;
 constexpr int kGroup = kEventPad;
+// descriptor word 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family targets
+constexpr int kBufferWord3 = 0x00020000;
+
+__device__ __forceinline__ float load_weight(__amdgpu_buffer_rsrc_t r, int lane_off,
+                                             int row_off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, lane_off, row_off, 0));
+}
 
 // kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every interval is solved.
 // kStaged: the event records come from a wave-private LDS stage filled one 64-event chunk
 // ahead (for launches with few waves per SIMD, whose per-event scalar loads would otherwise
 // be exposed); otherwise each record is a scalar load issued one event ahead.
+// W^T rows have Jp entries; wt_bytes (< 2^31) bounds the buffer resource.
 template <bool kAlwaysExact, bool kStaged>
 __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
     const Event *__restrict__ events, const FastEvent *__restrict__ fast,
     const int *__restrict__ evidx,
-    const int *__restrict__ counts, int Pcap, const float *__restrict__ WT, int J, int K,
-    double c, double dtt, double tau, int *__restrict__ out_n,
+    const int *__restrict__ counts, int Pcap, const float *__restrict__ WT, int J, int Jp,
+    int wt_bytes, int K, double c, double dtt, double tau, int *__restrict__ out_n,
     float *__restrict__ out_t, float *__restrict__ out_a, float *__restrict__ out_x,
     float *__restrict__ out_z) {
   __shared__ int s_n[kSolveMaxThreads];
@@ -405,7 +419,10 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
   const Event *ev = events + (size_t)b * Pcap;
   const FastEvent *fv = fast + (size_t)b * Pcap;
   const int *ix = evidx + (size_t)b * Pcap;
-  const float *wcol = WT + jc;
+  const int lane_off = jc * (int)sizeof(float);
+  const int row_bytes = Jp * (int)sizeof(float);
+  const __amdgpu_buffer_rsrc_t wres =
+      __builtin_amdgcn_make_buffer_rsrc((void *)WT, (short)0, wt_bytes, kBufferWord3);
   const double inv2c = 0.5 / c;
   const size_t row = ((size_t)b * J + jc) * K;
   // With the filter, a lane that is inactive or whose buffer is full carries a = -inf (it
@@ -414,10 +431,13 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
   int n = 0;
   bool live = active;
   float wn[kGroup];  // weights of the next group
+  int ixn[kGroup];   // input indices of the group after it
   if (Ppad > 0) {
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[u] * J];
+    for (int u = 0; u < kGroup; ++u) wn[u] = load_weight(wres, lane_off, ix[u] * row_bytes);
   }
+#pragma unroll
+  for (int u = 0; u < kGroup; ++u) ixn[u] = kGroup < Ppad ? ix[kGroup + u] : 0;
   // staged form: each wave keeps the records of the next 64 events in its own LDS slice,
   // filled with one coalesced load per chunk issued a whole chunk ahead
   __shared__ FastEvent s_stage[kStaged ? kSolveMaxThreads / 64 : 1][64];
@@ -446,7 +466,11 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
       for (int u = 0; u < kGroup; ++u) w[u] = wn[u];
       if (q0 + kGroup < Ppad) {
 #pragma unroll
-        for (int u = 0; u < kGroup; ++u) wn[u] = wcol[(size_t)ix[q0 + kGroup + u] * J];
+        for (int u = 0; u < kGroup; ++u) wn[u] = load_weight(wres, lane_off, ixn[u] * row_bytes);
+        if (q0 + 2 * kGroup < Ppad) {
+#pragma unroll
+          for (int u = 0; u < kGroup; ++u) ixn[u] = ix[q0 + 2 * kGroup + u];
+        }
       }
       const bool more = q0 + kGroup < qend;
 #pragma unroll
@@ -456,16 +480,19 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
         const double w64 = (double)w[u];
         a = fma(w64, cur.es, a);
         bb = fma(w64, cur.e, bb);
-        if (kAlwaysExact ? live : clamp_filter(a, bb, cur, c, inv2c)) {
-          const SolveStateZ r = solve_crossings_deferred(a, bb, n, cur.zq_lo, cur.zl_hi,
-                                                         ev + q0 + u, c, inv2c, dtt, tau, K,
-                                                         row, out_t, out_a, out_x, out_z);
-          bb = r.b;
-          n = r.n;
-          if (kAlwaysExact)
-            live = r.live != 0;
-          else if (!r.live)
-            a = -kInfD;
+        const bool pass = kAlwaysExact ? live : clamp_filter(a, bb, cur, c, inv2c);
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(pass) != 0, 0)) {
+          if (pass) {
+            const SolveStateZ r = solve_crossings_deferred(a, bb, n, cur.zq_lo, cur.zl_hi,
+                                                           ev + q0 + u, c, inv2c, dtt, tau, K,
+                                                           row, out_t, out_a, out_x, out_z);
+            bb = r.b;
+            n = r.n;
+            if (kAlwaysExact)
+              live = r.live != 0;
+            else if (!r.live)
+              a = -kInfD;
+          }
         }
         if (u + 1 < kGroup || more) cur = nxt;
       }
@@ -480,11 +507,13 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
   s_n[threadIdx.x] = active ? n : 0;
   if (active) out_n[(size_t)b * J + j] = n;
   __syncthreads();
+  // the +inf tail of the block's rows, streamed past the caches (no later kernel reads it
+  // before it has left L2, and it would evict the other workgroups' weights)
   const int jcount = min((int)blockDim.x, J - j0);
   float *tb = out_t + ((size_t)b * J + j0) * K;
   for (int e = threadIdx.x; e < jcount * K; e += blockDim.x) {
     const int jj = e / K;
-    if (e - jj * K >= s_n[jj]) tb[e] = kInfF;
+    if (e - jj * K >= s_n[jj]) __builtin_nontemporal_store(kInfF, tb + e);
   }
 }
 
@@ -683,8 +712,12 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   return check_launch("zero_counts");
 }
 
+// W^T row length: J rounded up to a multiple of 4 (rows start 16-byte aligned)
+static int wt_stride(int J) { return (J + 3) & ~3; }
+
 static size_t fwd_bytes(int B, int I, int K_pre, int J) {
-  return align_up(sizeof(float) * (size_t)I * J, 256) + event_lists_bytes(B, I * K_pre);
+  return align_up(sizeof(float) * (size_t)I * wt_stride(J), 256) +
+         event_lists_bytes(B, I * K_pre);
 }
 
 }  // namespace bats
@@ -710,8 +743,8 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
   BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
   BATS_REQUIRE(B <= 65535, "batch above 65535");
   BATS_REQUIRE(solver_mode >= 0 && solver_mode <= 4,
-               "solver_mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron), 3 (lane "
-               "per neuron, staged event records) or 4 (lane per neuron, scalar event records)");
+               "solver_mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron) or 3 / 4 "
+               "(lane per neuron, staged / scalar event records)");
   BATS_REQUIRE(solver_blocks >= 0, "solver_blocks must be >= 0 (0: automatic)");
   if (B == 0 || J == 0) return 0;
   BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
@@ -723,15 +756,16 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
   BATS_REQUIRE(workspace_bytes >= need, "workspace too small");
   char *ws = static_cast<char *>(workspace);
   float *WT = reinterpret_cast<float *>(ws);
-  void *ev_ws = ws + align_up(sizeof(float) * (size_t)I * J, 256);
+  const int Jp = wt_stride(J);
+  void *ev_ws = ws + align_up(sizeof(float) * (size_t)I * Jp, 256);
   const int always_exact = (c > 0.f) ? 0 : 1;
   const long long BJ = (long long)B * J;
   const bool wave_mode = solver_mode == 2 || (solver_mode == 0 && BJ < kSeqMinLanes);
   if (S > 0) {
     BATS_REQUIRE(pre_times && weights && pre_exp_tau_s && pre_exp_tau, "null input pointer");
     if (!wave_mode) {  // the wave-per-neuron solver reads W's rows directly
-      dim3 tg((I + 31) / 32, (J + 31) / 32);
-      hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I);
+      dim3 tg((I + 31) / 32, (Jp + 31) / 32);
+      hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I, Jp);
       if (int r = check_launch("transpose_kernel")) return r;
     }
   }
@@ -765,13 +799,15 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
   const long long waves = (long long)B * sg.x * (threads / 64);
   const bool staged = solver_mode == 3 ||
                       (solver_mode != 4 && waves < 3LL * 4 * (long long)cus);
+  const size_t wt_bytes = sizeof(float) * (size_t)I * Jp;
+  BATS_REQUIRE(wt_bytes < 0x7fffffffULL, "W^T above 2 GiB");
   const auto seq = always_exact ? (staged ? lif_solve_seq_kernel<true, true>
                                           : lif_solve_seq_kernel<true, false>)
                                 : (staged ? lif_solve_seq_kernel<false, true>
                                           : lif_solve_seq_kernel<false, false>);
   BATS_TIMED_IF(timed, "lif_solve_seq_kernel", st,
              hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
-                                el.counts, el.Pcap, WT, J, K_post, (double)c,
+                                el.counts, el.Pcap, WT, J, Jp, (int)wt_bytes, K_post, (double)c,
                                 (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
                                 out_z));
   return check_launch("lif_solve_seq_kernel");

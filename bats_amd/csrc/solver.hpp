@@ -36,7 +36,9 @@ __device__ __forceinline__ bool clamp_filter(double a, double b, const FastEvent
                                              double inv2c) {
   const double zv = clamp_uniform(b * inv2c, f.zq_lo, f.zl_hi);
   const double g = fma(fma(c, zv, -b), zv, a);
-  return (a > 0.0) & (g <= kFilterEps * a);
+  // (a > 0) and (g <= eps a) as one comparison, max(g - eps a, -a) <= 0; a = 0 also passes,
+  // and the closed form rejects it (its root is 0)
+  return fmax(fma(-kFilterEps, a, g), -a) <= 0.0;
 }
 
 // The reference's compute_spikes loop (compute_spike_times.cu:53-83) in float64.
@@ -194,8 +196,8 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred_ends(
     }
     if (!(valid & above & below)) break;
     const size_t slot = row + n;
-    out_a[slot] = (float)a;
-    out_x[slot] = (float)x;
+    __builtin_nontemporal_store((float)a, out_a + slot);
+    __builtin_nontemporal_store((float)x, out_x + slot);
     stash_z(slot, z, out_t, out_z);
     zlast = z;
     last_is_event = false;

@@ -29,7 +29,10 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--solver-mode", type=int, default=0,
+                    help="solver form of the timed dense forwards (bats_lif_forward_solver)")
     args = ap.parse_args()
+    sm = args.solver_mode
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(0)
     B = args.batch
@@ -73,15 +76,15 @@ def main():
 
     def run():
         if args.what == "forward1":
-            ops.lif_forward(t, w1, *c1, 1.0, 30)
+            ops.lif_forward(t, w1, *c1, 1.0, 30, solver_mode=sm)
         elif args.what == "forward2":
-            ops.lif_forward(h[1], w2, *c2, 1.0, 30)
+            ops.lif_forward(h[1], w2, *c2, 1.0, 30, solver_mode=sm)
         elif args.what == "backward1":
             ops.lif_backward(*h[:5], err1.clone(), t, h[5], h[6], w1, *c1)
         elif args.what == "backward2":
             ops.lif_backward(*o[:5], err2.clone(), h[1], o[5], o[6], w2, *c2)
         elif args.what == "forward_hh":
-            ops.lif_forward(h[1], w3, *c1, 1.0, 30)
+            ops.lif_forward(h[1], w3, *c1, 1.0, 30, solver_mode=sm)
         elif args.what == "backward_hh":
             ops.lif_backward(*hh[:5], err3.clone(), h[1], hh[5], hh[6], w3, *c1)
         elif args.what == "conv_forward1":

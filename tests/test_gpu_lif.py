@@ -232,8 +232,9 @@ def test_tied_spike_times():
 @pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_forward_both_solver_forms(mode):
     # the lane-per-neuron (automatic, staged and scalar event records) and wave-per-neuron
-    # solvers on the same hidden and output shapes, and with theta = 0 (no threshold
-    # quadratic: every interval takes the closed form); the form is chosen per call
+    # solvers on the same hidden and output shapes, an odd number of post neurons, and
+    # theta = 0 (no threshold quadratic: every interval takes the closed form); the form is
+    # chosen per call
     rng = np.random.default_rng(11)
     t, _ = latency_inputs(rng, 16, 784)
     W1 = (2.0 * rng.normal(0, 1.0, size=(800, 784))).astype(F32)
@@ -245,6 +246,9 @@ def test_forward_both_solver_forms(mode):
     check_forward(tb, W3, 0.05, 0.2, 0.2, 3, 0.6, f"buffer full mode {mode}", mode)
     W4 = rng.normal(0.2, 1.0, size=(96, 100)).astype(F32)
     check_forward(tb, W4, 0.05, 0.0, 0.3, 4, 0.6, f"theta 0 mode {mode}", mode)
+    W5 = rng.normal(0.3, 1.0, size=(97, 100)).astype(F32)
+    check_forward(tb, W5, 0.05, 0.2, 0.2, 5, 0.6, f"odd J mode {mode}", mode)
+    check_forward(tb, W5, 0.05, 0.0, 0.2, 5, 0.6, f"odd J theta 0 mode {mode}", mode)
 
 
 # ------------------------------------------------------------------ input encoding

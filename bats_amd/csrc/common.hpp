@@ -56,6 +56,25 @@ __device__ __forceinline__ bool is_inf_f(float v) { return isinf(v); }
 constexpr float kInfF = __builtin_huge_valf();
 constexpr double kInfD = __builtin_huge_val();
 
+// Descriptor word 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family targets.
+constexpr int kBufferWord3 = 0x00020000;
+
+// A float32 through a buffer resource: per-lane byte offset plus a wave-uniform byte offset
+// (held in an SGPR, so the lane's address needs no per-load arithmetic).
+__device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int lane_off,
+                                                 int uniform_off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, lane_off, uniform_off, 0));
+}
+
+// Read-only memory the same for every lane of a wave, read through scalar loads into SGPRs
+// (the constant address space): the memory must not be written while the kernel runs.
+typedef const __attribute__((address_space(4))) float uniform_f32;
+typedef const __attribute__((address_space(4))) int uniform_i32;
+
+__device__ __forceinline__ float4 uniform_load4(uniform_f32 *p, int k) {
+  return make_float4(p[4 * k], p[4 * k + 1], p[4 * k + 2], p[4 * k + 3]);
+}
+
 // One time-sorted input event of one sample, read uniformly by every lane of a wave.
 // es/e are the float32 exps exactly as stored for the backward, widened to float64. The
 // crossing filter's bounds for the interval after the event are precomputed in float64 with

@@ -20,6 +20,7 @@
 //   3. pre_err_tile_kernel lanes over pre neurons i of one sample, the same tile records
 //      walked tile by tile with the tile's block of W in LDS.
 // Layers of at most 16 post neurons (an output layer) run all of it in small_bwd_kernel.
+#include "backward_bins.hpp"
 #include "common.hpp"
 
 #include <algorithm>
@@ -45,11 +46,6 @@ constexpr int kBwdThreads = 256;
 // instead of one per spike; longer rows run the same arithmetic from global memory.
 // apply_reset = 0: the errors already hold the reset-chain result.
 constexpr int kPrepFast = 8;
-constexpr int kWT = 16;  // post rows per weight-gradient tile
-
-constexpr int kRecHead = 1;  // first spike slot of a tile record
-
-__host__ __device__ inline int tile_rec_slots(int K) { return kRecHead + kWT * K; }
 
 __global__ __launch_bounds__(kBwdThreads) void row_prep_kernel(
     int B, int J, int K, const int *__restrict__ n, const float *__restrict__ t,
@@ -659,23 +655,34 @@ __global__ __launch_bounds__(1024) void wgrad_spk_kernel(
   const int b0 = sl * b_per_slice, b1 = min(B, b0 + b_per_slice);
   const int R = tile_rec_slots(K);
   for (int e = tid; e < kWT * I; e += nth) acc[e] = 0.0;
-  // the next sample's record: the thread's entry tid (up to nth entries) in a register
-  float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (b0 < b1 && tid < R - kRecHead) rn = rec[((size_t)b0 * gj + tj) * R + kRecHead + tid];
+  // the next sample's spike count, record (the thread's entry tid, up to nth entries), pre
+  // spike count and the wave's first chunk of its pre spikes, in registers one sample ahead
+  float4 rn = make_float4(0.f, 0.f, 0.f, 0.f), Sn = make_float4(kInfF, 0.f, 0.f, 0.f);
+  int cn = 0, Pn = 0;
+  auto fetch = [&](int b) {
+    const float4 *grec = rec + ((size_t)b * gj + tj) * R;
+    cn = __float_as_int(grec[0].x);
+    if (tid < R - kRecHead) rn = grec[kRecHead + tid];
+    Pn = count[b];
+    const int sidx = wv * 64 + lane;
+    Sn = sidx < Pn ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
+  };
+  if (b0 < b1) fetch(b0);
   for (int b = b0; b < b1; ++b) {
     const float4 *grec = rec + ((size_t)b * gj + tj) * R;
     __syncthreads();  // the previous record is consumed (and acc is zeroed)
-    const int cnt = __float_as_int(grec[0].x);
+    const int cnt = cn, P = Pn;
+    const float4 Sfirst = Sn;
     if (tid < cnt) rb[tid] = rn;
     for (int e = tid + nth; e < cnt; e += nth) rb[e] = grec[kRecHead + e];
-    if (b + 1 < b1 && tid < R - kRecHead)
-      rn = rec[((size_t)(b + 1) * gj + tj) * R + kRecHead + tid];
+    if (b + 1 < b1) fetch(b + 1);
     __syncthreads();
-    const int P = count[b];
     for (int c = wv; c * 64 < P; c += nw) {
       const int sidx = c * 64 + lane;
       const bool valid = sidx < P;
-      const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
+      const float4 S = c == wv ? Sfirst
+                               : (valid ? spk[(size_t)b * cap + sidx]
+                                        : make_float4(kInfF, 0.f, 0.f, 0.f));
       double *al = acc + (valid ? __float_as_int(S.w) : 0);
       // entries eight, then four at a time (their LDS reads in one round). The record lists
       // the tile's spikes row by row and every lane reads the same entry, so a row change is
@@ -734,7 +741,8 @@ constexpr int kPreSpkWaves = 8;  // waves (64 pre spikes each) sharing one tile 
 
 __global__ __launch_bounds__(64 * kPreSpkWaves) void pre_err_spk_kernel(
     int I, int K_pre, int J, int K, int gj, int cap, const float4 *__restrict__ spk,
-    const int *__restrict__ slot, const int *__restrict__ count, const float *__restrict__ W,
+    const int *__restrict__ slot, const int *__restrict__ count, int count_stride,
+    const float *__restrict__ W,
     const float4 *__restrict__ rec, double inv_tau_s, double inv_tau,
     float *__restrict__ pre_errors) {
   constexpr int kNth = 64 * kPreSpkWaves;
@@ -745,20 +753,24 @@ __global__ __launch_bounds__(64 * kPreSpkWaves) void pre_err_spk_kernel(
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int b = blockIdx.y;
-  const int P = count[b];
+  const int P = count[(size_t)b * count_stride];
   const int s0 = blockIdx.x * kNth;
   if (s0 >= P) return;  // whole block
   const int sidx = s0 + tid;
   const bool valid = sidx < P;
   const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
-  const int i = valid ? __float_as_int(S.w) : 0;
+  // the neuron index is the low 24 bits (the binned list keeps the spike's bin above them)
+  const int i = valid ? (__float_as_int(S.w) & 0xffffff) : 0;
   const float esd = (float)((double)S.y * inv_tau_s), ed = (float)((double)S.z * inv_tau);
   float *ww = wl + wv * kWT * 64;
   const int R = tile_rec_slots(K);
-  // the next tile's record entries (tid, tid + kNth) and weights, requested one tile ahead
+  // the next tile's spike count, record entries (tid, tid + kNth) and weights, requested one
+  // tile ahead
   float4 rn[kPf];
   float wn[kWT];
+  int cn = 0;
   auto prefetch = [&](int tj) {
+    cn = __float_as_int(rec[((size_t)b * gj + tj) * R].x);
     const float4 *grec = rec + ((size_t)b * gj + tj) * R + kRecHead;
 #pragma unroll
     for (int p = 0; p < kPf; ++p) {
@@ -775,7 +787,7 @@ __global__ __launch_bounds__(64 * kPreSpkWaves) void pre_err_spk_kernel(
   for (int tj = 0; tj < gj; ++tj) {
     const float4 *grec = rec + ((size_t)b * gj + tj) * R;
     __syncthreads();  // the previous tile is consumed
-    const int cnt = __float_as_int(grec[0].x);
+    const int cnt = cn;
 #pragma unroll
     for (int p = 0; p < kPf; ++p)
       if (tid + p * kNth < cnt) rb[tid + p * kNth] = rn[p];
@@ -1137,7 +1149,11 @@ using namespace bats;
 
 extern "C" size_t bats_lif_backward_workspace_size(int B, int I, int K_pre, int J, int K_post) {
   if (B < 0 || I < 0 || K_pre < 0 || J < 0 || K_post < 0) return 0;
-  return bwd_layout(B, I, K_pre, J, K_post).total;
+  size_t total = bwd_layout(B, I, K_pre, J, K_post).total;
+  // the binned path (batch-reduced gradients of multi-spike pre layers) has its own layout
+  if (K_post >= 1 && bin_path_applies(I, K_pre, J, K_post, 0))
+    total = std::max(total, bin_layout(B, I, K_pre, J, K_post).total);
+  return total;
 }
 
 // Largest post-spike buffer the backward supports (LDS row stages of the narrow-layer and
@@ -1217,6 +1233,31 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
   }
 
   int prep_rc = 0;
+  if (bin_path_applies(I, K_pre, J, K_post, grad_mode)) {
+    // multi-spike pre layer: binned GEMMs on the matrix cores plus the in-bin pairs
+    const BinLayout Lb = bin_layout(B, I, K_pre, J, K_post);
+    BATS_REQUIRE(workspace_bytes >= Lb.total, "workspace too small");
+    BATS_REQUIRE((int64_t)I * K_pre < (int64_t)1 << 31, "pre spike buffer too large");
+    BATS_TIMED("row_prep_kernel", st,
+               prep_rc = launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c,
+                                         (double)tau, (double)delta_theta_tau,
+                                         reinterpret_cast<int *>(ws + Lb.n_eff),
+                                         reinterpret_cast<float2 *>(ws + Lb.D), nullptr));
+    if (prep_rc) return prep_rc;
+    if (want_pre) {
+      const int64_t nfill = (int64_t)B * I * K_pre;
+      hipLaunchKernelGGL(fill_kernel<float>,
+                         dim3((unsigned)std::min<int64_t>((nfill + 255) / 256, 65535)), dim3(256),
+                         0, st, pre_errors, nfill, 0.f);
+      if (int r = check_launch("fill pre errors")) return r;
+    }
+    if (int r = bins_backward(st, Lb, B, I, K_pre, J, K_post, t, pre_times, pre_exp_tau_s,
+                              pre_exp_tau, weights, 1.0 / (double)tau_s, 1.0 / (double)tau,
+                              want_pre ? pre_errors : nullptr, ws))
+      return r;
+    return reduce_slices(st, Lb.gemm_splits + Lb.walk_slices, JI, inv_b,
+                         reinterpret_cast<const float *>(ws + Lb.partial), grad);
+  }
   BATS_TIMED("row_prep_kernel", st,
              prep_rc = launch_row_prep(st, B, J, K_post, n, t, a, x, z, errors, (double)c,
                                        (double)tau, (double)delta_theta_tau, nullptr,
@@ -1281,7 +1322,7 @@ extern "C" int bats_lif_backward(void *stream, int B, int I, int K_pre, int J, i
     const int groups = (cap + 64 * kPreSpkWaves - 1) / (64 * kPreSpkWaves);
     BATS_TIMED("pre_err_spk_kernel", st,
                hipLaunchKernelGGL(pre_err_spk_kernel, dim3(groups, B), dim3(64 * kPreSpkWaves), lds,
-                                  st, I, K_pre, J, K_post, gj, cap, spk, spk_slot, spk_count,
+                                  st, I, K_pre, J, K_post, gj, cap, spk, spk_slot, spk_count, 1,
                                   weights, rec, 1.0 / (double)tau_s, 1.0 / (double)tau,
                                   pre_errors));
     if (int r = check_launch("pre_err_spk_kernel")) return r;

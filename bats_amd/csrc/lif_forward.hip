@@ -387,12 +387,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
 // untaken branch. The spike times of a lane are computed after its walk.
 constexpr int kSolveMaxThreads = 1024;
 constexpr int kGroup = kEventPad;
-// descriptor word 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family targets
-constexpr int kBufferWord3 = 0x00020000;
-
 __device__ __forceinline__ float load_weight(__amdgpu_buffer_rsrc_t r, int lane_off,
                                              int row_off) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, lane_off, row_off, 0));
+  return buffer_load_f32(r, lane_off, row_off);
 }
 
 // kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every interval is solved.

@@ -4,11 +4,23 @@ Requests are single samples given as event lists (input neuron index, spike time
 length; several spikes per input neuron are allowed. The server keeps the network's weights
 and all batch buffers resident on the device and runs batches back to back on two streams:
 while one batch computes, the next is assembled from whatever requests have arrived and
-admitted, and launched as soon as a stream is free. A request is admitted when a batch slot is
-free and its device footprint fits the memory budget; it is released when its result has been
-copied out. Each batch replays a HIP graph captured for its padded shape (batch bucket,
-events-per-request bucket, spikes-per-neuron bucket), so the host cost per batch is one packed
-copy of the batch's event lists and one graph launch.
+launched as soon as a stream is free. Each batch replays a HIP graph captured for its padded
+shape (batch bucket, events-per-request bucket, spikes-per-neuron bucket), so the host cost per
+batch is one packed copy of the batch's event lists and one graph launch.
+
+Batching is length-aware. ``length_classes`` cut the event counts into classes, each with its
+own queue. A batch takes the class of the oldest waiting request and fills up with requests of
+that class and shorter ones, oldest first: a batch's duration is set by its longest sample (one
+workgroup walks a sample's events), so short requests that are the oldest do not wait behind
+long ones. Inside a batch the requests are ordered longest first, so that the workgroups of the
+longest samples start first. A graph's shape does not depend on the classes (the event
+conversion and the solver work on the real events only), and a graph is replayed on whichever
+stream is free: a second copy of a shape is made only when two batches of it are in flight.
+
+Device memory: ``memory_budget`` bounds the bytes of the captured graphs the server holds
+(their static buffers and private memory pools, measured when a graph is made). A batch shape
+whose graph is not resident evicts idle graphs, least recently used first, to make room; when
+it still does not fit, the batch is made smaller.
 
 The host does no per-event work: a batch's event lists are concatenated into one packed list
 with row offsets, and the device converts it to per-neuron buffers. The spikes-per-input-neuron
@@ -28,6 +40,7 @@ from __future__ import annotations
 import collections
 import dataclasses
 import gc
+import heapq
 import time
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -46,7 +59,6 @@ class Request:
     times: np.ndarray    # float32 [P], ascending
     arrival: float       # host clock (seconds)
     multiplicity: int = 0  # largest per-neuron count, when known (0: not known)
-    held_bytes: int = 0
     done: float = 0.0
     pred: int = -1
     counts: Optional[np.ndarray] = None
@@ -108,6 +120,12 @@ class _GraphBatch:
             self._round_trip()
         stream.synchronize()
 
+    def release(self) -> None:
+        """Drops the graph and its device buffers (the caller returns the memory)."""
+        self.graph.reset()
+        self.graph = self.outputs = self.per = None
+        self.off = self.idx = self.t = self.zero_targets = None
+
     def _round_trip(self, total: Optional[int] = None) -> None:
         """Copies the packed lists in, replays the graph, copies the results out and records
         ``done`` (on the current stream)."""
@@ -138,16 +156,18 @@ class _GraphBatch:
 
 
 class InferenceServer:
-    """Admits requests, batches them continuously and runs them on the GPU.
+    """Admits requests, batches them continuously by length class and runs them on the GPU.
 
-    ``memory_budget`` bounds the device bytes held by admitted requests (their padded share of
-    the batch buffers); a request that does not fit waits in the queue.
+    ``memory_budget`` (bytes, optional) bounds the device memory of the captured batch graphs
+    the server holds; the weights are outside it. ``batch_growth`` is the ratio of successive
+    batch buckets (fewer distinct graphs with a larger ratio, more padding).
     """
 
     def __init__(self, layers: Sequence[LayerSpec], n_inputs: int, max_batch: int = 1024,
                  max_simulation: float = 1.0, memory_budget: Optional[int] = None,
                  device=None, min_batch_bucket: int = 8, min_events_bucket: int = 64,
-                 min_spikes_bucket: int = 8, n_streams: int = 2):
+                 min_spikes_bucket: int = 8, n_streams: int = 2, batch_growth: int = 2,
+                 length_classes: Sequence[int] = ()):
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
         self.n_inputs = int(n_inputs)
@@ -157,68 +177,179 @@ class InferenceServer:
         self.max_batch = int(max_batch)
         self.max_simulation = float(max_simulation)
         self.min_batch_bucket = int(min_batch_bucket)
+        self.batch_growth = max(2, int(batch_growth))
         # larger minimum buckets mean fewer distinct padded shapes (graphs) at the cost of
-        # padding work; a server with a known request mix sets them to its upper ends. A
-        # request with more spikes on one input neuron than the spikes bucket is served twice
-        # (the second time in a bucket that holds them).
+        # padding work. A request with more spikes on one input neuron than the spikes bucket
+        # is served twice (the second time in a bucket that holds them).
         self.min_events_bucket = int(min_events_bucket)
         self.min_spikes_bucket = int(min_spikes_bucket)
+        self.length_classes = sorted(int(v) for v in length_classes)
         self.w_dev = [ops.as_device(L.weights, torch.float32, self.device) for L in self.layers]
         self.memory_budget = memory_budget
-        self.memory_in_use = 0
+        self.memory_in_use = 0  # device bytes of the resident graphs
+        self.graphs_made = 0
+        self.graphs_evicted = 0
         # batches in flight at once (one per stream): with two, a batch is assembled and
         # launched while the previous one computes
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(n_streams)))]
-        self.graphs: Dict[Tuple[int, int, int], _GraphBatch] = {}
-        self.queue: collections.deque = collections.deque()
+        # (B, P, K_in, copy) -> graph, least recently used first
+        self.graphs: "collections.OrderedDict[Tuple[int, int, int, int], _GraphBatch]" = \
+            collections.OrderedDict()
+        self.queues: Dict[int, collections.deque] = {}
         self.inflight: List[Tuple[int, _GraphBatch]] = []
         self.completed: List[Request] = []
 
-    # -------------------------------------------------------------- accounting
-    def request_bytes(self, r: Request) -> int:
-        """Device bytes a request holds while admitted: its share of the packed event
-        buffers, the per-neuron input buffer and every layer's outputs and workspace rows."""
-        P = _bucket(len(r.indices), self.min_events_bucket)
-        K_in = _bucket(max(r.multiplicity, 1), self.min_spikes_bucket)
-        b = 8 * P + 4 * self.n_inputs * K_in * 3 + 64 * self.n_inputs * K_in
+    # -------------------------------------------------------------- shapes and memory
+    def events_class(self, r: Request) -> int:
+        """The request's length class: the number of class bounds below its event count."""
+        n = len(r.indices)
+        return sum(1 for v in self.length_classes if n > v)
+
+    def batch_bucket(self, n: int) -> int:
+        b = self.min_batch_bucket
+        while b < n:
+            b *= self.batch_growth
+        return min(b, self.max_batch)
+
+    def graph_bytes_estimate(self, B: int, P: int, K_in: int) -> int:
+        """Device bytes of a batch graph of this shape: packed lists, per-neuron inputs, every
+        layer's outputs, input exps and workspace, the prediction (with a margin for the
+        allocator's rounding)."""
+        lib = _lib.load()
+        N = self.n_inputs
+        b = 8 * B * P + 4 * (B + 1) + 4 * B * N * K_in + 4 * B * N + 4 * B
+        I, Kp = N, K_in
         for L in self.layers:
-            b += 4 * L.weights.shape[0] * (1 + 5 * L.max_n_spike)
-        return b
+            J, K = L.weights.shape[0], L.max_n_spike
+            b += 4 * B * J + 16 * B * J * K + 8 * B * I * Kp
+            b += int(lib.bats_lif_forward_workspace_size(B, I, Kp, J))
+            I, Kp = J, K
+        b += 4 * B * I + 8 * B
+        return int(b * 1.25) + (8 << 20)
 
-    def submit(self, r: Request) -> None:
-        self.queue.append(r)
+    def _resident_bytes(self) -> int:
+        return sum(g.nbytes for g in self.graphs.values())
 
-    # -------------------------------------------------------------- batching
-    def _admit(self) -> List[Request]:
-        batch: List[Request] = []
-        while self.queue and len(batch) < self.max_batch:
-            r = self.queue[0]
-            need = self.request_bytes(r)
-            if self.memory_budget is not None and need > self.memory_budget:
-                raise ValueError(f"request {r.rid} needs {need} bytes, more than the whole "
-                                 f"memory budget ({self.memory_budget})")
-            if self.memory_budget is not None and self.memory_in_use + need > self.memory_budget:
+    def _evict_for(self, need: int, keep=None) -> bool:
+        """Evicts idle graphs, least recently used first, until ``need`` more bytes fit the
+        budget. Returns whether they fit."""
+        if self.memory_budget is None:
+            return True
+        busy = {id(g) for _, g in self.inflight}
+        freed = False
+        for key in list(self.graphs.keys()):
+            if self.memory_in_use + need <= self.memory_budget:
                 break
-            self.queue.popleft()
-            self.memory_in_use += need
-            r.held_bytes = need
-            batch.append(r)
-        return batch
+            g = self.graphs[key]
+            if id(g) in busy or g is keep:
+                continue
+            del self.graphs[key]
+            self.memory_in_use -= g.nbytes
+            g.release()
+            self.graphs_evicted += 1
+            freed = True
+        if freed:
+            gc.collect()
+            torch.cuda.empty_cache()
+        return self.memory_in_use + need <= self.memory_budget
 
-    def _graph_for(self, B: int, P: int, K_in: int, stream) -> _GraphBatch:
-        key = (B, P, K_in, id(stream))
-        g = self.graphs.get(key)
-        if g is None:
-            g = _GraphBatch(self, B, P, K_in, stream)
-            self.graphs[key] = g
+    def _graph_for(self, B: int, P: int, K_in: int, si: int, copy: Optional[int] = None):
+        """An idle graph of the shape (the first idle copy, or copy ``copy``), made if there
+        is none and it fits the budget; None otherwise."""
+        busy = {id(g) for _, g in self.inflight}
+        copies = [k for k in self.graphs if k[:3] == (B, P, K_in)]
+        for key in copies:
+            if (copy is None or key[3] == copy) and id(self.graphs[key]) not in busy:
+                self.graphs.move_to_end(key)
+                return self.graphs[key]
+        if copy is None:
+            used = {k[3] for k in copies}
+            copy = next(c for c in range(len(copies) + 1) if c not in used)
+        elif (B, P, K_in, copy) in self.graphs:
+            return None
+        key = (B, P, K_in, copy)
+        need = self.graph_bytes_estimate(B, P, K_in)
+        if not self._evict_for(need):
+            return None
+        # the graph's footprint: device memory reserved by making it, with the allocator's
+        # free cache emptied before and after (the warm-up run's temporaries are released)
+        torch.cuda.synchronize(self.device)
+        gc.collect()
+        torch.cuda.empty_cache()
+        before = torch.cuda.memory_reserved(self.device)
+        g = _GraphBatch(self, B, P, K_in, self.streams[si])
+        torch.cuda.synchronize(self.device)
+        torch.cuda.empty_cache()
+        g.nbytes = max(torch.cuda.memory_reserved(self.device) - before, 1)
+        self.graphs[key] = g
+        self.memory_in_use += g.nbytes
+        self.graphs_made += 1
+        # the measured size may exceed the estimate: make room after the fact
+        self._evict_for(0, keep=g)
         return g
 
-    def _launch(self, batch: List[Request], stream) -> _GraphBatch:
-        Bb = min(_bucket(len(batch), self.min_batch_bucket), _bucket(self.max_batch, 1))
-        lens = [len(r.indices) for r in batch]
-        Pb = _bucket(max(lens), self.min_events_bucket)
+    # -------------------------------------------------------------- queues
+    def submit(self, r: Request) -> None:
+        c = self.events_class(r)
+        q = self.queues.get(c)
+        if q is None:
+            q = self.queues[c] = collections.deque()
+        q.append(r)
+
+    @property
+    def queue(self) -> List[Request]:
+        """The waiting requests (all classes)."""
+        return [r for q in self.queues.values() for r in q]
+
+    def _waiting(self) -> bool:
+        return any(self.queues.values())
+
+    def _take(self) -> List[Request]:
+        """Up to max_batch requests: the class of the oldest waiting request and the shorter
+        classes, oldest first."""
+        heads = [(q[0].arrival, q[0].rid, c) for c, q in self.queues.items() if q]
+        if not heads:
+            return []
+        c_top = min(heads)[2]
+        heap = [(q[0].arrival, q[0].rid, c) for c, q in self.queues.items() if q and c <= c_top]
+        heapq.heapify(heap)
+        batch: List[Request] = []
+        while heap and len(batch) < self.max_batch:
+            _, _, c = heapq.heappop(heap)
+            q = self.queues[c]
+            batch.append(q.popleft())
+            if q:
+                heapq.heappush(heap, (q[0].arrival, q[0].rid, c))
+        return batch
+
+    def _give_back(self, reqs: Sequence[Request]) -> None:
+        """Returns requests to the front of their queues (in their order)."""
+        for r in reversed(list(reqs)):
+            c = self.events_class(r)
+            self.queues.setdefault(c, collections.deque()).appendleft(r)
+
+    # -------------------------------------------------------------- batching
+    def _launch(self, batch: List[Request], si: int) -> Optional[_GraphBatch]:
+        batch.sort(key=lambda r: -len(r.indices))  # longest first
+        Pb = _bucket(len(batch[0].indices), self.min_events_bucket)
         Kb = _bucket(max(max(r.multiplicity for r in batch), 1), self.min_spikes_bucket)
-        g = self._graph_for(Bb, Pb, Kb, stream)
+        Bb = self.batch_bucket(len(batch))
+        g = self._graph_for(Bb, Pb, Kb, si)
+        while g is None and Bb > self.min_batch_bucket:
+            # the shape does not fit the memory budget: a smaller batch
+            Bb = max(self.min_batch_bucket, Bb // self.batch_growth)
+            g = self._graph_for(Bb, Pb, Kb, si)
+        if g is None:
+            if not self.inflight and not self.graphs:
+                raise ValueError(f"the memory budget ({self.memory_budget} bytes) cannot hold "
+                                 f"the smallest batch graph ({self.graph_bytes_estimate(Bb, Pb, Kb)} "
+                                 f"bytes estimated)")
+            self._give_back(batch)
+            return None
+        if len(batch) > Bb:
+            self._give_back(batch[Bb:])
+            batch = batch[:Bb]
+        lens = [len(r.indices) for r in batch]
         off = g.h_off.numpy()
         off[0] = 0
         np.cumsum(lens, out=off[1:len(batch) + 1])
@@ -227,7 +358,7 @@ class InferenceServer:
         if total:
             np.concatenate([r.indices for r in batch], out=g.h_idx.numpy()[:total])
             np.concatenate([r.times for r in batch], out=g.h_t.numpy()[:total])
-        with torch.cuda.stream(stream):
+        with torch.cuda.stream(self.streams[si]):
             g._round_trip(total)
         g.requests = batch
         return g
@@ -239,26 +370,21 @@ class InferenceServer:
         st = g.h_st.numpy()[:nb].copy()
         pred = g.h_pred.numpy()[:nb].tolist()
         row_max = g.h_rowmax.numpy()[:nb]
-        over = np.nonzero(row_max > g.K_in)[0]
+        over = set(np.nonzero(row_max > g.K_in)[0].tolist())
         redo = []
-        for s in over.tolist():
-            # the request's events did not all fit its bucket: serve it again with one that
-            # holds them (its slot's results above are discarded)
-            r = g.requests[s]
-            r.multiplicity = int(row_max[s])
-            redo.append(r)
-        skip = set(over.tolist())
         for s, r in enumerate(g.requests):
-            self.memory_in_use -= r.held_bytes
-            r.held_bytes = 0
-            if s in skip:
+            if s in over:
+                # the request's events did not all fit its bucket: serve it again with one
+                # that holds them (its slot's results above are discarded)
+                r.multiplicity = int(row_max[s])
+                redo.append(r)
                 continue
             r.done = now
             r.pred = pred[s]
             r.counts = n[s]
             r.spike_times = st[s]
-            self.completed.append(r)
-        self.queue.extendleft(reversed(redo))
+        self.completed.extend(r for s, r in enumerate(g.requests) if s not in over)
+        self._give_back(redo)
         g.requests = []
 
     def poll(self) -> None:
@@ -271,23 +397,27 @@ class InferenceServer:
                 still.append((si, g))
         self.inflight = still
         busy = {si for si, _ in self.inflight}
-        for si, stream in enumerate(self.streams):
-            if si in busy or not self.queue:
+        for si in range(len(self.streams)):
+            if si in busy or not self._waiting():
                 continue
-            batch = self._admit()
+            batch = self._take()
             if not batch:
                 break
-            self.inflight.append((si, self._launch(batch, stream)))
+            g = self._launch(batch, si)
+            if g is None:
+                break  # waits for memory held by batches in flight
+            self.inflight.append((si, g))
 
     def drain(self) -> None:
-        while self.queue or self.inflight:
+        while self._waiting() or self.inflight:
             self.poll()
 
-    def warmup(self, shapes: Sequence[Tuple[int, int, int]]) -> None:
-        """Capture the graphs of the given (batch, events, spikes-per-neuron) buckets."""
+    def warmup(self, shapes: Sequence[Tuple[int, int, int]], copies: int = 1) -> None:
+        """Capture the graphs of the given (batch, events, spikes-per-neuron) buckets, each in
+        ``copies`` copies."""
         for B, P, K in shapes:
-            for s in self.streams:
-                self._graph_for(B, P, K, s)
+            for c in range(copies):
+                self._graph_for(B, P, K, c % len(self.streams), copy=c)
         torch.cuda.synchronize()
 
 
@@ -304,7 +434,7 @@ def run_poisson(server: InferenceServer, requests: Sequence[Request], rate: floa
     t0 = time.perf_counter()
     arrivals = t0 + np.cumsum(gaps)
     k = 0
-    while k < len(requests) or server.queue or server.inflight:
+    while k < len(requests) or server._waiting() or server.inflight:
         now = time.perf_counter()
         while k < len(requests) and arrivals[k] <= now:
             requests[k].arrival = arrivals[k]

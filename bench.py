@@ -305,22 +305,27 @@ def serve_bench(args):
     _, _, _, w1, w2 = make_inputs(args.seed, 8, args.variant)
     layers = [LayerSpec(w1, TAU_S, THETA_H, THETA_H, K_HID),
               LayerSpec(w2, TAU_S, THETA_O, THETA_O, K_OUT)]
-    # padded shapes: events to 2048, spikes per input neuron to 16 (the request mix's upper
-    # ends), batches to powers of two
+    # length classes cut at 256, 512 and 1024 events; padded shapes: 2048 events and 16 spikes
+    # per input neuron (the request mix's upper ends), batches of 8, 32, 128, 512 and
+    # max_batch, two copies of each (one per stream); the captured graphs are held within the
+    # memory budget
     srv = InferenceServer(layers, N_IN, max_batch=args.serve_batch, max_simulation=T_MAX,
-                          min_events_bucket=2048, min_spikes_bucket=16)
+                          min_events_bucket=2048, min_spikes_bucket=16, batch_growth=4,
+                          length_classes=(256, 512, 1024),
+                          memory_budget=int(args.serve_memory_gib * (1 << 30)))
     seed = [args.seed + 1]
 
     def requests(count):
         seed[0] += 1
         return make_requests(count, n_inputs=N_IN, p_min=50, p_max=2000, seed=seed[0])
 
-    Bb = 8
-    shapes = []
-    while Bb <= args.serve_batch:
-        shapes.append((Bb, 2048, 16))
-        Bb *= 2
-    srv.warmup(shapes)
+    shapes, Bb = [], 8
+    while True:
+        shapes.append((min(Bb, args.serve_batch), 2048, 16))
+        if Bb >= args.serve_batch:
+            break
+        Bb *= 4
+    srv.warmup(shapes, copies=2)
     agreement = plain_count_agreement(args, layers, requests(256), srv)
     # saturated throughput
     reqs = requests(args.serve_requests)
@@ -358,6 +363,12 @@ def serve_bench(args):
             "bytes_per_request": round(per_req, 1), "mean_events_per_request": round(p_mean, 1)}
     return {"saturated_samples_per_s": round(sat, 1), "max_batch": args.serve_batch,
             "requests": "P ~ U{50..2000} events, idx ~ U{0..783} with replacement, t ~ U[0,1]",
+            "batching": "length classes cut at 256/512/1024 events (oldest request's class "
+                        "plus shorter ones), longest first; batch buckets 8/32/128/512/max_batch; "
+                        "two streams",
+            "memory_budget_gib": args.serve_memory_gib,
+            "graphs_resident_gib": round(srv.memory_in_use / (1 << 30), 2),
+            "graphs_made": srv.graphs_made, "graphs_evicted": srv.graphs_evicted,
             "roofline": roof, "count_agreement_with_plain": agreement, "loads": loads}
 
 
@@ -667,7 +678,9 @@ def main():
                     help="timed steps of the plain form (default: --steps)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
-    ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-batch", type=int, default=1024)
+    ap.add_argument("--serve-memory-gib", type=float, default=48.0,
+                    help="device memory budget of the server's captured batch graphs")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--configs", default="ttfs,deep,conv",

@@ -1,6 +1,6 @@
 """The continuous-batching server: every request's result is bitwise the same alone and in
-a mixed batch, matches the FP64 oracle (identical counts, times within 1e-5 relative), and
-admission never exceeds the memory budget."""
+a mixed batch, matches the FP64 oracle (identical counts, times within 1e-5 relative), batches
+follow the length classes, and the resident graphs stay within the memory budget."""
 import numpy as np
 import pytest
 import torch
@@ -50,24 +50,70 @@ def test_alone_equals_mixed_and_oracle():
         assert np.all(np.abs(t_m[m] - o.t[0][m]) <= 1e-5 * np.abs(o.t[0][m]))
 
 
-def test_memory_budget_admission():
+def test_memory_budget_bounds_graphs_and_results_hold():
+    """The budget bounds the device memory of the resident batch graphs, measured when they
+    are made: with room for about two graphs, a mix of lengths (graphs for 128 to 1024 events)
+    and batch sizes makes the server evict least-recently-used graphs and shrink batches, its device
+    memory never exceeds what it holds by more than the graph being made, and every result
+    equals that of an unbounded server."""
     from bats_amd.serving import InferenceServer, make_requests
     layers = _layers()
-    reqs = make_requests(30, p_min=50, p_max=300, seed=4)
-    probe = InferenceServer(layers, 784, max_batch=64)
-    for r in reqs:
-        probe.submit(r)
-    budget = 5 * max(probe.request_bytes(r) for r in reqs)
-    srv = InferenceServer(layers, 784, max_batch=64, memory_budget=budget)
-    for r in make_requests(30, p_min=50, p_max=300, seed=4):
+    free = InferenceServer(layers, 784, max_batch=64, min_events_bucket=128,
+                           length_classes=(256, 512))
+    for r in make_requests(120, p_min=50, p_max=1000, seed=4):
+        free.submit(r)
+    free.drain()
+    ref = {r.rid: (r.counts, r.spike_times, r.pred) for r in free.completed}
+    del free
+    probe = InferenceServer(layers, 784, max_batch=64, min_events_bucket=128)
+    budget = 2 * probe.graph_bytes_estimate(32, 1024, 8)
+    del probe
+    srv = InferenceServer(layers, 784, max_batch=64, memory_budget=budget,
+                          min_events_bucket=128, length_classes=(256, 512))
+    base = torch.cuda.memory_reserved()
+    for r in make_requests(120, p_min=50, p_max=1000, seed=4):
         srv.submit(r)
-    peak = 0
-    while srv.queue or srv.inflight:
+    while srv._waiting() or srv.inflight:
         srv.poll()
-        peak = max(peak, srv.memory_in_use)
         assert srv.memory_in_use <= budget
-    assert len(srv.completed) == 30
-    assert srv.memory_in_use == 0
+        assert srv.memory_in_use == sum(g.nbytes for g in srv.graphs.values())
+        assert torch.cuda.memory_reserved() - base <= budget + (64 << 20)
+    assert len(srv.completed) == 120
+    assert srv.graphs_evicted > 0
+    for r in srv.completed:
+        n, t, p = ref[r.rid]
+        assert np.array_equal(r.counts, n) and np.array_equal(r.spike_times, t) and r.pred == p
+
+
+def test_length_classes_and_longest_first():
+    """Batches take the class of the oldest request and shorter ones only; results are the
+    same as in arrival-order batches."""
+    from bats_amd.serving import InferenceServer, make_requests
+    layers = _layers()
+    srv = InferenceServer(layers, 784, max_batch=16, length_classes=(256, 512, 700))
+    reqs = make_requests(40, p_min=50, p_max=900, seed=8)
+    for k, r in enumerate(reqs):
+        r.arrival = float(k)
+        srv.submit(r)
+    assert len(srv.queues) == 4
+    batch = srv._take()
+    oldest = srv.events_class(reqs[0])
+    assert 0 < oldest < 3  # (the seed's first request is of a middle class)
+    assert batch[0] is reqs[0]
+    assert all(srv.events_class(r) <= oldest for r in batch)
+    assert [r.arrival for r in batch] == sorted(r.arrival for r in batch)
+    srv._give_back(batch)
+    assert sorted(r.rid for r in srv.queue) == list(range(40))
+    srv.drain()
+    assert len(srv.completed) == 40
+    one = InferenceServer(layers, 784, max_batch=1, min_batch_bucket=1)
+    for r in make_requests(40, p_min=50, p_max=900, seed=8)[:8]:
+        one.submit(r)
+        one.drain()
+    got = {r.rid: r for r in srv.completed}
+    for r in one.completed:
+        assert np.array_equal(got[r.rid].counts, r.counts)
+        assert np.array_equal(got[r.rid].spike_times, r.spike_times)
 
 
 def test_spikes_bucket_overflow_is_served_again():
@@ -86,7 +132,7 @@ def test_spikes_bucket_overflow_is_served_again():
     for r in make_requests(48, p_min=50, p_max=900, seed=6):
         tight.submit(r)
     tight.drain()
-    assert len(tight.completed) == 48 and tight.memory_in_use == 0
+    assert len(tight.completed) == 48
     redone = 0
     for r in tight.completed:
         true_mult = int(np.bincount(r.indices, minlength=784).max())

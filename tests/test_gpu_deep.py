@@ -6,7 +6,9 @@ autograd Functions.
 Each layer's backward receives the errors the GPU propagated from the layer above (not the
 oracle's), so the comparison covers the error carried through the whole chain. Bar, for every
 layer: spike counts identical; spike times within 1e-5 relative; weight gradients within
-1e-4 |ref| + 1e-6 max|ref| elementwise. The per-layer errors are printed."""
+1e-4 |ref| + 1e-6 max|ref| elementwise. The per-layer errors are printed. A third test checks
+the chained autograd gradients against central finite differences on 64 weights of the four
+layers."""
 import numpy as np
 import pytest
 import torch
@@ -134,3 +136,93 @@ def test_deep_network_autograd_chain():
         assert np.array_equal(outs[k].cpu().numpy(), states[k].n), f"layer {k}: counts differ"
         g = Wd[k].grad.cpu().numpy().astype(np.float64)
         assert_close_rel(g, ref_grads[k], 1e-4, 1e-6, f"autograd layer {k} weight gradient")
+
+
+def test_deep_network_finite_differences():
+    """Central finite differences on 64 random weights of the deep network (16 per layer, drawn
+    among the weights whose gradient is at least 1 % of their layer's largest,
+    h = 1e-4 (|w| + 0.1)), through the chained autograd Functions, of a smooth loss of every
+    layer's spike times: L = sum over layers and slots of e * t with fixed random e (float64
+    sum of the float32 times), batch 8.
+
+    A stencil is compared when every layer's spike counts are those of the unperturbed
+    network on both sides, some float32 spike time moved (where none does, g h is below the
+    times' resolution), and the two one-sided slopes agree (a spike that moves across an
+    input spike time anywhere downstream makes the loss only piecewise smooth; the central
+    difference then averages two slopes). The bar is the single-layer test's: 2e-2 |g| plus
+    the rounding the float32 times carry into the difference, sum |e| ulp(t) / (2h) over the
+    spikes whose times differ between the two sides."""
+    from bats_amd.autograd import lif
+
+    rng = np.random.default_rng(33)
+    Bf = 8
+    t, _ = latency_inputs(rng, Bf, SIZES[0])
+    Ws = [rng.normal(0.0, 1.0, size=(SIZES[k + 1], SIZES[k])).astype(F32) for k in range(4)]
+    es = [rng.normal(size=(Bf, SIZES[k + 1], K)) for k in range(4)]
+    x0 = torch.as_tensor(t, device="cuda")
+    e_dev = [torch.as_tensor(e, device="cuda") for e in es]
+
+    def run(Wlist, grad=False):
+        Wd = [torch.as_tensor(W, device="cuda").requires_grad_(grad) for W in Wlist]
+        x, L, times, counts = x0, 0.0, [], []
+        for k, (W, th) in enumerate(zip(Wd, THETAS)):
+            x, nk = lif(x, W, tau_s=TAU_S, theta=th, delta_theta=th, max_n_spike=K,
+                        t_max=T_MAX)
+            L = L + torch.where(torch.isfinite(x), x.double() * e_dev[k],
+                                torch.zeros_like(x, dtype=torch.float64)).sum()
+            times.append(x.detach().cpu().numpy())
+            counts.append(nk.cpu().numpy())
+        if grad:
+            L.backward()
+            return float(L.item()), times, counts, [W.grad.cpu().numpy() for W in Wd]
+        return float(L.item()), times, counts
+
+    L0, t0, n0, grads = run(Ws, grad=True)
+    assert all(c.sum() > 0 for c in n0)
+    # 16 random weights per layer among those whose gradient is at least 1 % of the layer's
+    # largest (a finite difference with h = 1e-4 cannot resolve much smaller slopes through
+    # float32 spike times)
+    picks = []
+    for k in range(4):
+        ok = np.argwhere(np.abs(grads[k]) >= 1e-2 * np.abs(grads[k]).max())
+        for r in rng.choice(len(ok), size=16, replace=False):
+            picks.append((k, int(ok[r][0]), int(ok[r][1])))
+    checked = [0] * 4
+    skipped_counts = skipped_kinks = skipped_flat = 0
+    for k, j, i in picks:
+        W = Ws[k]
+        h = np.float32(1e-4 * (abs(W[j, i]) + 0.1))
+        Wp = [w.copy() for w in Ws]
+        Wm = [w.copy() for w in Ws]
+        Wp[k][j, i] += h
+        Wm[k][j, i] -= h
+        Lp, tp, np_ = run(Wp)
+        Lm, tm, nm = run(Wm)
+        if not all(np.array_equal(a, b) and np.array_equal(c, b)
+                   for a, b, c in zip(np_, n0, nm)):
+            skipped_counts += 1
+            continue
+        hp = float(Wp[k][j, i]) - float(W[j, i])
+        hm = float(W[j, i]) - float(Wm[k][j, i])
+        dw = hp + hm
+        noise, moved = 0.0, 0
+        for a, b, e in zip(tp, tm, es):
+            ch = np.isfinite(a) & (a != b)
+            moved += int(ch.sum())
+            noise += float(np.sum(np.abs(e[ch]) * np.spacing(np.abs(a[ch]).astype(F32))))
+        noise /= dw
+        if moved == 0:
+            # no float32 spike time changed: g * 2h is below the times' resolution here
+            skipped_flat += 1
+            continue
+        sp, sm = (Lp - L0) / hp, (L0 - Lm) / hm
+        g = float(grads[k][j, i])
+        if abs(sp - sm) > 2e-2 * max(abs(sp), abs(sm)) + 4 * noise:
+            skipped_kinks += 1
+            continue
+        fd = (Lp - Lm) / dw
+        assert abs(fd - g) <= 2e-2 * abs(g) + noise, (k, j, i, fd, g, noise)
+        checked[k] += 1
+    print(f"[deep fd] checked per layer {checked}, skipped: counts changed {skipped_counts}, "
+          f"kinks {skipped_kinks}, no time moved {skipped_flat}", flush=True)
+    assert sum(checked) >= 48 and min(checked) >= 8

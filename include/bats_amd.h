@@ -308,6 +308,20 @@ int bats_events_csr_to_per_neuron(void *stream, int B, int N, int K, const int *
                                   const int *indices, const float *times, float *out_times,
                                   int *out_counts, int *out_row_max);
 
+/*
+ * Host-side helper of an inference server (no GPU work, no counterpart in the reference, which
+ * has no server): gathers n event lists from a host arena into one packed list.
+ * Request r's events are src_indices/src_times[src_offsets[r] .. + lengths[r]); they are
+ * copied back to back into dst_indices/dst_times, and row_offsets [n+1] receives their
+ * offsets (the layout bats_events_csr_to_per_neuron reads). The copies run on a small pool of
+ * host threads kept for the life of the process. Fails if the total exceeds dst_capacity
+ * events. Calls from several host threads are serialised.
+ */
+int bats_pack_event_lists(int n, const int64_t *src_offsets, const int32_t *lengths,
+                          const int32_t *src_indices, const float *src_times,
+                          int32_t *row_offsets, int32_t *dst_indices, float *dst_times,
+                          int64_t dst_capacity);
+
 /* ------------------------------------------------------------------ observability */
 
 /*

@@ -39,8 +39,8 @@ def test_alone_equals_mixed_and_oracle():
         assert np.array_equal(r.counts, n_m)
         assert np.array_equal(r.spike_times, t_m)
         assert r.pred == p_m
-    # oracle on a few requests
-    for r in reqs[:4]:
+    # oracle on a few requests (their events again: a served request's leave the arena)
+    for r in make_requests(40, p_min=50, p_max=600, seed=3)[:4]:
         per, _ = oev.events_to_per_neuron(r.indices[None, :], r.times[None, :], 784)
         h = olif.lif_forward(per, layers[0].weights, 0.130, 1.0, 1.0, 30, tmax=1.0)
         o = olif.lif_forward(h.t, layers[1].weights, 0.130, 1.3, 1.3, 30, tmax=1.0)
@@ -134,8 +134,9 @@ def test_spikes_bucket_overflow_is_served_again():
     tight.drain()
     assert len(tight.completed) == 48
     redone = 0
+    events = {r.rid: r.indices for r in make_requests(48, p_min=50, p_max=900, seed=6)}
     for r in tight.completed:
-        true_mult = int(np.bincount(r.indices, minlength=784).max())
+        true_mult = int(np.bincount(events[r.rid], minlength=784).max())
         if true_mult > 1:
             assert r.multiplicity == true_mult
             redone += 1

@@ -166,6 +166,57 @@ __device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0) {
   return src;
 }
 
+// Writes event q of a sample's time-ordered list (P events, then padding up to a multiple of
+// kEventPad): the Event record, its 32-byte fast form and its input index. t is the event's
+// time, tn the next event's (or +inf), idx its input neuron; es32 / e32 receive the float32
+// exps (as stored for the backward) of a real event.
+__device__ __forceinline__ void write_event(int q, int P, float t, float tn, int idx, double dtau_s,
+                                            double dtau, double dc, float tmax, Event *ev,
+                                            FastEvent *fv, int *ix, float &es32, float &e32) {
+  Event e;
+  if (q < P) {
+    const float tl = fminf(tn, tmax);
+    const double es64 = exp((double)t / dtau_s);
+    const double zq = exp((double)t / dtau);
+    const double zl = fmin(exp((double)tl / dtau), 1e300);
+    es32 = (float)es64;
+    e32 = (float)zq;
+    e.es = (double)es32;
+    e.e = (double)e32;
+    e.czl2 = dc * zl * zl;
+    e.zl1 = zl * (1.0 + kFilterEps);
+    e.blo = 2.0 * dc * zq * (1.0 - kFilterEps);
+    e.bhi = 2.0 * dc * zl * (1.0 + kFilterEps);
+    e.zq = zq;
+    e.zl = zl;
+    e.idx = idx;
+    e.t = t;
+    e.tl = tl;
+  } else {
+    // padding: no contribution, never a candidate
+    e.es = 0.0;
+    e.e = 0.0;
+    e.czl2 = kInfD;
+    e.zl1 = 0.0;
+    e.blo = kInfD;
+    e.bhi = -kInfD;
+    e.zq = kInfD;
+    e.zl = kInfD;
+    e.idx = 0;
+    e.t = kInfF;
+    e.tl = kInfF;
+  }
+  e.pad = 0.f;
+  ev[q] = e;
+  FastEvent f;
+  f.es = e.es;
+  f.e = e.e;
+  f.zq_lo = (q < P) ? e.zq * (1.0 - kFilterEps) : kInfD;  // padding: never a candidate
+  f.zl_hi = (q < P) ? e.zl * (1.0 + kFilterEps) : kInfD;
+  fv[q] = f;
+  ix[q] = e.idx;
+}
+
 // One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
 // sample's Event list padded to a multiple of kEventPad, the event indices, the count, and
 // the per-slot float32 exps the backward reads. Up to 1024 events each wave sorts 128 keys in
@@ -323,53 +374,58 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
   const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
   const int Ppad = (P + kEventPad - 1) / kEventPad * kEventPad;
   for (int q = threadIdx.x; q < Ppad; q += blockDim.x) {
-    Event e;
+    float t = kInfF, tn = kInfF, es32 = 0.f, e32 = 0.f;
+    int s = 0;
     if (q < P) {
       const uint64_t key = keys[q];
-      const float t = key_float((uint32_t)(key >> 32));
-      const int s = (int)(uint32_t)key;
-      const float tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
-      const float tl = fminf(tn, tmax);
-      const double es64 = exp((double)t / dtau_s);
-      const double zq = exp((double)t / dtau);
-      const double zl = fmin(exp((double)tl / dtau), 1e300);
-      const float es32 = (float)es64, e32 = (float)zq;
-      e.es = (double)es32;
-      e.e = (double)e32;
-      e.czl2 = dc * zl * zl;
-      e.zl1 = zl * (1.0 + kFilterEps);
-      e.blo = 2.0 * dc * zq * (1.0 - kFilterEps);
-      e.bhi = 2.0 * dc * zl * (1.0 + kFilterEps);
-      e.zq = zq;
-      e.zl = zl;
-      e.idx = s / K_pre;
-      e.t = t;
-      e.tl = tl;
+      t = key_float((uint32_t)(key >> 32));
+      s = (int)(uint32_t)key;
+      tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
+    }
+    write_event(q, P, t, tn, s / K_pre, dtau_s, dtau, dc, tmax, ev, fv, ix, es32, e32);
+    if (q < P) {
       pre_exp_s[(size_t)b * S + s] = es32;
       pre_exp[(size_t)b * S + s] = e32;
-    } else {
-      // padding: no contribution, never a candidate
-      e.es = 0.0;
-      e.e = 0.0;
-      e.czl2 = kInfD;
-      e.zl1 = 0.0;
-      e.blo = kInfD;
-      e.bhi = -kInfD;
-      e.zq = kInfD;
-      e.zl = kInfD;
-      e.idx = 0;
-      e.t = kInfF;
-      e.tl = kInfF;
     }
-    e.pad = 0.f;
-    ev[q] = e;
-    FastEvent f;
-    f.es = e.es;
-    f.e = e.e;
-    f.zq_lo = (q < P) ? e.zq * (1.0 - kFilterEps) : kInfD;  // padding: never a candidate
-    f.zl_hi = (q < P) ? e.zl * (1.0 + kFilterEps) : kInfD;
-    fv[q] = f;
-    ix[q] = e.idx;
+  }
+  if (threadIdx.x == 0) counts[b] = P;
+}
+
+// One workgroup per sample of a packed list: sample b's events are
+// indices/times[row_offsets[b] .. row_offsets[b+1]) in time order, and an index outside
+// [0, I) or a time that is not finite ends the list. They become the sample's Event list directly (no per-neuron buffer,
+// no sort): events of equal time keep their list order.
+__global__ __launch_bounds__(256) void packed_events_kernel(
+    const int *__restrict__ row_offsets, const int *__restrict__ indices,
+    const float *__restrict__ times, int I, float tau_s, float tau, float c, float tmax,
+    int P_max, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
+    int *__restrict__ evidx, int *__restrict__ counts) {
+  __shared__ int s_end;
+  const int b = blockIdx.x;
+  const int o0 = row_offsets[b];
+  const int n = min(max(row_offsets[b + 1] - o0, 0), P_max);  // events past P_max are dropped
+  if (threadIdx.x == 0) s_end = n;
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int i = indices[o0 + q];
+    if (i < 0 || i >= I || !isfinite(times[o0 + q])) atomicMin(&s_end, q);
+  }
+  __syncthreads();
+  const int P = s_end;
+  const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
+  const int Ppad = (P + kEventPad - 1) / kEventPad * kEventPad;
+  Event *ev = events + (size_t)b * Pcap;
+  FastEvent *fv = fast + (size_t)b * Pcap;
+  int *ix = evidx + (size_t)b * Pcap;
+  for (int q = threadIdx.x; q < Ppad; q += blockDim.x) {
+    float t = kInfF, tn = kInfF, es32, e32;
+    int i = 0;
+    if (q < P) {
+      t = times[o0 + q];
+      i = indices[o0 + q];
+      tn = q + 1 < P ? times[o0 + q + 1] : kInfF;
+    }
+    write_event(q, P, t, tn, i, dtau_s, dtau, dc, tmax, ev, fv, ix, es32, e32);
   }
   if (threadIdx.x == 0) counts[b] = P;
 }
@@ -728,6 +784,56 @@ extern "C" size_t bats_lif_forward_workspace_size(int B, int I, int K_pre, int J
 
 // The forward of one dense layer. solver_mode / solver_blocks choose the solver form and the
 // lane solver's workgroups per sample (0: automatic); `timed` allows library timing events.
+// The solver launch over a batch's event lists (el): one wave per (sample, post neuron) in
+// wave_mode (W read row by row), else one lane per pair over W^T (WT, rows of Jp).
+static int solve_event_lists(hipStream_t st, int B, int I, int J, int K_post,
+                             const EventLists &el, bool any_events, const float *weights,
+                             const float *WT, int Jp, bool wave_mode, int solver_mode,
+                             int solver_blocks, float c, float delta_theta_tau, float tau,
+                             int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                             bool timed) {
+  const int always_exact = (c > 0.f) ? 0 : 1;
+  const long long BJ = (long long)B * J;
+  if (wave_mode && any_events) {
+    BATS_REQUIRE(BJ < 2147483647LL, "too many (sample, neuron) pairs");
+    const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));
+    BATS_TIMED_IF(timed, "lif_solve_wave_kernel", st,
+               hipLaunchKernelGGL(lif_solve_wave_kernel, dim3(blocks), dim3(kWaveThreads), 0, st,
+                                  el.events, el.idx, el.counts, el.Pcap, weights, I, J, (int)BJ,
+                                  K_post,
+                                  (double)c, (double)delta_theta_tau, (double)tau, always_exact,
+                                  out_n, out_t, out_a, out_x, out_z));
+    return check_launch("lif_solve_wave_kernel");
+  }
+  // blocks per sample, and lanes per block rounded up to whole waves; a batch with fewer
+  // samples than the device has CUs spreads each sample's neurons over more workgroups (of
+  // at least one wave), so that the launch reaches every CU
+  int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
+  const int cus = cu_count();
+  if ((long long)B * nblk < cus) nblk = std::max(nblk, std::min((cus + B - 1) / B, (J + 63) / 64));
+  if (solver_blocks > 0) nblk = std::max(1, std::min(solver_blocks, (J + 63) / 64));
+  const int per = (J + nblk - 1) / nblk;
+  const int threads = (per + 63) / 64 * 64;
+  dim3 sg((J + threads - 1) / threads, B);
+  // staged event records when the launch has fewer than 3 waves per SIMD (4 SIMDs per CU):
+  // with more, the other waves hide the per-event scalar loads and staging costs a little
+  const long long waves = (long long)B * sg.x * (threads / 64);
+  const bool staged = solver_mode == 3 ||
+                      (solver_mode != 4 && waves < 3LL * 4 * (long long)cus);
+  const size_t wt_bytes = sizeof(float) * (size_t)I * Jp;
+  BATS_REQUIRE(wt_bytes < 0x7fffffffULL, "W^T above 2 GiB");
+  const auto seq = always_exact ? (staged ? lif_solve_seq_kernel<true, true>
+                                          : lif_solve_seq_kernel<true, false>)
+                                : (staged ? lif_solve_seq_kernel<false, true>
+                                          : lif_solve_seq_kernel<false, false>);
+  BATS_TIMED_IF(timed, "lif_solve_seq_kernel", st,
+             hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
+                                el.counts, el.Pcap, WT, J, Jp, (int)wt_bytes, K_post, (double)c,
+                                (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
+                                out_z));
+  return check_launch("lif_solve_seq_kernel");
+}
+
 static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_post,
                             const float *pre_times, const float *weights, float tau_s,
                             float tau, float c, float delta_theta_tau, float max_simulation,
@@ -770,44 +876,9 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
   if (int r = build_event_lists(st, B, S, K_pre, pre_times, tau_s, tau, c, max_simulation,
                                 pre_exp_tau_s, pre_exp_tau, ev_ws, &el, timed))
     return r;
-  if (wave_mode && S > 0) {
-    BATS_REQUIRE(BJ < 2147483647LL, "too many (sample, neuron) pairs");
-    const unsigned blocks = (unsigned)((BJ + (kWaveThreads / 64) - 1) / (kWaveThreads / 64));
-    BATS_TIMED_IF(timed, "lif_solve_wave_kernel", st,
-               hipLaunchKernelGGL(lif_solve_wave_kernel, dim3(blocks), dim3(kWaveThreads), 0, st,
-                                  el.events, el.idx, el.counts, el.Pcap, weights, I, J, (int)BJ,
-                                  K_post,
-                                  (double)c, (double)delta_theta_tau, (double)tau, always_exact,
-                                  out_n, out_t, out_a, out_x, out_z));
-    return check_launch("lif_solve_wave_kernel");
-  }
-  // blocks per sample, and lanes per block rounded up to whole waves; a batch with fewer
-  // samples than the device has CUs spreads each sample's neurons over more workgroups (of
-  // at least one wave), so that the launch reaches every CU
-  int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
-  const int cus = cu_count();
-  if ((long long)B * nblk < cus) nblk = std::max(nblk, std::min((cus + B - 1) / B, (J + 63) / 64));
-  if (solver_blocks > 0) nblk = std::max(1, std::min(solver_blocks, (J + 63) / 64));
-  const int per = (J + nblk - 1) / nblk;
-  const int threads = (per + 63) / 64 * 64;
-  dim3 sg((J + threads - 1) / threads, B);
-  // staged event records when the launch has fewer than 3 waves per SIMD (4 SIMDs per CU):
-  // with more, the other waves hide the per-event scalar loads and staging costs a little
-  const long long waves = (long long)B * sg.x * (threads / 64);
-  const bool staged = solver_mode == 3 ||
-                      (solver_mode != 4 && waves < 3LL * 4 * (long long)cus);
-  const size_t wt_bytes = sizeof(float) * (size_t)I * Jp;
-  BATS_REQUIRE(wt_bytes < 0x7fffffffULL, "W^T above 2 GiB");
-  const auto seq = always_exact ? (staged ? lif_solve_seq_kernel<true, true>
-                                          : lif_solve_seq_kernel<true, false>)
-                                : (staged ? lif_solve_seq_kernel<false, true>
-                                          : lif_solve_seq_kernel<false, false>);
-  BATS_TIMED_IF(timed, "lif_solve_seq_kernel", st,
-             hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
-                                el.counts, el.Pcap, WT, J, Jp, (int)wt_bytes, K_post, (double)c,
-                                (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
-                                out_z));
-  return check_launch("lif_solve_seq_kernel");
+  return solve_event_lists(st, B, I, J, K_post, el, S > 0, weights, WT, Jp, wave_mode,
+                           solver_mode, solver_blocks, c, delta_theta_tau, tau, out_n, out_t,
+                           out_a, out_x, out_z, timed);
 }
 
 extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
@@ -833,6 +904,78 @@ extern "C" int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, in
                           delta_theta_tau, max_simulation, out_n, out_t, out_a, out_x, out_z,
                           pre_exp_tau_s, pre_exp_tau, solver_mode, solver_blocks, true,
                           workspace, workspace_bytes);
+}
+
+// ------------------------------------------------------------------ forward from packed lists
+// The first layer of an inference server: its inputs arrive as packed, time-ordered event
+// lists, so the Event lists are written from them directly (packed_events_kernel) instead of
+// going through per-neuron buffers and the per-sample sort. Inference only: no per-slot exps
+// are written for a backward.
+static size_t packed_lists_bytes(int B, int P_max) {
+  const int Pcap = event_capacity(P_max);
+  size_t off = 0;
+  off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  off = align_up(off + sizeof(FastEvent) * (size_t)B * Pcap, 256);
+  off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
+  off = align_up(off + sizeof(int) * (size_t)B, 256);
+  return off;
+}
+
+extern "C" size_t bats_lif_forward_packed_workspace_size(int B, int I, int P_max, int J) {
+  if (B < 0 || I < 0 || P_max < 0 || J < 0) return 0;
+  return align_up(sizeof(float) * (size_t)I * wt_stride(J), 256) + packed_lists_bytes(B, P_max);
+}
+
+extern "C" int bats_lif_forward_packed(void *stream, int B, int I, int J, int K_post, int P_max,
+                                       const int *row_offsets, const int *indices,
+                                       const float *times, const float *weights, float tau_s,
+                                       float tau, float c, float delta_theta_tau,
+                                       float max_simulation, int *out_n, float *out_t,
+                                       float *out_a, float *out_x, float *out_z,
+                                       void *workspace, size_t workspace_bytes) {
+  BATS_REQUIRE(B >= 0 && I >= 0 && J >= 0 && P_max >= 0, "negative size");
+  BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
+  BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
+  BATS_REQUIRE(B <= 65535, "batch above 65535");
+  BATS_REQUIRE((int64_t)B * event_capacity(P_max) < ((int64_t)1 << 31), "too many events");
+  if (B == 0 || J == 0) return 0;
+  BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
+  BATS_REQUIRE(row_offsets && weights, "null input pointer");
+  BATS_REQUIRE(P_max == 0 || (indices && times), "null input pointer");
+  BATS_REQUIRE(tau_s > 0.f && tau > 0.f, "tau_s and tau must be positive");
+  hipStream_t st = as_stream(stream);
+  BATS_REQUIRE(workspace != nullptr, "null workspace");
+  BATS_REQUIRE(workspace_bytes >= bats_lif_forward_packed_workspace_size(B, I, P_max, J),
+               "workspace too small");
+  char *ws = static_cast<char *>(workspace);
+  float *WT = reinterpret_cast<float *>(ws);
+  const int Jp = wt_stride(J);
+  const bool wave_mode = (long long)B * J < kSeqMinLanes;
+  const bool any = P_max > 0 && I > 0;
+  if (any && !wave_mode) {
+    dim3 tg((I + 31) / 32, (Jp + 31) / 32);
+    hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I, Jp);
+    if (int r = check_launch("transpose_kernel")) return r;
+  }
+  EventLists el;
+  const int Pcap = event_capacity(P_max);
+  char *base = ws + align_up(sizeof(float) * (size_t)I * Jp, 256);
+  size_t off = 0;
+  el.events = reinterpret_cast<Event *>(base + off);
+  off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
+  el.fast = reinterpret_cast<FastEvent *>(base + off);
+  off = align_up(off + sizeof(FastEvent) * (size_t)B * Pcap, 256);
+  el.idx = reinterpret_cast<int *>(base + off);
+  off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
+  el.counts = reinterpret_cast<int *>(base + off);
+  el.Pcap = Pcap;
+  BATS_TIMED("packed_events_kernel", st,
+             hipLaunchKernelGGL(packed_events_kernel, dim3(B), dim3(256), 0, st, row_offsets,
+                                indices, times, any ? I : 0, tau_s, tau, c, max_simulation, P_max,
+                                Pcap, el.events, el.fast, el.idx, el.counts));
+  if (int r = check_launch("packed_events_kernel")) return r;
+  return solve_event_lists(st, B, I, J, K_post, el, any, weights, WT, Jp, wave_mode, 0, 0, c,
+                           delta_theta_tau, tau, out_n, out_t, out_a, out_x, out_z, true);
 }
 
 // ------------------------------------------------------------------ forward without allocation

@@ -89,6 +89,40 @@ def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float3
     return n, t, a, x, z, es, e
 
 
+def lif_forward_packed(row_offsets: torch.Tensor, indices: torch.Tensor, times: torch.Tensor,
+                       P_max: int, weights: torch.Tensor, tau_s, tau, c, delta_theta_tau,
+                       max_simulation: float, max_n_post: int, out=None):
+    """Dense LIF forward of a first layer fed by packed, time-ordered event lists (inference):
+    sample b's events are ``indices/times[row_offsets[b]:row_offsets[b+1]]``, at most
+    ``P_max`` each; an index outside [0, I) or a non-finite time ends a list. Returns
+    (n, t, a, x, z) as :func:`lif_forward` (no pre exps). ``out`` may hold those five tensors
+    (written in place, for graph capture)."""
+    _require_gpu()
+    dev = weights.device
+    off = as_device(row_offsets, I32, dev)
+    idx = as_device(indices, I32, dev)
+    tim = as_device(times, F32, dev)
+    B = int(off.shape[0]) - 1
+    J, I = int(weights.shape[0]), int(weights.shape[1])
+    K = int(max_n_post)
+    if out is None:
+        out = (torch.empty((B, J), dtype=I32, device=dev),
+               torch.empty((B, J, K), dtype=F32, device=dev),
+               torch.empty((B, J, K), dtype=F32, device=dev),
+               torch.empty((B, J, K), dtype=F32, device=dev),
+               torch.empty((B, J, K), dtype=F32, device=dev))
+    n, t, a, x, z = out
+    lib = _lib.load()
+    ws_bytes = lib.bats_lif_forward_packed_workspace_size(B, I, int(P_max), J)
+    ws = _workspace(ws_bytes, dev)
+    _lib.check(lib.bats_lif_forward_packed(
+        _stream(), B, I, J, K, int(P_max), _ptr(off), _ptr(idx), _ptr(tim),
+        _ptr(weights, F32), float(tau_s), float(tau), float(c), float(delta_theta_tau),
+        float(max_simulation), _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), _ptr(ws),
+        ws_bytes))
+    return n, t, a, x, z
+
+
 def lif_backward(n, t, a, x, z, errors, pre_times, pre_es, pre_e, weights, tau_s, tau, c,
                  delta_theta_tau, per_sample: bool = False, pre_trainable: bool = True,
                  batch_sum: bool = False):

@@ -83,6 +83,27 @@ int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, int J, int K_
                             void *workspace, size_t workspace_bytes);
 
 /*
+ * bats_lif_forward for a first layer fed by packed event lists, for inference: sample b's
+ * input events are indices/times[row_offsets[b] .. row_offsets[b+1]) (row_offsets [B+1] int32
+ * on the device), in ascending time, at most P_max per sample (later ones are dropped); an
+ * index outside [0, I) or a time that is not finite ends a sample's list. The events become the
+ * solver's time-ordered event lists directly, without the per-neuron buffers and the per-sample
+ * sort of bats_lif_forward (events of equal time keep their list order; bats_lif_forward
+ * orders them by input neuron, which can change float64 sums in their last bits). No per-slot
+ * exps are written: there is no backward from this entry point. Outputs as bats_lif_forward.
+ * Same reference interfaces (bats/Layers/LIFLayer.py:63-91, and the event-list conversion of
+ * bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33 that it makes
+ * unnecessary).
+ */
+size_t bats_lif_forward_packed_workspace_size(int B, int I, int P_max, int J);
+int bats_lif_forward_packed(void *stream, int B, int I, int J, int K_post, int P_max,
+                            const int *row_offsets, const int *indices, const float *times,
+                            const float *weights, float tau_s, float tau, float c,
+                            float delta_theta_tau, float max_simulation,
+                            int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                            void *workspace, size_t workspace_bytes);
+
+/*
  * bats_lif_forward without a workspace argument and without any allocation: the workspace is
  * a file-scope device array of bats_lif_forward_noalloc_max_bytes() bytes (64 MiB) in the
  * library, one per GPU, held until the process ends. No counterpart in the reference: it is

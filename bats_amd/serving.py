@@ -5,8 +5,8 @@ length; several spikes per input neuron are allowed. The server keeps the networ
 and all batch buffers resident on the device and runs batches back to back on two streams:
 while one batch computes, the next is assembled from whatever requests have arrived and
 launched as soon as a stream is free. Each batch replays a HIP graph captured for its padded
-shape (batch bucket, events-per-request bucket, spikes-per-neuron bucket), so the host cost per
-batch is one packed copy of the batch's event lists and one graph launch.
+shape (batch bucket, events-per-request bucket), so the host cost per batch is one packed copy
+of the batch's event lists and one graph launch.
 
 Batching is length-aware. ``length_classes`` cut the event counts into classes, each with its
 own queue. A batch takes the class of the oldest waiting request and fills up with requests of
@@ -24,26 +24,27 @@ it still does not fit, the batch is made smaller.
 
 The host does no per-event work in Python: a request's events live in the server's pinned
 host arena from submission (``new_request`` hands out arena space for a producer to fill;
-``submit`` copies other arrays in), a batch's event lists are gathered back to back into the
-batch's pinned buffer by the library's host thread pool (``bats_pack_event_lists``), and the
-device converts the packed list to per-neuron buffers. The spikes-per-input-neuron
-bucket is the server's minimum bucket unless a request is known to need more; the device
-reports every sample's largest per-neuron count, and a request whose events did not all fit
-is served again in a batch with a bucket that holds them (its first result is discarded).
+``submit`` copies other arrays in), and a batch's event lists are gathered back to back into
+the batch's pinned buffer by the library's host thread pool (``bats_pack_event_lists``). The
+first layer reads the packed lists directly (``bats_lif_forward_packed``: a request's events
+are already in time order, so no per-neuron buffers and no sort), so any number of events per
+input neuron is served in one pass.
 
 The inference of a request is independent of the other requests of its batch (each sample is
 sorted and solved on its own), so a request's spike trains are the same alone and in any mix.
 
 Reference counterparts: bats/Network.py:32-36 (forward), bats/Losses/SpikeCountLoss.py:17-18
-(predict), bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33 (event
-list to per-neuron buffers). The reference has no server; this is new.
+(predict), bats/CudaKernels/Wrappers/Inference/convert_to_spike_per_neuron.py:20-33 (the event
+lists the first layer reads). The reference has no server; this is new.
 """
 from __future__ import annotations
 
+import bisect
 import collections
 import dataclasses
 import gc
 import heapq
+import operator
 import time
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -55,18 +56,56 @@ from ._constants import lif_constants
 from .ops import _ptr
 
 
-@dataclasses.dataclass
 class Request:
-    rid: int
-    indices: np.ndarray  # int32 [P], sorted by time
-    times: np.ndarray    # float32 [P], ascending
-    arrival: float       # host clock (seconds)
-    multiplicity: int = 0  # largest per-neuron count, when known (0: not known)
-    done: float = 0.0
-    block: Optional[list] = None  # [start, end, free] of its events in the server's arena
-    pred: int = -1
-    counts: Optional[np.ndarray] = None
-    spike_times: Optional[np.ndarray] = None
+    """One inference request: ``indices`` (int32 [P]) and ``times`` (float32 [P], ascending)
+    of its input events, its ``arrival`` on the host clock (seconds). Once served: ``counts``
+    (int32 [N_out]), ``spike_times`` (float32 [N_out, K], +inf past the count), ``pred`` and
+    ``done`` (host clock), read from its batch's result arrays (a request keeps them alive);
+    ``indices`` and ``times`` are then None (their arena space is reused). While the events
+    are in a server's arena, ``indices`` and ``times`` are views of it, made on each access."""
+
+    __slots__ = ("rid", "arrival", "n_events", "block", "cls", "_data", "_out", "_slot")
+
+    def __init__(self, rid: int, indices: np.ndarray, times: np.ndarray, arrival: float = 0.0):
+        self.rid = rid
+        self.arrival = arrival
+        self.n_events = len(indices)
+        self.block: Optional[list] = None  # [start, end, free] of its events in the arena
+        self.cls = -1
+        # (indices, times) of the caller, or the arena that holds them (at ``block``)
+        self._data = (indices, times)
+        self._out: Optional[tuple] = None  # (counts, spike_times, preds, done) of its batch
+        self._slot = 0
+
+    @property
+    def indices(self) -> Optional[np.ndarray]:
+        d = self._data
+        if d is None or d.__class__ is tuple:
+            return None if d is None else d[0]
+        return d.idx[self.block[0]:self.block[1]]
+
+    @property
+    def times(self) -> Optional[np.ndarray]:
+        d = self._data
+        if d is None or d.__class__ is tuple:
+            return None if d is None else d[1]
+        return d.t[self.block[0]:self.block[1]]
+
+    @property
+    def done(self) -> float:
+        return 0.0 if self._out is None else self._out[3]
+
+    @property
+    def pred(self) -> int:
+        return -1 if self._out is None else self._out[2][self._slot]
+
+    @property
+    def counts(self) -> Optional[np.ndarray]:
+        return None if self._out is None else self._out[0][self._slot]
+
+    @property
+    def spike_times(self) -> Optional[np.ndarray]:
+        return None if self._out is None else self._out[1][self._slot]
 
 
 @dataclasses.dataclass
@@ -76,6 +115,10 @@ class LayerSpec:
     theta: float
     delta_theta: float
     max_n_spike: int
+
+
+_n_events = operator.attrgetter("n_events")
+_NO_EVENTS = (np.zeros(0, np.int32), np.zeros(0, np.float32))
 
 
 def _bucket(v: int, minimum: int) -> int:
@@ -140,10 +183,10 @@ class _Arena:
 
 class _GraphBatch:
     """Static buffers + captured graph for one padded batch shape: room for B requests of up
-    to P events each (packed back to back), K spikes per input neuron."""
+    to P events each (packed back to back)."""
 
-    def __init__(self, server: "InferenceServer", B: int, P: int, K_in: int, stream):
-        self.B, self.P, self.K_in = B, P, K_in
+    def __init__(self, server: "InferenceServer", B: int, P: int, stream):
+        self.B, self.P = B, P
         dev = server.device
         self.stream = stream
         self.h_off = torch.zeros((B + 1,), dtype=torch.int32, pin_memory=True)
@@ -152,10 +195,6 @@ class _GraphBatch:
         self.off = torch.zeros((B + 1,), dtype=torch.int32, device=dev)
         self.idx = torch.full((B * P,), -1, dtype=torch.int32, device=dev)
         self.t = torch.full((B * P,), float("inf"), dtype=torch.float32, device=dev)
-        N = server.n_inputs
-        self.per = (torch.empty((B, N, K_in), dtype=torch.float32, device=dev),
-                    torch.empty((B, N), dtype=torch.int32, device=dev),
-                    torch.empty((B,), dtype=torch.int32, device=dev))
         n_out = server.layers[-1].weights.shape[0]
         self.zero_targets = torch.zeros((B, n_out), dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
@@ -168,7 +207,6 @@ class _GraphBatch:
         self.h_n = torch.empty(self.outputs[0].shape, dtype=torch.int32, pin_memory=True)
         self.h_st = torch.empty(self.outputs[1].shape, dtype=torch.float32, pin_memory=True)
         self.h_pred = torch.empty(self.outputs[2].shape, dtype=torch.int64, pin_memory=True)
-        self.h_rowmax = torch.empty((B,), dtype=torch.int32, pin_memory=True)
         self.done = torch.cuda.Event()
         self.requests: List[Request] = []
         # the first use of a graph and of its host buffers in copies carries one-time set-up
@@ -180,7 +218,7 @@ class _GraphBatch:
     def release(self) -> None:
         """Drops the graph and its device buffers (the caller returns the memory)."""
         self.graph.reset()
-        self.graph = self.outputs = self.per = None
+        self.graph = self.outputs = None
         self.off = self.idx = self.t = self.zero_targets = None
 
     def _round_trip(self, total: Optional[int] = None) -> None:
@@ -195,21 +233,19 @@ class _GraphBatch:
         self.h_n.copy_(self.outputs[0], non_blocking=True)
         self.h_st.copy_(self.outputs[1], non_blocking=True)
         self.h_pred.copy_(self.outputs[2], non_blocking=True)
-        self.h_rowmax.copy_(self.outputs[3], non_blocking=True)
         self.done.record()
 
     def _run(self, server):
-        per_t, _, row_max = ops.events_csr_to_per_neuron(self.off, self.idx, self.t,
-                                                         server.n_inputs, self.K_in,
-                                                         out=self.per)
-        x = per_t
-        n = None
-        for L, w in zip(server.layers, server.w_dev):
+        L0, w0 = server.layers[0], server.w_dev[0]
+        n, x, *_ = ops.lif_forward_packed(self.off, self.idx, self.t, self.P, w0,
+                                          *lif_constants(L0.tau_s, L0.theta, L0.delta_theta),
+                                          server.max_simulation, L0.max_n_spike)
+        for L, w in zip(server.layers[1:], server.w_dev[1:]):
             n, x, *_ = ops.lif_forward(x, w, *lif_constants(L.tau_s, L.theta, L.delta_theta),
                                        server.max_simulation, L.max_n_spike)
         _, _, pred = ops.count_loss(n, x.shape[2], targets=self.zero_targets, want_loss=False,
                                     want_errors=False, want_pred=True)
-        return n, x, pred, row_max
+        return n, x, pred
 
 
 class InferenceServer:
@@ -223,7 +259,7 @@ class InferenceServer:
     def __init__(self, layers: Sequence[LayerSpec], n_inputs: int, max_batch: int = 1024,
                  max_simulation: float = 1.0, memory_budget: Optional[int] = None,
                  device=None, min_batch_bucket: int = 8, min_events_bucket: int = 64,
-                 min_spikes_bucket: int = 8, n_streams: int = 2, batch_growth: int = 2,
+                 n_streams: int = 2, batch_growth: int = 2,
                  length_classes: Sequence[int] = (), arena_events: int = 1 << 24):
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.layers = list(layers)
@@ -236,10 +272,8 @@ class InferenceServer:
         self.min_batch_bucket = int(min_batch_bucket)
         self.batch_growth = max(2, int(batch_growth))
         # larger minimum buckets mean fewer distinct padded shapes (graphs) at the cost of
-        # padding work. A request with more spikes on one input neuron than the spikes bucket
-        # is served twice (the second time in a bucket that holds them).
+        # larger packed buffers (the kernels work on the real events only)
         self.min_events_bucket = int(min_events_bucket)
-        self.min_spikes_bucket = int(min_spikes_bucket)
         self.length_classes = sorted(int(v) for v in length_classes)
         # accepted requests' event lists (pinned host memory, 8 bytes per event)
         self.arena = _Arena(arena_events)
@@ -251,7 +285,7 @@ class InferenceServer:
         # batches in flight at once (one per stream): with two, a batch is assembled and
         # launched while the previous one computes
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(n_streams)))]
-        # (B, P, K_in, copy) -> graph, least recently used first
+        # (B, P, copy) -> graph, least recently used first
         self.graphs: "collections.OrderedDict[Tuple[int, int, int, int], _GraphBatch]" = \
             collections.OrderedDict()
         self.queues: Dict[int, collections.deque] = {}
@@ -261,8 +295,7 @@ class InferenceServer:
     # -------------------------------------------------------------- shapes and memory
     def events_class(self, r: Request) -> int:
         """The request's length class: the number of class bounds below its event count."""
-        n = len(r.indices)
-        return sum(1 for v in self.length_classes if n > v)
+        return bisect.bisect_left(self.length_classes, r.n_events)
 
     def batch_bucket(self, n: int) -> int:
         b = self.min_batch_bucket
@@ -270,18 +303,21 @@ class InferenceServer:
             b *= self.batch_growth
         return min(b, self.max_batch)
 
-    def graph_bytes_estimate(self, B: int, P: int, K_in: int) -> int:
-        """Device bytes of a batch graph of this shape: packed lists, per-neuron inputs, every
-        layer's outputs, input exps and workspace, the prediction (with a margin for the
+    def graph_bytes_estimate(self, B: int, P: int) -> int:
+        """Device bytes of a batch graph of this shape: packed lists, every layer's outputs and
+        workspace (and the later layers' input exps), the prediction (with a margin for the
         allocator's rounding)."""
         lib = _lib.load()
         N = self.n_inputs
-        b = 8 * B * P + 4 * (B + 1) + 4 * B * N * K_in + 4 * B * N + 4 * B
-        I, Kp = N, K_in
-        for L in self.layers:
+        b = 8 * B * P + 4 * (B + 1)
+        I, Kp = N, 0
+        for k, L in enumerate(self.layers):
             J, K = L.weights.shape[0], L.max_n_spike
-            b += 4 * B * J + 16 * B * J * K + 8 * B * I * Kp
-            b += int(lib.bats_lif_forward_workspace_size(B, I, Kp, J))
+            b += 4 * B * J + 16 * B * J * K
+            if k == 0:
+                b += int(lib.bats_lif_forward_packed_workspace_size(B, I, P, J))
+            else:
+                b += 8 * B * I * Kp + int(lib.bats_lif_forward_workspace_size(B, I, Kp, J))
             I, Kp = J, K
         b += 4 * B * I + 8 * B
         return int(b * 1.25) + (8 << 20)
@@ -312,22 +348,22 @@ class InferenceServer:
             torch.cuda.empty_cache()
         return self.memory_in_use + need <= self.memory_budget
 
-    def _graph_for(self, B: int, P: int, K_in: int, si: int, copy: Optional[int] = None):
+    def _graph_for(self, B: int, P: int, si: int, copy: Optional[int] = None):
         """An idle graph of the shape (the first idle copy, or copy ``copy``), made if there
         is none and it fits the budget; None otherwise."""
         busy = {id(g) for _, g in self.inflight}
-        copies = [k for k in self.graphs if k[:3] == (B, P, K_in)]
+        copies = [k for k in self.graphs if k[:2] == (B, P)]
         for key in copies:
-            if (copy is None or key[3] == copy) and id(self.graphs[key]) not in busy:
+            if (copy is None or key[2] == copy) and id(self.graphs[key]) not in busy:
                 self.graphs.move_to_end(key)
                 return self.graphs[key]
         if copy is None:
-            used = {k[3] for k in copies}
+            used = {k[2] for k in copies}
             copy = next(c for c in range(len(copies) + 1) if c not in used)
-        elif (B, P, K_in, copy) in self.graphs:
+        elif (B, P, copy) in self.graphs:
             return None
-        key = (B, P, K_in, copy)
-        need = self.graph_bytes_estimate(B, P, K_in)
+        key = (B, P, copy)
+        need = self.graph_bytes_estimate(B, P)
         if not self._evict_for(need):
             return None
         # the graph's footprint: device memory reserved by making it, with the allocator's
@@ -336,7 +372,7 @@ class InferenceServer:
         gc.collect()
         torch.cuda.empty_cache()
         before = torch.cuda.memory_reserved(self.device)
-        g = _GraphBatch(self, B, P, K_in, self.streams[si])
+        g = _GraphBatch(self, B, P, self.streams[si])
         torch.cuda.synchronize(self.device)
         torch.cuda.empty_cache()
         g.nbytes = max(torch.cuda.memory_reserved(self.device) - before, 1)
@@ -357,21 +393,22 @@ class InferenceServer:
         if blk is None:
             raise MemoryError(f"the request arena ({self.arena.capacity} events) has no room "
                               f"for {n_events} events")
-        return Request(rid=rid, indices=self.arena.idx[blk[0]:blk[1]],
-                       times=self.arena.t[blk[0]:blk[1]], arrival=arrival, block=blk)
+        r = Request(rid, _NO_EVENTS[0], _NO_EVENTS[1], arrival)
+        r.n_events, r.block, r._data = int(n_events), blk, self.arena
+        return r
 
     def submit(self, r: Request) -> None:
         if r.block is None:  # events outside the arena: copy them in
-            n = len(r.indices)
+            idx, tm = r._data
+            n = r.n_events = len(idx)
             blk = self.arena.alloc(n)
             if blk is None:
                 raise MemoryError(f"the request arena ({self.arena.capacity} events) has no "
                                   f"room for {n} events")
-            self.arena.idx[blk[0]:blk[1]] = r.indices
-            self.arena.t[blk[0]:blk[1]] = r.times
-            r.indices, r.times, r.block = (self.arena.idx[blk[0]:blk[1]],
-                                           self.arena.t[blk[0]:blk[1]], blk)
-        c = self.events_class(r)
+            self.arena.idx[blk[0]:blk[1]] = idx
+            self.arena.t[blk[0]:blk[1]] = tm
+            r.block, r._data = blk, self.arena
+        c = r.cls = self.events_class(r)
         q = self.queues.get(c)
         if q is None:
             q = self.queues[c] = collections.deque()
@@ -387,14 +424,21 @@ class InferenceServer:
 
     def _take(self) -> List[Request]:
         """Up to max_batch requests: the class of the oldest waiting request and the shorter
-        classes, oldest first."""
+        classes, oldest first (all of them, in class order, when they fit one batch: the
+        batch is ordered by length at launch anyway)."""
         heads = [(q[0].arrival, q[0].rid, c) for c, q in self.queues.items() if q]
         if not heads:
             return []
         c_top = min(heads)[2]
+        eligible = [q for c, q in self.queues.items() if q and c <= c_top]
+        batch: List[Request] = []
+        if sum(len(q) for q in eligible) <= self.max_batch:
+            for q in eligible:
+                batch.extend(q)
+                q.clear()
+            return batch
         heap = [(q[0].arrival, q[0].rid, c) for c, q in self.queues.items() if q and c <= c_top]
         heapq.heapify(heap)
-        batch: List[Request] = []
         while heap and len(batch) < self.max_batch:
             _, _, c = heapq.heappop(heap)
             q = self.queues[c]
@@ -406,24 +450,22 @@ class InferenceServer:
     def _give_back(self, reqs: Sequence[Request]) -> None:
         """Returns requests to the front of their queues (in their order)."""
         for r in reversed(list(reqs)):
-            c = self.events_class(r)
-            self.queues.setdefault(c, collections.deque()).appendleft(r)
+            self.queues.setdefault(r.cls, collections.deque()).appendleft(r)
 
     # -------------------------------------------------------------- batching
     def _launch(self, batch: List[Request], si: int) -> Optional[_GraphBatch]:
-        batch.sort(key=lambda r: -len(r.indices))  # longest first
-        Pb = _bucket(len(batch[0].indices), self.min_events_bucket)
-        Kb = _bucket(max(max(r.multiplicity for r in batch), 1), self.min_spikes_bucket)
+        batch.sort(key=_n_events, reverse=True)  # longest first (stable)
+        Pb = _bucket(batch[0].n_events, self.min_events_bucket)
         Bb = self.batch_bucket(len(batch))
-        g = self._graph_for(Bb, Pb, Kb, si)
+        g = self._graph_for(Bb, Pb, si)
         while g is None and Bb > self.min_batch_bucket:
             # the shape does not fit the memory budget: a smaller batch
             Bb = max(self.min_batch_bucket, Bb // self.batch_growth)
-            g = self._graph_for(Bb, Pb, Kb, si)
+            g = self._graph_for(Bb, Pb, si)
         if g is None:
             if not self.inflight and not self.graphs:
                 raise ValueError(f"the memory budget ({self.memory_budget} bytes) cannot hold "
-                                 f"the smallest batch graph ({self.graph_bytes_estimate(Bb, Pb, Kb)} "
+                                 f"the smallest batch graph ({self.graph_bytes_estimate(Bb, Pb)} "
                                  f"bytes estimated)")
             self._give_back(batch)
             return None
@@ -432,7 +474,7 @@ class InferenceServer:
             batch = batch[:Bb]
         nb = len(batch)
         starts = np.fromiter((r.block[0] for r in batch), dtype=np.int64, count=nb)
-        lens = np.fromiter((r.block[1] - r.block[0] for r in batch), dtype=np.int32, count=nb)
+        lens = np.fromiter(map(_n_events, batch), dtype=np.int32, count=nb)
         off = g.h_off.numpy()
         _lib.check(_lib.load().bats_pack_event_lists(
             nb, starts.ctypes.data, lens.ctypes.data, self.arena.idx_ptr, self.arena.t_ptr,
@@ -445,33 +487,16 @@ class InferenceServer:
         return g
 
     def _retire(self, g: _GraphBatch) -> None:
-        now = time.perf_counter()
         nb = len(g.requests)
-        n = g.h_n.numpy()[:nb].copy()
-        st = g.h_st.numpy()[:nb].copy()
-        pred = g.h_pred.numpy()[:nb].tolist()
-        row_max = g.h_rowmax.numpy()[:nb]
-        over = np.nonzero(row_max > g.K_in)[0].tolist()
-        redo = []
-        for s in over:
-            # the request's events did not all fit its bucket: serve it again with one that
-            # holds them (its slot's results below are discarded)
-            r = g.requests[s]
-            r.multiplicity = int(row_max[s])
-            redo.append(r)
-        done = [r for s, r in enumerate(g.requests) if s not in set(over)] if over \
-            else g.requests
-        slots = [s for s in range(nb) if s not in set(over)] if over else range(nb)
-        for s, r in zip(slots, done):
-            r.done = now
-            r.pred = pred[s]
-            r.counts = n[s]
-            r.spike_times = st[s]
+        out = (g.h_n.numpy()[:nb].copy(), g.h_st.numpy()[:nb].copy(),
+               g.h_pred.numpy()[:nb].tolist(), time.perf_counter())
+        for s, r in enumerate(g.requests):
+            r._out = out
+            r._slot = s
             r.block[2] = True  # its arena space is free (returned in order below)
-            r.block = r.indices = r.times = None
+            r.block = r._data = None
         self.arena.release()
-        self.completed.extend(done)
-        self._give_back(redo)
+        self.completed.extend(g.requests)
         g.requests = []
 
     def poll(self) -> None:
@@ -499,12 +524,12 @@ class InferenceServer:
         while self._waiting() or self.inflight:
             self.poll()
 
-    def warmup(self, shapes: Sequence[Tuple[int, int, int]], copies: int = 1) -> None:
-        """Capture the graphs of the given (batch, events, spikes-per-neuron) buckets, each in
-        ``copies`` copies."""
-        for B, P, K in shapes:
+    def warmup(self, shapes: Sequence[Tuple[int, int]], copies: int = 1) -> None:
+        """Capture the graphs of the given (batch, events) buckets, each in ``copies``
+        copies."""
+        for B, P in shapes:
             for c in range(copies):
-                self._graph_for(B, P, K, c % len(self.streams), copy=c)
+                self._graph_for(B, P, c % len(self.streams), copy=c)
         torch.cuda.synchronize()
 
 

@@ -305,12 +305,11 @@ def serve_bench(args):
     _, _, _, w1, w2 = make_inputs(args.seed, 8, args.variant)
     layers = [LayerSpec(w1, TAU_S, THETA_H, THETA_H, K_HID),
               LayerSpec(w2, TAU_S, THETA_O, THETA_O, K_OUT)]
-    # length classes cut at 256, 512 and 1024 events; padded shapes: 2048 events and 16 spikes
-    # per input neuron (the request mix's upper ends), batches of 8, 32, 128, 512 and
-    # max_batch, two copies of each (one per stream); the captured graphs are held within the
-    # memory budget
+    # length classes cut at 256, 512 and 1024 events; padded shapes: 2048 events (the request
+    # mix's upper end), batches of 8, 32, 128, 512 and max_batch, two copies of each (one per
+    # stream); the captured graphs are held within the memory budget
     srv = InferenceServer(layers, N_IN, max_batch=args.serve_batch, max_simulation=T_MAX,
-                          min_events_bucket=2048, min_spikes_bucket=16, batch_growth=4,
+                          min_events_bucket=2048, batch_growth=4,
                           length_classes=(256, 512, 1024),
                           memory_budget=int(args.serve_memory_gib * (1 << 30)),
                           arena_events=1 << 25)
@@ -325,7 +324,7 @@ def serve_bench(args):
 
     shapes, Bb = [], 8
     while True:
-        shapes.append((min(Bb, args.serve_batch), 2048, 16))
+        shapes.append((min(Bb, args.serve_batch), 2048))
         if Bb >= args.serve_batch:
             break
         Bb *= 4

@@ -52,7 +52,7 @@ def test_alone_equals_mixed_and_oracle():
 
 def test_memory_budget_bounds_graphs_and_results_hold():
     """The budget bounds the device memory of the resident batch graphs, measured when they
-    are made: with room for about two graphs, a mix of lengths (graphs for 128 to 1024 events)
+    are made: with room for about one 32-request graph, a mix of lengths (graphs for 128 to 1024 events)
     and batch sizes makes the server evict least-recently-used graphs and shrink batches, its device
     memory never exceeds what it holds by more than the graph being made, and every result
     equals that of an unbounded server."""
@@ -66,7 +66,7 @@ def test_memory_budget_bounds_graphs_and_results_hold():
     ref = {r.rid: (r.counts, r.spike_times, r.pred) for r in free.completed}
     del free
     probe = InferenceServer(layers, 784, max_batch=64, min_events_bucket=128)
-    budget = 2 * probe.graph_bytes_estimate(32, 1024, 8)
+    budget = int(1.3 * probe.graph_bytes_estimate(32, 1024))
     del probe
     srv = InferenceServer(layers, 784, max_batch=64, memory_budget=budget,
                           min_events_bucket=128, length_classes=(256, 512))
@@ -116,30 +116,37 @@ def test_length_classes_and_longest_first():
         assert np.array_equal(got[r.rid].spike_times, r.spike_times)
 
 
-def test_spikes_bucket_overflow_is_served_again():
-    # with a spikes-per-input-neuron bucket of 1, every request with two events on one input
-    # neuron overflows on the device and is served again in a bucket that holds it; results
-    # equal those of a server whose bucket holds every request at once
-    from bats_amd.serving import InferenceServer, make_requests
+def test_many_events_on_one_input():
+    """The first layer reads the packed lists directly, so any number of events per input
+    neuron is served in one pass: a request with 300 events on one input neuron and one with
+    40 events on each of 20 neurons, in a mix, against the oracle and against themselves alone."""
+    from bats_amd.serving import InferenceServer, Request, make_requests
     layers = _layers()
-    wide = InferenceServer(layers, 784, max_batch=32, min_spikes_bucket=16)
-    for r in make_requests(48, p_min=50, p_max=900, seed=6):
-        wide.submit(r)
-    wide.drain()
-    ref = {r.rid: (r.counts, r.spike_times, r.pred) for r in wide.completed}
-    assert all(r.multiplicity == 0 for r in wide.completed)
-    tight = InferenceServer(layers, 784, max_batch=32, min_spikes_bucket=1)
-    for r in make_requests(48, p_min=50, p_max=900, seed=6):
-        tight.submit(r)
-    tight.drain()
-    assert len(tight.completed) == 48
-    redone = 0
-    events = {r.rid: r.indices for r in make_requests(48, p_min=50, p_max=900, seed=6)}
-    for r in tight.completed:
-        true_mult = int(np.bincount(events[r.rid], minlength=784).max())
-        if true_mult > 1:
-            assert r.multiplicity == true_mult
-            redone += 1
-        n, t, p = ref[r.rid]
-        assert np.array_equal(r.counts, n) and np.array_equal(r.spike_times, t) and r.pred == p
-    assert redone > 10
+    rng = np.random.default_rng(9)
+    special = []
+    for k, (neurons, per) in enumerate(((np.array([5]), 300), (np.arange(20) * 37, 40))):
+        idx = np.repeat(neurons, per).astype(np.int32)
+        tm = rng.uniform(0.0, 1.0, size=len(idx)).astype(F32)
+        order = np.argsort(tm, kind="stable")
+        special.append((idx[order], tm[order]))
+    mixed = InferenceServer(layers, 784, max_batch=32)
+    for r in make_requests(20, p_min=50, p_max=900, seed=6):
+        mixed.submit(r)
+    for k, (idx, tm) in enumerate(special):
+        mixed.submit(Request(rid=100 + k, indices=idx.copy(), times=tm.copy(), arrival=0.0))
+    mixed.drain()
+    got = {r.rid: r for r in mixed.completed}
+    assert len(got) == 22
+    alone = InferenceServer(layers, 784, max_batch=1, min_batch_bucket=1)
+    for k, (idx, tm) in enumerate(special):
+        r = Request(rid=100 + k, indices=idx.copy(), times=tm.copy(), arrival=0.0)
+        alone.submit(r)
+        alone.drain()
+        assert np.array_equal(r.counts, got[100 + k].counts)
+        assert np.array_equal(r.spike_times, got[100 + k].spike_times)
+        per, _ = oev.events_to_per_neuron(idx[None, :], tm[None, :], 784)
+        h = olif.lif_forward(per, layers[0].weights, 0.130, 1.0, 1.0, 30, tmax=1.0)
+        o = olif.lif_forward(h.t, layers[1].weights, 0.130, 1.3, 1.3, 30, tmax=1.0)
+        assert np.array_equal(r.counts, o.n[0])
+        m = np.arange(30)[None, :] < o.n[0][:, None]
+        assert np.all(np.abs(r.spike_times[m] - o.t[0][m]) <= 1e-5 * np.abs(o.t[0][m]))

@@ -461,7 +461,14 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
     int wt_bytes, int K, double c, double dtt, double tau, int *__restrict__ out_n,
     float *__restrict__ out_t, float *__restrict__ out_a, float *__restrict__ out_x,
     float *__restrict__ out_z) {
-  __shared__ int s_n[kSolveMaxThreads];
+  // dynamic LDS (seq_solver_lds_bytes): the spike stash [kLaneStash][nt] float64, the row
+  // heads (first spike times of each row) [nt][kLaneStash] float32, the counts [nt] and, in
+  // the staged form, the wave-private record stages [nt / 64][64] FastEvent
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  const int nt = blockDim.x;
+  double *s_z = reinterpret_cast<double *>(s_dyn);
+  float *s_head = reinterpret_cast<float *>(s_z + kLaneStash * nt);
+  int *s_n = reinterpret_cast<int *>(s_head + kLaneStash * nt);
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * blockDim.x;
   const int j = j0 + threadIdx.x;
@@ -478,6 +485,7 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
       __builtin_amdgcn_make_buffer_rsrc((void *)WT, (short)0, wt_bytes, kBufferWord3);
   const double inv2c = 0.5 / c;
   const size_t row = ((size_t)b * J + jc) * K;
+  const LaneStash stash{s_z, (int)threadIdx.x, nt, out_t, out_z};
   // With the filter, a lane that is inactive or whose buffer is full carries a = -inf (it
   // stays -inf and fails the filter's a > 0); in the always-exact form a flag does this.
   double a = (kAlwaysExact || active) ? 0.0 : -kInfD, bb = 0.0;
@@ -493,8 +501,7 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
   for (int u = 0; u < kGroup; ++u) ixn[u] = kGroup < Ppad ? ix[kGroup + u] : 0;
   // staged form: each wave keeps the records of the next 64 events in its own LDS slice,
   // filled with one coalesced load per chunk issued a whole chunk ahead
-  __shared__ FastEvent s_stage[kStaged ? kSolveMaxThreads / 64 : 1][64];
-  FastEvent *stage = s_stage[kStaged ? (threadIdx.x >> 6) : 0];
+  FastEvent *stage = reinterpret_cast<FastEvent *>(s_n + nt) + (kStaged ? (threadIdx.x & ~63) : 0);
   const int lane = threadIdx.x & 63;
   FastEvent reg{};
   if (kStaged && lane < Ppad) reg = fv[lane];
@@ -536,9 +543,9 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
         const bool pass = kAlwaysExact ? live : clamp_filter(a, bb, cur, c, inv2c);
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(pass) != 0, 0)) {
           if (pass) {
-            const SolveStateZ r = solve_crossings_deferred(a, bb, n, cur.zq_lo, cur.zl_hi,
-                                                           ev + q0 + u, c, inv2c, dtt, tau, K,
-                                                           row, out_t, out_a, out_x, out_z);
+            const SolveStateZ r = solve_crossings_deferred_stash(
+                a, bb, n, cur.zq_lo, cur.zl_hi, EventEnds{ev + q0 + u}, c, inv2c, dtt, tau, K,
+                row, out_a, out_x, stash);
             bb = r.b;
             n = r.n;
             if (kAlwaysExact)
@@ -556,18 +563,52 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
       __builtin_amdgcn_wave_barrier();
     }
   }
-  if (active) finish_spike_times(row, n, tau, out_t, out_z);
-  s_n[threadIdx.x] = active ? n : 0;
-  if (active) out_n[(size_t)b * J + j] = n;
-  __syncthreads();
-  // the +inf tail of the block's rows, streamed past the caches (no later kernel reads it
-  // before it has left L2, and it would evict the other workgroups' weights)
-  const int jcount = min((int)blockDim.x, J - j0);
-  float *tb = out_t + ((size_t)b * J + j0) * K;
-  for (int e = threadIdx.x; e < jcount * K; e += blockDim.x) {
-    const int jj = e / K;
-    if (e - jj * K >= s_n[jj]) __builtin_nontemporal_store(kInfF, tb + e);
+  // spike times: the first kLaneStash of a row from the LDS stash into the row heads (their t
+  // slots are written below with the row), later ones in place from their parked words
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < kLaneStash; ++k) {
+      if (k < n) {
+        const double z = s_z[k * nt + threadIdx.x];
+        s_head[threadIdx.x * kLaneStash + k] = (float)(tau * log(z));
+        out_z[row + k] = (float)z;
+      }
+    }
+    if (n > kLaneStash) finish_spike_times(row, n, tau, out_t, out_z, kLaneStash);
+    out_n[(size_t)b * J + j] = n;
   }
+  s_n[threadIdx.x] = active ? n : 0;
+  __syncthreads();
+  // the block's rows of t, written row by row with consecutive lanes on consecutive slots
+  // (whole lines): the row heads, then +inf; slots of later spikes are already written.
+  // Streamed past the caches (no later kernel reads them before they have left L2, and they
+  // would evict the other workgroups' weights).
+  const int jcount = min(nt, J - j0);
+  float *tb = out_t + ((size_t)b * J + j0) * K;
+  const auto put = [&](int r, int k) {
+    const int nr = s_n[r];
+    if (k < nr && k < kLaneStash)
+      __builtin_nontemporal_store(s_head[r * kLaneStash + k], tb + (size_t)r * K + k);
+    else if (k >= nr)
+      __builtin_nontemporal_store(kInfF, tb + (size_t)r * K + k);
+  };
+  if (K <= 32) {
+    // two rows per wave and pass: lanes 0-31 one row, lanes 32-63 the next
+    const int lane = threadIdx.x & 63, k = lane & 31;
+    for (int r = 2 * (threadIdx.x >> 6) + (lane >> 5); r < jcount; r += 2 * (nt >> 6))
+      if (k < K) put(r, k);
+  } else {
+    for (int e = threadIdx.x; e < jcount * K; e += nt) {
+      const int r = e / K;
+      put(r, e - r * K);
+    }
+  }
+}
+
+// Dynamic LDS of lif_solve_seq_kernel for nt lanes per workgroup.
+static size_t seq_solver_lds_bytes(int nt, bool staged) {
+  return (size_t)nt * (kLaneStash * (sizeof(double) + sizeof(float)) + sizeof(int)) +
+         (staged ? (size_t)nt * sizeof(FastEvent) : 0);
 }
 
 // ------------------------------------------------------------------ solver, wave per neuron
@@ -712,6 +753,28 @@ static int cu_count() {
   return cached[dev];
 }
 
+// The lane solver's dynamic LDS reaches about 86 KB at 1024 lanes in the staged form: the
+// limit above the default 64 KB is raised once per device for all four instantiations.
+static int allow_seq_solver_lds() {
+  static int status[64];
+  static std::once_flag once[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+  std::call_once(once[dev], [dev]() {
+    const int bytes = (int)seq_solver_lds_bytes(kSolveMaxThreads, true);
+    const void *fns[4] = {(const void *)lif_solve_seq_kernel<true, true>,
+                          (const void *)lif_solve_seq_kernel<true, false>,
+                          (const void *)lif_solve_seq_kernel<false, true>,
+                          (const void *)lif_solve_seq_kernel<false, false>};
+    int rc = 0;
+    for (const void *f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+        rc = -1;
+    status[dev] = rc;
+  });
+  return status[dev];
+}
+
 static int event_capacity(int S) {
   return std::max(kEventPad, (S + kEventPad - 1) / kEventPad * kEventPad);
 }
@@ -826,8 +889,10 @@ static int solve_event_lists(hipStream_t st, int B, int I, int J, int K_post,
                                           : lif_solve_seq_kernel<true, false>)
                                 : (staged ? lif_solve_seq_kernel<false, true>
                                           : lif_solve_seq_kernel<false, false>);
+  const size_t lds = seq_solver_lds_bytes(threads, staged);
+  BATS_REQUIRE(allow_seq_solver_lds() == 0, "could not raise the solver's LDS limit");
   BATS_TIMED_IF(timed, "lif_solve_seq_kernel", st,
-             hipLaunchKernelGGL(seq, sg, dim3(threads), 0, st, el.events, el.fast, el.idx,
+             hipLaunchKernelGGL(seq, sg, dim3(threads), lds, st, el.events, el.fast, el.idx,
                                 el.counts, el.Pcap, WT, J, Jp, (int)wt_bytes, K_post, (double)c,
                                 (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
                                 out_z));

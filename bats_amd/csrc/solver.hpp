@@ -144,6 +144,29 @@ __device__ __forceinline__ void stash_z(size_t slot, double z, float *__restrict
   out_z[slot] = __uint_as_float((unsigned)bits);
 }
 
+// Where an accepted spike's float64 z waits for its logarithm. GlobalStash parks it in the
+// spike's own float32 t and z slots; LaneStash keeps a lane's first kLaneStash spikes in LDS
+// (one float64 per slot, [slot][lane]) and parks the later ones like GlobalStash.
+struct GlobalStash {
+  float *out_t, *out_z;
+  __device__ __forceinline__ void operator()(size_t slot, int, double z) const {
+    stash_z(slot, z, out_t, out_z);
+  }
+};
+
+constexpr int kLaneStash = 4;
+struct LaneStash {
+  double *lds;  // [kLaneStash][stride]
+  int lane, stride;
+  float *out_t, *out_z;
+  __device__ __forceinline__ void operator()(size_t slot, int k, double z) const {
+    if (k < kLaneStash)
+      lds[k * stride + lane] = z;
+    else
+      stash_z(slot, z, out_t, out_z);
+  }
+};
+
 __device__ __forceinline__ double tau_log(double tau, double z) { return tau * log(z); }
 
 // The interval is (t_lo, t_hi] with z_lo = exp(t_lo/tau), z_hi = exp(t_hi/tau); t_lo is an
@@ -160,11 +183,11 @@ struct EventEnds {
   __device__ float hi() const { return ev->tl; }
 };
 
-template <class Ends>
-__device__ __forceinline__ SolveStateZ solve_crossings_deferred_ends(
+template <class Ends, class Stash>
+__device__ __forceinline__ SolveStateZ solve_crossings_deferred_stash(
     double a, double b, int n, double zq_lo, double zl_hi, const Ends ends, double c,
-    double inv2c, double dtt, double tau, int K, size_t row, float *out_t, float *out_a,
-    float *out_x, float *out_z) {
+    double inv2c, double dtt, double tau, int K, size_t row, float *out_a, float *out_x,
+    const Stash &stash) {
   const double z_lo = zq_lo * (1.0 / (1.0 - kFilterEps));
   const double z_hi = zl_hi * (1.0 / (1.0 + kFilterEps));
   const bool hi_finite = z_hi < 1e299;  // an end of +inf is stored as 1e300
@@ -198,7 +221,7 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred_ends(
     const size_t slot = row + n;
     __builtin_nontemporal_store((float)a, out_a + slot);
     __builtin_nontemporal_store((float)x, out_x + slot);
-    stash_z(slot, z, out_t, out_z);
+    stash(slot, n, z);
     zlast = z;
     last_is_event = false;
     b = b - dtt * z;
@@ -215,6 +238,15 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred_ends(
   return r;
 }
 
+template <class Ends>
+__device__ __forceinline__ SolveStateZ solve_crossings_deferred_ends(
+    double a, double b, int n, double zq_lo, double zl_hi, const Ends ends, double c,
+    double inv2c, double dtt, double tau, int K, size_t row, float *out_t, float *out_a,
+    float *out_x, float *out_z) {
+  return solve_crossings_deferred_stash(a, b, n, zq_lo, zl_hi, ends, c, inv2c, dtt, tau, K, row,
+                                        out_a, out_x, GlobalStash{out_t, out_z});
+}
+
 __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
     double a, double b, int n, double zq_lo, double zl_hi, const Event *ev, double c,
     double inv2c, double dtt, double tau, int K, size_t row, float *out_t, float *out_a,
@@ -227,8 +259,8 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
 // solve_crossings stores them.
 __device__ __forceinline__ void finish_spike_times(size_t row, int n, double tau,
                                                    float *__restrict__ out_t,
-                                                   float *__restrict__ out_z) {
-  for (int k = 0; k < n; ++k) {
+                                                   float *__restrict__ out_z, int k0 = 0) {
+  for (int k = k0; k < n; ++k) {
     const unsigned hi = __float_as_uint(out_t[row + k]);
     const unsigned lo = __float_as_uint(out_z[row + k]);
     const double z = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));

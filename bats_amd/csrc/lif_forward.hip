@@ -620,13 +620,38 @@ static size_t seq_solver_lds_bytes(int nt, bool staged) {
 // which the filter is re-evaluated for them.
 constexpr int kWaveThreads = 256;
 
-__device__ __forceinline__ double wave_scan_incl(double v, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double y = __shfl_up(v, d, 64);
-    if (lane >= d) v += y;
-  }
+// A float64 moved between lanes with one DPP control, half by half; lanes without a source
+// (and rows outside row_mask) receive 0.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_move_f64(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, kCtrl, kRowMask, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), kCtrl, kRowMask, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Inclusive float64 prefix sum over the 64 lanes with DPP moves (no LDS): steps 1, 2, 4, 8
+// within each row of 16 lanes (row_shr), then the last lane of row 0 (row 2) is added to row 1
+// (row 3) (row_bcast:15) and the last lane of row 1 to rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ double wave_scan_incl(double v) {
+  v += dpp_move_f64<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move_f64<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move_f64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move_f64<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
   return v;
+}
+
+// The value of a wave-uniform lane: two lane reads into SGPRs.
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
@@ -675,16 +700,16 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
       pa = w * e.es;
       pb = w * e.e;
     }
-    double A = wave_scan_incl(pa, lane) + carry_a;
-    double Bv = wave_scan_incl(pb, lane) + carry_b;
+    double A = wave_scan_incl(pa) + carry_a;
+    double Bv = wave_scan_incl(pb) + carry_b;
     bool cand = valid & (always_exact | crossing_filter(A, Bv, e, c4));
     unsigned long long mask = __ballot(cand);
     while (mask) {
-      const int f = __ffsll(mask) - 1;
-      const double af = __shfl(A, f, 64);
-      const double bin = __shfl(Bv, f, 64);
-      const double tq = (double)__shfl(e.t, f, 64);
-      const double tl = (double)__shfl(e.tl, f, 64);
+      const int f = __builtin_amdgcn_readfirstlane(__ffsll(mask) - 1);
+      const double af = lane_value(A, f);
+      const double bin = lane_value(Bv, f);
+      const double tq = (double)lane_value(e.t, f);
+      const double tl = (double)lane_value(e.tl, f);
       double bf = bin;
       const int n0 = n;
       live = solve_crossings(af, bf, n, tq, tl, c, dtt, tau, K, row, out_t, out_a, out_x, out_z,
@@ -700,9 +725,9 @@ __global__ __launch_bounds__(kWaveThreads) void lif_solve_wave_kernel(
         mask &= ~(1ull << f);
       }
     }
-    const int last_lane = min(63, P - 1 - q0);
-    carry_a = __shfl(A, last_lane, 64);
-    carry_b = __shfl(Bv, last_lane, 64);
+    const int last_lane = __builtin_amdgcn_readfirstlane(min(63, P - 1 - q0));
+    carry_a = lane_value(A, last_lane);
+    carry_b = lane_value(Bv, last_lane);
   }
   if (lane == 0) out_n[(size_t)b * J + j] = n;
   for (int k = n + lane; k < K; k += 64) out_t[row + k] = kInfF;

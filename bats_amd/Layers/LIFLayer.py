@@ -62,6 +62,10 @@ class LIFLayer(AbstractLayer):
         """(tau_s, tau, c = theta/tau, delta_theta/tau) as float32."""
         return self.__tau_s, self.__tau, self.__c, self.__delta_theta_tau
 
+    @property
+    def previous_layer(self) -> AbstractLayer:
+        return self.__previous_layer
+
     def reset(self) -> None:
         self.__n_spike_per_neuron = None
         self.__spike_times_per_neuron = None
@@ -70,13 +74,43 @@ class LIFLayer(AbstractLayer):
         self.__post_exp_tau = None
         self.__pre_exp_tau_s = None
         self.__pre_exp_tau = None
+        self.__event_lists = None
 
-    def forward(self, max_simulation: float, training: bool = False) -> None:
+    def accept_event_lists(self, lists: Optional[ops.EventLists]) -> None:
+        """Pre exps and event lists of this layer's next inputs, made by the previous layer's
+        forward (used once, if they still match the inputs and constants)."""
+        self.__event_lists = lists
+
+    def forward(self, max_simulation: float, training: bool = False,
+                next_layer: Optional["LIFLayer"] = None) -> None:
+        """``next_layer`` (given by Network.forward when the next layer is a dense LIF layer
+        fed by this one): this call also makes that layer's pre exps and event lists, so its
+        forward needs no per-sample sort; the results are the same."""
         pre_times, _ = self.__previous_layer.spike_trains
+        lists, self.__event_lists = self.__event_lists, None
+        consts = (self.__tau_s, self.__tau, self.__c, self.__delta_theta_tau)
+        chain = None
+        if next_layer is not None:
+            nts, nt, nc, _ = next_layer.constants
+            chain = (nts, nt, nc, max_simulation)
+        nxt = None
+        key = (float(self.__tau_s), float(self.__tau), float(self.__c),
+               float(np.float32(max_simulation)))
+        if lists is not None and lists.matches(pre_times, key):
+            res = ops.lif_forward_events(lists, self.__weights, *consts, self.__max_n_spike,
+                                         chain=chain)
+            if chain is not None:
+                res, nxt = res
+        elif chain is not None:
+            res, nxt = ops.lif_forward_chain(pre_times, self.__weights, *consts, max_simulation,
+                                             self.__max_n_spike, chain)
+        else:
+            res = ops.lif_forward(pre_times, self.__weights, *consts, max_simulation,
+                                  self.__max_n_spike)
         (self.__n_spike_per_neuron, self.__spike_times_per_neuron, self.__a, self.__x,
-         self.__post_exp_tau, self.__pre_exp_tau_s, self.__pre_exp_tau) = ops.lif_forward(
-            pre_times, self.__weights, self.__tau_s, self.__tau, self.__c,
-            self.__delta_theta_tau, max_simulation, self.__max_n_spike)
+         self.__post_exp_tau, self.__pre_exp_tau_s, self.__pre_exp_tau) = res
+        if next_layer is not None:
+            next_layer.accept_event_lists(nxt)
 
     def backward(self, errors: torch.Tensor) -> Optional[Tuple[torch.Tensor, torch.Tensor]]:
         """``errors`` [B,J,K] is updated in place with the reset-chain errors (as the

@@ -43,8 +43,17 @@ class Network:
     def forward(self, spikes_per_neuron, n_spikes_per_neuron, max_simulation: float = np.inf,
                 training: bool = False) -> None:
         self.__input_layer.set_spike_trains(spikes_per_neuron, n_spikes_per_neuron)
-        for layer in self.__layers:
-            layer.forward(max_simulation, training)
+        from .Layers.LIFLayer import LIFLayer
+        layers = self.__layers
+        for k, layer in enumerate(layers):
+            nxt = layers[k + 1] if k + 1 < len(layers) else None
+            # a dense LIF layer feeding the next dense LIF layer makes that layer's event
+            # lists in its own forward (bats_lif_forward_chain): same results, no sort pass
+            if (type(layer) is LIFLayer and type(nxt) is LIFLayer
+                    and nxt.previous_layer is layer):
+                layer.forward(max_simulation, training, next_layer=nxt)
+            else:
+                layer.forward(max_simulation, training)
 
     def backward(self, output_errors: torch.Tensor) -> List[Optional[torch.Tensor]]:
         """Reverse layer loop, stopping at the first non-trainable layer (bats/Network.py:38-48)."""

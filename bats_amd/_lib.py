@@ -26,6 +26,14 @@ SIGNATURES = {
                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
     "bats_lif_forward_solver": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _sz]),
+    "bats_event_lists_size": (_sz, [_i, _i]),
+    "bats_lif_forward_chain": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
+                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _vp,
+                                    _vp, _sz, _vp, _sz]),
+    "bats_lif_forward_events_workspace_size": (_sz, [_i, _i]),
+    "bats_lif_forward_events": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _sz, _vp, _f, _f, _f, _f,
+                                     _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _vp, _vp, _vp, _sz,
+                                     _vp, _sz]),
     "bats_lif_forward_noalloc_max_bytes": (_sz, []),
     "bats_lif_forward_packed_workspace_size": (_sz, [_i, _i, _i, _i]),
     "bats_lif_forward_packed": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _f, _f,

@@ -131,13 +131,15 @@ __device__ __forceinline__ void bitonic128_wave(uint64_t &x0, uint64_t &x1, int 
   }
 }
 
-// Merge passes over sorted runs of L0 keys (N / L0 a power of two, N >= 2 * blockDim.x): each
-// pass merges run pairs into runs of twice the length, every thread producing N / blockDim.x
-// consecutive outputs from its merge-path split (a binary search on its diagonal). Keys are
-// distinct except the ~0 padding. Returns the buffer holding the sorted keys.
-__device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0) {
-  const int E = N / (int)blockDim.x;
+// Merge passes over sorted runs of L0 keys (N / L0 a power of two, N >= 2 * nthr, nthr a power
+// of two): each pass merges run pairs into runs of twice the length, each of the first nthr
+// threads producing N / nthr consecutive outputs from its merge-path split (a binary search on
+// its diagonal); every thread of the block takes part in the barriers. Keys are distinct except
+// the ~0 padding. Returns the buffer holding the sorted keys.
+__device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0, int nthr) {
+  const int E = N / nthr;
   for (int L = L0; L < N; L <<= 1) {
+    if ((int)threadIdx.x < nthr) {
     const int d = threadIdx.x * E;
     const int pair = d / (2 * L), d0 = d - pair * 2 * L;
     const uint64_t *A = src + (size_t)pair * 2 * L, *Bv = A + L;
@@ -157,6 +159,7 @@ __device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0) {
       out[e] = take_a ? av : bv;
       ia += take_a ? 1 : 0;
       ib += take_a ? 0 : 1;
+    }
     }
     __syncthreads();
     uint64_t *tmp = src;
@@ -215,6 +218,37 @@ __device__ __forceinline__ void write_event(int q, int P, float t, float tn, int
   f.zl_hi = (q < P) ? e.zl * (1.0 + kFilterEps) : kInfD;
   fv[q] = f;
   ix[q] = e.idx;
+}
+
+// The event list of sample b from its P ascending (time, slot) keys, by the whole workgroup:
+// the records (padded to a multiple of kEventPad), the input indices (slot / K_pre), the
+// per-slot float32 exps at the events' slots and the count.
+__device__ __forceinline__ void write_sorted_events(
+    const uint64_t *keys, int P, int b, int S, int K_pre, float tau_s, float tau, float c,
+    float tmax, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
+    int *__restrict__ evidx, int *__restrict__ counts, float *__restrict__ pre_exp_s,
+    float *__restrict__ pre_exp) {
+  Event *ev = events + (size_t)b * Pcap;
+  FastEvent *fv = fast + (size_t)b * Pcap;
+  int *ix = evidx + (size_t)b * Pcap;
+  const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
+  const int Ppad = (P + kEventPad - 1) / kEventPad * kEventPad;
+  for (int q = threadIdx.x; q < Ppad; q += blockDim.x) {
+    float t = kInfF, tn = kInfF, es32 = 0.f, e32 = 0.f;
+    int s = 0;
+    if (q < P) {
+      const uint64_t key = keys[q];
+      t = key_float((uint32_t)(key >> 32));
+      s = (int)(uint32_t)key;
+      tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
+    }
+    write_event(q, P, t, tn, s / K_pre, dtau_s, dtau, dc, tmax, ev, fv, ix, es32, e32);
+    if (q < P) {
+      pre_exp_s[(size_t)b * S + s] = es32;
+      pre_exp[(size_t)b * S + s] = e32;
+    }
+  }
+  if (threadIdx.x == 0) counts[b] = P;
 }
 
 // One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
@@ -345,7 +379,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
         keys[seg + 2 * l + 1] = x1;
       }
       __syncthreads();
-      keys = merge_runs(keys, keys + 1024, Ns, 128);
+      keys = merge_runs(keys, keys + 1024, Ns, 128, (int)blockDim.x);
     }
   } else if (!use_global && 2 * N <= lds_cap) {
     // 1024-key runs sorted in registers, then merge-path passes between two LDS buffers
@@ -360,7 +394,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
       keys[c0 + 2 * t + 1] = x1;
       __syncthreads();
     }
-    keys = merge_runs(keys, keys + N, N, 1024);
+    keys = merge_runs(keys, keys + N, N, 1024, (int)blockDim.x);
   } else {
     for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
     if (use_global) __threadfence();
@@ -368,27 +402,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     bitonic_sort(keys, N, use_global);
   }
 
-  Event *ev = events + (size_t)b * Pcap;
-  FastEvent *fv = fast + (size_t)b * Pcap;
-  int *ix = evidx + (size_t)b * Pcap;
-  const double dtau_s = (double)tau_s, dtau = (double)tau, dc = (double)c;
-  const int Ppad = (P + kEventPad - 1) / kEventPad * kEventPad;
-  for (int q = threadIdx.x; q < Ppad; q += blockDim.x) {
-    float t = kInfF, tn = kInfF, es32 = 0.f, e32 = 0.f;
-    int s = 0;
-    if (q < P) {
-      const uint64_t key = keys[q];
-      t = key_float((uint32_t)(key >> 32));
-      s = (int)(uint32_t)key;
-      tn = (q + 1 < P) ? key_float((uint32_t)(keys[q + 1] >> 32)) : kInfF;
-    }
-    write_event(q, P, t, tn, s / K_pre, dtau_s, dtau, dc, tmax, ev, fv, ix, es32, e32);
-    if (q < P) {
-      pre_exp_s[(size_t)b * S + s] = es32;
-      pre_exp[(size_t)b * S + s] = e32;
-    }
-  }
-  if (threadIdx.x == 0) counts[b] = P;
+  write_sorted_events(keys, P, b, S, K_pre, tau_s, tau, c, tmax, Pcap, events, fast, evidx, counts,
+                      pre_exp_s, pre_exp);
 }
 
 // One workgroup per sample of a packed list: sample b's events are
@@ -448,6 +463,23 @@ __device__ __forceinline__ float load_weight(__amdgpu_buffer_rsrc_t r, int lane_
   return buffer_load_f32(r, lane_off, row_off);
 }
 
+// The next dense layer's event lists, built by the lane solver from its own spikes when the
+// whole of a sample's layer is one workgroup: the regions build_event_lists lays out for that
+// layer's S = J * K slots (records, indices, counts, global sort keys of gcap per sample), its
+// per-slot exps, and its constants. lds_keys (a power of two) is the LDS room for a sample's
+// sort keys; 0 turns the chain off. A sample with more keys sorts them in its global region
+// (the host guarantees gcap >= the next power of two of S then).
+struct ChainOut {
+  Event *events;
+  FastEvent *fast;
+  int *idx;
+  int *counts;
+  uint64_t *gkeys;
+  float *pre_es, *pre_e;
+  int Pcap, gcap, lds_keys;
+  float tau_s, tau, c, tmax;
+};
+
 // kAlwaysExact: c <= 0 (no threshold quadratic to filter with), every interval is solved.
 // kStaged: the event records come from a wave-private LDS stage filled one 64-event chunk
 // ahead (for launches with few waves per SIMD, whose per-event scalar loads would otherwise
@@ -460,10 +492,11 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
     const int *__restrict__ counts, int Pcap, const float *__restrict__ WT, int J, int Jp,
     int wt_bytes, int K, double c, double dtt, double tau, int *__restrict__ out_n,
     float *__restrict__ out_t, float *__restrict__ out_a, float *__restrict__ out_x,
-    float *__restrict__ out_z) {
+    float *__restrict__ out_z, const ChainOut chain) {
   // dynamic LDS (seq_solver_lds_bytes): the spike stash [kLaneStash][nt] float64, the row
-  // heads (first spike times of each row) [nt][kLaneStash] float32, the counts [nt] and, in
-  // the staged form, the wave-private record stages [nt / 64][64] FastEvent
+  // heads (first spike times of each row) [nt][kLaneStash] float32, the counts [nt], in the
+  // staged form the wave-private record stages [nt / 64][64] FastEvent, and with the chain
+  // the sort keys [chain.lds_keys] and the wave totals [nt / 64]
   extern __shared__ __align__(16) unsigned char s_dyn[];
   const int nt = blockDim.x;
   double *s_z = reinterpret_cast<double *>(s_dyn);
@@ -593,23 +626,110 @@ __global__ __launch_bounds__(kSolveMaxThreads) void lif_solve_seq_kernel(
       __builtin_nontemporal_store(kInfF, tb + (size_t)r * K + k);
   };
   if (K <= 32) {
-    // two rows per wave and pass: lanes 0-31 one row, lanes 32-63 the next
+    // two rows per wave and pass (lanes 0-31 one row, lanes 32-63 the next), four passes'
+    // LDS reads in one round
     const int lane = threadIdx.x & 63, k = lane & 31;
-    for (int r = 2 * (threadIdx.x >> 6) + (lane >> 5); r < jcount; r += 2 * (nt >> 6))
-      if (k < K) put(r, k);
+    const int step = 2 * (nt >> 6);
+    for (int r0 = 2 * (threadIdx.x >> 6) + (lane >> 5); r0 < jcount; r0 += 4 * step) {
+      int nr[4];
+      float hv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = min(r0 + u * step, jcount - 1);
+        nr[u] = s_n[r];
+        hv[u] = s_head[r * kLaneStash + min(k, kLaneStash - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + u * step;
+        if (r < jcount && k < K) {
+          if (k < nr[u] && k < kLaneStash)
+            __builtin_nontemporal_store(hv[u], tb + (size_t)r * K + k);
+          else if (k >= nr[u])
+            __builtin_nontemporal_store(kInfF, tb + (size_t)r * K + k);
+        }
+      }
+    }
   } else {
     for (int e = threadIdx.x; e < jcount * K; e += nt) {
       const int r = e / K;
       put(r, e - r * K);
     }
   }
+  if (chain.lds_keys > 0) {
+    // the next layer's event list of this sample (the workgroup holds all its neurons: j0 = 0):
+    // (time, slot) keys of every spike at offsets from a block scan of the counts, sorted,
+    // then the records, as sort_events_kernel writes them from the same spike trains
+    uint64_t *s_keys = reinterpret_cast<uint64_t *>(
+        reinterpret_cast<char *>(s_n + nt) + (kStaged ? (size_t)nt * sizeof(FastEvent) : 0));
+    int *s_wtot = reinterpret_cast<int *>(s_keys + chain.lds_keys);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = nt >> 6;
+    const int cnt = s_n[threadIdx.x];
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) s_wtot[wv] = incl;
+    __syncthreads();
+    int base = 0, P2 = 0;
+    for (int w = 0; w < nw; ++w) {
+      const int v = s_wtot[w];
+      base += w < wv ? v : 0;
+      P2 += v;
+    }
+    const int excl = base + incl - cnt;
+    int N2 = 1;
+    while (N2 < P2) N2 <<= 1;
+    const bool glob = N2 > chain.lds_keys;
+    uint64_t *keys = glob ? chain.gkeys + (size_t)b * chain.gcap : s_keys;
+    for (int k = 0; k < cnt; ++k) {
+      // a row's later spike times were written in place by this lane
+      const float tk = k < kLaneStash ? s_head[threadIdx.x * kLaneStash + k] : out_t[row + k];
+      keys[excl + k] = ((uint64_t)float_key(tk) << 32) | (uint32_t)(j * K + k);
+    }
+    // runs of 128 keys sorted by the waves in registers, then merge-path passes between two
+    // LDS halves, when both fit; otherwise a bitonic network in place (in LDS, or in the
+    // sample's global key region)
+    const int Ns = N2 > 128 ? N2 : 128;
+    const bool runs = !glob && 2 * Ns <= chain.lds_keys;
+    const int Nfill = runs ? Ns : N2;
+    for (int q = P2 + threadIdx.x; q < Nfill; q += nt) keys[q] = ~0ull;
+    if (glob) __threadfence();
+    __syncthreads();
+    if (runs) {
+      for (int r0 = wv * 128; r0 < Ns; r0 += nw * 128) {
+        uint64_t x0 = keys[r0 + 2 * lane], x1 = keys[r0 + 2 * lane + 1];
+        bitonic128_wave(x0, x1, lane);
+        keys[r0 + 2 * lane] = x0;
+        keys[r0 + 2 * lane + 1] = x1;
+      }
+      __syncthreads();
+      if (Ns > 128) {
+        int nthr = 1;
+        while (2 * nthr <= nt && 2 * nthr <= Ns / 2) nthr <<= 1;
+        keys = merge_runs(keys, keys + Ns, Ns, 128, nthr);
+      }
+    } else if (P2 > 1) {
+      bitonic_sort(keys, N2, glob);
+    }
+    write_sorted_events(keys, P2, b, J * K, K, chain.tau_s, chain.tau, chain.c, chain.tmax,
+                        chain.Pcap, chain.events, chain.fast, chain.idx, chain.counts,
+                        chain.pre_es, chain.pre_e);
+  }
 }
 
-// Dynamic LDS of lif_solve_seq_kernel for nt lanes per workgroup.
-static size_t seq_solver_lds_bytes(int nt, bool staged) {
+// Dynamic LDS of lif_solve_seq_kernel for nt lanes per workgroup (and lds_keys sort keys of
+// the chained event lists).
+static size_t seq_solver_lds_bytes(int nt, bool staged, int lds_keys = 0) {
   return (size_t)nt * (kLaneStash * (sizeof(double) + sizeof(float)) + sizeof(int)) +
-         (staged ? (size_t)nt * sizeof(FastEvent) : 0);
+         (staged ? (size_t)nt * sizeof(FastEvent) : 0) +
+         (lds_keys > 0 ? (size_t)lds_keys * sizeof(uint64_t) + sizeof(int) * (nt / 64) : 0);
 }
+
+// Sort keys of a chained sample held in LDS at most (beyond: the global key region).
+constexpr int kChainLdsKeys = 8192;
 
 // ------------------------------------------------------------------ solver, wave per neuron
 // One wave per (sample, post neuron); lane l holds event q0 + l of the current 64-event chunk.
@@ -778,15 +898,16 @@ static int cu_count() {
   return cached[dev];
 }
 
-// The lane solver's dynamic LDS reaches about 86 KB at 1024 lanes in the staged form: the
-// limit above the default 64 KB is raised once per device for all four instantiations.
+// The lane solver's dynamic LDS reaches about 151 KB at 1024 lanes in the staged form with
+// chained event lists: the limit above the default 64 KB is raised once per device for all
+// four instantiations.
 static int allow_seq_solver_lds() {
   static int status[64];
   static std::once_flag once[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
   std::call_once(once[dev], [dev]() {
-    const int bytes = (int)seq_solver_lds_bytes(kSolveMaxThreads, true);
+    const int bytes = (int)seq_solver_lds_bytes(kSolveMaxThreads, true, kChainLdsKeys);
     const void *fns[4] = {(const void *)lif_solve_seq_kernel<true, true>,
                           (const void *)lif_solve_seq_kernel<true, false>,
                           (const void *)lif_solve_seq_kernel<false, true>,
@@ -819,26 +940,42 @@ size_t event_lists_bytes(int B, int S) {
   return off;
 }
 
-int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,
-                      float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
-                      void *ws, EventLists *out, bool timed) {
+// The regions of a batch's event lists inside `ws` (event_lists_bytes(B, S) bytes): records,
+// fast records, indices, counts and, for lists longer than the LDS sort holds, global sort
+// keys of gcap per sample.
+static EventLists event_lists_at(void *ws, int B, int S, uint64_t **gkeys, int *gcap_out) {
   const int cap = lds_capacity_entries();
   int N = 1;
   while (N < S) N <<= 1;
   const int gcap = (N > cap) ? N : 0;
   const int Pcap = event_capacity(S);
   char *base = static_cast<char *>(ws);
+  EventLists el;
   size_t off = 0;
-  out->events = reinterpret_cast<Event *>(base + off);
+  el.events = reinterpret_cast<Event *>(base + off);
   off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
-  out->fast = reinterpret_cast<FastEvent *>(base + off);
+  el.fast = reinterpret_cast<FastEvent *>(base + off);
   off = align_up(off + sizeof(FastEvent) * (size_t)B * Pcap, 256);
-  out->idx = reinterpret_cast<int *>(base + off);
+  el.idx = reinterpret_cast<int *>(base + off);
   off = align_up(off + sizeof(int) * (size_t)B * Pcap, 256);
-  out->counts = reinterpret_cast<int *>(base + off);
+  el.counts = reinterpret_cast<int *>(base + off);
   off = align_up(off + sizeof(int) * (size_t)B, 256);
-  uint64_t *gkeys = reinterpret_cast<uint64_t *>(base + off);
-  out->Pcap = Pcap;
+  el.Pcap = Pcap;
+  if (gkeys) *gkeys = reinterpret_cast<uint64_t *>(base + off);
+  if (gcap_out) *gcap_out = gcap;
+  return el;
+}
+
+int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_times,
+                      float tau_s, float tau, float c, float tmax, float *pre_es, float *pre_e,
+                      void *ws, EventLists *out, bool timed) {
+  const int cap = lds_capacity_entries();
+  int N = 1;
+  while (N < S) N <<= 1;
+  uint64_t *gkeys = nullptr;
+  int gcap = 0;
+  *out = event_lists_at(ws, B, S, &gkeys, &gcap);
+  const int Pcap = out->Pcap;
   if (S > 0) {
     const int lds_entries = std::max(2048, std::min(N, cap));  // the short-list merge buffers
     BATS_TIMED_IF(timed, "sort_events_kernel", st,
@@ -874,12 +1011,15 @@ extern "C" size_t bats_lif_forward_workspace_size(int B, int I, int K_pre, int J
 // lane solver's workgroups per sample (0: automatic); `timed` allows library timing events.
 // The solver launch over a batch's event lists (el): one wave per (sample, post neuron) in
 // wave_mode (W read row by row), else one lane per pair over W^T (WT, rows of Jp).
+// With `chain`, the lane solver also builds the next layer's event lists when one workgroup
+// holds a whole sample (chain->lds_keys is then set; otherwise it stays 0 and the caller
+// builds them from the spike trains).
 static int solve_event_lists(hipStream_t st, int B, int I, int J, int K_post,
                              const EventLists &el, bool any_events, const float *weights,
                              const float *WT, int Jp, bool wave_mode, int solver_mode,
                              int solver_blocks, float c, float delta_theta_tau, float tau,
                              int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
-                             bool timed) {
+                             bool timed, ChainOut *chain = nullptr) {
   const int always_exact = (c > 0.f) ? 0 : 1;
   const long long BJ = (long long)B * J;
   if (wave_mode && any_events) {
@@ -914,13 +1054,24 @@ static int solve_event_lists(hipStream_t st, int B, int I, int J, int K_post,
                                           : lif_solve_seq_kernel<true, false>)
                                 : (staged ? lif_solve_seq_kernel<false, true>
                                           : lif_solve_seq_kernel<false, false>);
-  const size_t lds = seq_solver_lds_bytes(threads, staged);
+  ChainOut co{};
+  if (chain && sg.x == 1) {
+    // every sample's keys must fit the LDS room or its global key region
+    long long n2 = 1;
+    while (n2 < (long long)J * K_post) n2 <<= 1;
+    if (n2 <= kChainLdsKeys || chain->gcap >= n2) {
+      // room for two halves (the merge passes) up to the LDS limit
+      chain->lds_keys = (int)std::min<long long>(2 * n2, kChainLdsKeys);
+      co = *chain;
+    }
+  }
+  const size_t lds = seq_solver_lds_bytes(threads, staged, co.lds_keys);
   BATS_REQUIRE(allow_seq_solver_lds() == 0, "could not raise the solver's LDS limit");
   BATS_TIMED_IF(timed, "lif_solve_seq_kernel", st,
              hipLaunchKernelGGL(seq, sg, dim3(threads), lds, st, el.events, el.fast, el.idx,
                                 el.counts, el.Pcap, WT, J, Jp, (int)wt_bytes, K_post, (double)c,
                                 (double)delta_theta_tau, (double)tau, out_n, out_t, out_a, out_x,
-                                out_z));
+                                out_z, co));
   return check_launch("lif_solve_seq_kernel");
 }
 
@@ -930,7 +1081,7 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
                             int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
                             float *pre_exp_tau_s, float *pre_exp_tau, int solver_mode,
                             int solver_blocks, bool timed, void *workspace,
-                            size_t workspace_bytes) {
+                            size_t workspace_bytes, ChainOut *chain = nullptr) {
   BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0, "negative size");
   BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
   BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
@@ -968,7 +1119,48 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
     return r;
   return solve_event_lists(st, B, I, J, K_post, el, S > 0, weights, WT, Jp, wave_mode,
                            solver_mode, solver_blocks, c, delta_theta_tau, tau, out_n, out_t,
-                           out_a, out_x, out_z, timed);
+                           out_a, out_x, out_z, timed, chain);
+}
+
+// ------------------------------------------------------------------ chained layers
+// The next dense layer's event lists from this layer's spike trains out_t [B, J, K_post]: the
+// chain's regions (next_events, laid out by event_lists_at for S = J * K_post slots) and
+// constants. `built` says whether the lane solver already wrote them; otherwise they are built
+// here with the per-sample sort, from the spike trains.
+static int prepare_chain(int B, int J, int K_post, float next_tau_s, float next_tau,
+                         float next_c, float next_max_simulation, float *next_pre_exp_tau_s,
+                         float *next_pre_exp_tau, void *next_events, size_t next_events_bytes,
+                         ChainOut *ch) {
+  BATS_REQUIRE(next_tau_s > 0.f && next_tau > 0.f, "next layer: tau_s and tau must be positive");
+  const int S2 = J * K_post;
+  BATS_REQUIRE(next_events != nullptr && next_pre_exp_tau_s && next_pre_exp_tau,
+               "next layer: null event-list or exp pointer");
+  BATS_REQUIRE(next_events_bytes >= event_lists_bytes(B, S2), "next layer: event lists too small");
+  *ch = ChainOut{};
+  int gcap = 0;
+  const EventLists el = event_lists_at(next_events, B, S2, &ch->gkeys, &gcap);
+  ch->events = el.events;
+  ch->fast = el.fast;
+  ch->idx = el.idx;
+  ch->counts = el.counts;
+  ch->pre_es = next_pre_exp_tau_s;
+  ch->pre_e = next_pre_exp_tau;
+  ch->Pcap = el.Pcap;
+  ch->gcap = gcap;
+  ch->lds_keys = 0;
+  ch->tau_s = next_tau_s;
+  ch->tau = next_tau;
+  ch->c = next_c;
+  ch->tmax = next_max_simulation;
+  return 0;
+}
+
+static int finish_chain(hipStream_t st, int B, int J, int K_post, const float *out_t,
+                        const ChainOut &ch, void *next_events) {
+  if (ch.lds_keys > 0) return 0;  // built by the lane solver
+  EventLists el;
+  return build_event_lists(st, B, J * K_post, K_post, out_t, ch.tau_s, ch.tau, ch.c, ch.tmax,
+                           ch.pre_es, ch.pre_e, next_events, &el);
 }
 
 extern "C" int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
@@ -994,6 +1186,85 @@ extern "C" int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, in
                           delta_theta_tau, max_simulation, out_n, out_t, out_a, out_x, out_z,
                           pre_exp_tau_s, pre_exp_tau, solver_mode, solver_blocks, true,
                           workspace, workspace_bytes);
+}
+
+extern "C" size_t bats_event_lists_size(int B, int S) {
+  if (B < 0 || S < 0) return 0;
+  return event_lists_bytes(B, S);
+}
+
+extern "C" int bats_lif_forward_chain(
+    void *stream, int B, int I, int K_pre, int J, int K_post, const float *pre_times,
+    const float *weights, float tau_s, float tau, float c, float delta_theta_tau,
+    float max_simulation, int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+    float *pre_exp_tau_s, float *pre_exp_tau, float next_tau_s, float next_tau, float next_c,
+    float next_max_simulation, float *next_pre_exp_tau_s, float *next_pre_exp_tau,
+    void *next_events, size_t next_events_bytes, void *workspace, size_t workspace_bytes) {
+  BATS_REQUIRE(B >= 0 && J >= 0 && K_post >= 1, "bad sizes");
+  if (B == 0) return 0;
+  ChainOut ch;
+  if (int r = prepare_chain(B, J, K_post, next_tau_s, next_tau, next_c, next_max_simulation,
+                            next_pre_exp_tau_s, next_pre_exp_tau, next_events, next_events_bytes,
+                            &ch))
+    return r;
+  if (int r = lif_forward_impl(stream, B, I, K_pre, J, K_post, pre_times, weights, tau_s, tau, c,
+                               delta_theta_tau, max_simulation, out_n, out_t, out_a, out_x,
+                               out_z, pre_exp_tau_s, pre_exp_tau, 0, 0, true, workspace,
+                               workspace_bytes, &ch))
+    return r;
+  return finish_chain(as_stream(stream), B, J, K_post, out_t, ch, next_events);
+}
+
+extern "C" size_t bats_lif_forward_events_workspace_size(int I, int J) {
+  if (I < 0 || J < 0) return 0;
+  return std::max<size_t>(256, align_up(sizeof(float) * (size_t)I * wt_stride(J), 256));
+}
+
+extern "C" int bats_lif_forward_events(
+    void *stream, int B, int I, int K_pre, int J, int K_post, const void *events,
+    size_t events_bytes, const float *weights, float tau_s, float tau, float c,
+    float delta_theta_tau, int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+    float next_tau_s, float next_tau, float next_c, float next_max_simulation,
+    float *next_pre_exp_tau_s, float *next_pre_exp_tau, void *next_events,
+    size_t next_events_bytes, void *workspace, size_t workspace_bytes) {
+  BATS_REQUIRE(B >= 0 && I >= 0 && K_pre >= 0 && J >= 0, "negative size");
+  BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
+  BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
+  BATS_REQUIRE(B <= 65535, "batch above 65535");
+  if (B == 0 || J == 0) return 0;
+  BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");
+  BATS_REQUIRE(tau_s > 0.f && tau > 0.f, "tau_s and tau must be positive");
+  const int S = I * K_pre;
+  BATS_REQUIRE(events != nullptr && events_bytes >= event_lists_bytes(B, S),
+               "event lists missing or too small");
+  BATS_REQUIRE(workspace != nullptr &&
+                   workspace_bytes >= bats_lif_forward_events_workspace_size(I, J),
+               "workspace too small");
+  BATS_REQUIRE(S == 0 || weights, "null weights");
+  ChainOut ch;
+  ChainOut *chp = nullptr;
+  if (next_events) {
+    if (int r = prepare_chain(B, J, K_post, next_tau_s, next_tau, next_c, next_max_simulation,
+                              next_pre_exp_tau_s, next_pre_exp_tau, next_events,
+                              next_events_bytes, &ch))
+      return r;
+    chp = &ch;
+  }
+  hipStream_t st = as_stream(stream);
+  float *WT = static_cast<float *>(workspace);
+  const int Jp = wt_stride(J);
+  const bool wave_mode = (long long)B * J < kSeqMinLanes;
+  if (S > 0 && !wave_mode) {
+    dim3 tg((I + 31) / 32, (Jp + 31) / 32);
+    hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, weights, WT, J, I, Jp);
+    if (int r = check_launch("transpose_kernel")) return r;
+  }
+  const EventLists el = event_lists_at(const_cast<void *>(events), B, S, nullptr, nullptr);
+  if (int r = solve_event_lists(st, B, I, J, K_post, el, S > 0, weights, WT, Jp, wave_mode, 0, 0,
+                                c, delta_theta_tau, tau, out_n, out_t, out_a, out_x, out_z, true,
+                                chp))
+    return r;
+  return chp ? finish_chain(st, B, J, K_post, out_t, ch, next_events) : 0;
 }
 
 // ------------------------------------------------------------------ forward from packed lists

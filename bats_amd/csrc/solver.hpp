@@ -255,17 +255,29 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred(
                                        K, row, out_t, out_a, out_x, out_z);
 }
 
-// Spike times of one lane's n stashed spikes: t = tau*log(z) and z rounded to float32, as
-// solve_crossings stores them.
+// Spike times of one lane's stashed spikes k0 .. n-1: t = tau*log(z) and z rounded to float32,
+// as solve_crossings stores them. Four slots at a time, their parked words loaded in one round.
 __device__ __forceinline__ void finish_spike_times(size_t row, int n, double tau,
                                                    float *__restrict__ out_t,
                                                    float *__restrict__ out_z, int k0 = 0) {
-  for (int k = k0; k < n; ++k) {
-    const unsigned hi = __float_as_uint(out_t[row + k]);
-    const unsigned lo = __float_as_uint(out_z[row + k]);
-    const double z = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-    out_t[row + k] = (float)(tau * log(z));
-    out_z[row + k] = (float)z;
+  for (int kb = k0; kb < n; kb += 4) {
+    unsigned hi[4], lo[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (kb + u < n) {
+        hi[u] = __float_as_uint(out_t[row + kb + u]);
+        lo[u] = __float_as_uint(out_z[row + kb + u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (kb + u < n) {
+        const double z =
+            __longlong_as_double((long long)(((unsigned long long)hi[u] << 32) | lo[u]));
+        out_t[row + kb + u] = (float)(tau * log(z));
+        out_z[row + kb + u] = (float)z;
+      }
+    }
   }
 }
 

@@ -89,6 +89,107 @@ def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float3
     return n, t, a, x, z, es, e
 
 
+class EventLists:
+    """The next dense layer's pre exps and time-ordered event lists, made by the layer before
+    it (:func:`lif_forward_chain`, or :func:`lif_forward_events` with ``chain``) from its
+    spike trains ``source`` [B, I, K_pre]; consumed by :func:`lif_forward_events`. It belongs
+    to those spike trains as they were: :meth:`matches` is False once they were written to
+    (their tensor version changed) or for other constants."""
+
+    __slots__ = ("buf", "nbytes", "B", "I", "K_pre", "es", "e", "source", "version", "key")
+
+    def __init__(self, buf, nbytes, source, es, e, key):
+        self.buf, self.nbytes = buf, int(nbytes)
+        self.B, self.I, self.K_pre = (int(v) for v in source.shape)
+        self.es, self.e = es, e
+        self.source, self.version = source, source._version
+        self.key = key
+
+    def matches(self, pre_times: torch.Tensor, key) -> bool:
+        return (pre_times is self.source and pre_times._version == self.version
+                and key == self.key)
+
+
+def _chain_args(chain, B, J, K, dev):
+    """(next tau_s, tau, c, max_simulation, next exps es/e, list buffer, bytes, key) of the
+    layer after a [B, J, K] layer; ``chain`` = (tau_s, tau, c, max_simulation) of that layer."""
+    tau_s2, tau2, c2, tmax2 = (float(np.float32(v)) for v in chain)
+    es2 = torch.empty((B, J, K), dtype=F32, device=dev)
+    e2 = torch.empty((B, J, K), dtype=F32, device=dev)
+    lib = _lib.load()
+    nbytes = int(lib.bats_event_lists_size(B, J * K))
+    buf = _workspace(nbytes, dev)
+    return tau_s2, tau2, c2, tmax2, es2, e2, buf, nbytes, (tau_s2, tau2, c2, tmax2)
+
+
+def lif_forward_chain(pre_times: torch.Tensor, weights: torch.Tensor, tau_s, tau, c,
+                      delta_theta_tau, max_simulation: float, max_n_post: int, chain):
+    """:func:`lif_forward` of a layer whose spike trains feed a dense layer with constants
+    ``chain`` = (tau_s, tau, c, max_simulation): also makes that layer's pre exps and event
+    lists (bats_lif_forward_chain). Returns the 7 outputs of :func:`lif_forward` and an
+    :class:`EventLists` for :func:`lif_forward_events`."""
+    _require_gpu()
+    B, I, Kp = pre_times.shape
+    J, I2 = weights.shape
+    if I2 != I:
+        raise ValueError(f"weights are [{J},{I2}] but the previous layer has {I} neurons")
+    K = int(max_n_post)
+    dev = pre_times.device
+    n = torch.empty((B, J), dtype=I32, device=dev)
+    t = torch.empty((B, J, K), dtype=F32, device=dev)
+    a = torch.empty((B, J, K), dtype=F32, device=dev)
+    x = torch.empty((B, J, K), dtype=F32, device=dev)
+    z = torch.empty((B, J, K), dtype=F32, device=dev)
+    es = torch.empty((B, I, Kp), dtype=F32, device=dev)
+    e = torch.empty((B, I, Kp), dtype=F32, device=dev)
+    tau_s2, tau2, c2, tmax2, es2, e2, buf, nbytes, key = _chain_args(chain, B, J, K, dev)
+    lib = _lib.load()
+    ws_bytes = lib.bats_lif_forward_workspace_size(B, I, Kp, J)
+    ws = _workspace(ws_bytes, dev)
+    _lib.check(lib.bats_lif_forward_chain(
+        _stream(), B, I, Kp, J, K, _ptr(pre_times, F32), _ptr(weights, F32), float(tau_s),
+        float(tau), float(c), float(delta_theta_tau), float(np.float32(max_simulation)),
+        _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), _ptr(es), _ptr(e), tau_s2, tau2, c2, tmax2,
+        _ptr(es2), _ptr(e2), _ptr(buf), nbytes, _ptr(ws), ws_bytes))
+    return (n, t, a, x, z, es, e), EventLists(buf, nbytes, t, es2, e2, key)
+
+
+def lif_forward_events(lists: EventLists, weights: torch.Tensor, tau_s, tau, c,
+                       delta_theta_tau, max_n_post: int, chain=None):
+    """Dense LIF forward on event lists made by the layer before it (bats_lif_forward_events).
+    Returns the 7 outputs of :func:`lif_forward` (the pre exps are ``lists.es`` / ``lists.e``)
+    and, with ``chain`` (the next layer's constants, as for :func:`lif_forward_chain`), the
+    next layer's :class:`EventLists` as an 8th value."""
+    _require_gpu()
+    B, I, Kp = lists.B, lists.I, lists.K_pre
+    J, I2 = weights.shape
+    if I2 != I:
+        raise ValueError(f"weights are [{J},{I2}] but the previous layer has {I} neurons")
+    K = int(max_n_post)
+    dev = weights.device
+    n = torch.empty((B, J), dtype=I32, device=dev)
+    t = torch.empty((B, J, K), dtype=F32, device=dev)
+    a = torch.empty((B, J, K), dtype=F32, device=dev)
+    x = torch.empty((B, J, K), dtype=F32, device=dev)
+    z = torch.empty((B, J, K), dtype=F32, device=dev)
+    lib = _lib.load()
+    ws_bytes = lib.bats_lif_forward_events_workspace_size(I, J)
+    ws = _workspace(ws_bytes, dev)
+    if chain is not None:
+        tau_s2, tau2, c2, tmax2, es2, e2, buf, nbytes, key = _chain_args(chain, B, J, K, dev)
+        nxt = (tau_s2, tau2, c2, tmax2, _ptr(es2), _ptr(e2), _ptr(buf), nbytes)
+    else:
+        nxt = (1.0, 1.0, 0.0, 0.0, None, None, None, 0)
+    _lib.check(lib.bats_lif_forward_events(
+        _stream(), B, I, Kp, J, K, _ptr(lists.buf), lists.nbytes, _ptr(weights, F32),
+        float(tau_s), float(tau), float(c), float(delta_theta_tau), _ptr(n), _ptr(t), _ptr(a),
+        _ptr(x), _ptr(z), *nxt, _ptr(ws), ws_bytes))
+    out = (n, t, a, x, z, lists.es, lists.e)
+    if chain is not None:
+        return out, EventLists(buf, nbytes, t, es2, e2, key)
+    return out
+
+
 def lif_forward_packed(row_offsets: torch.Tensor, indices: torch.Tensor, times: torch.Tensor,
                        P_max: int, weights: torch.Tensor, tau_s, tau, c, delta_theta_tau,
                        max_simulation: float, max_n_post: int, out=None):

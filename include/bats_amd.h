@@ -83,6 +83,57 @@ int bats_lif_forward_solver(void *stream, int B, int I, int K_pre, int J, int K_
                             void *workspace, size_t workspace_bytes);
 
 /*
+ * Two dense layers in a row (the second's inputs are the first's spike trains): the second
+ * layer's pre-spike exps and per-sample time-ordered event lists are made by the first
+ * layer's call, from its own spikes, and the second layer's call solves on them. This replaces
+ * the second layer's part of bats/Layers/LIFLayer.py:63-91 that only depends on its inputs
+ * (compute_pre_exps, bats/CudaKernels/Wrappers/Inference/compute_pre_exps.py:10, and
+ * get_sorted_spikes_indices, bats/CudaKernels/Wrappers/Inference/get_sorted_spikes_indices.py:7)
+ * by the first layer's solver: when one workgroup holds all of a sample's neurons, the solver
+ * sorts the sample's spikes in LDS after its walk (no per-sample sort of a [B, J, K_post]
+ * buffer). Results are those of two bats_lif_forward calls, bit for bit.
+ *
+ * bats_event_lists_size(B, S): bytes of the event lists of B samples with S input slots each
+ * (for the second layer, S = J * K_post of the first).
+ *
+ * bats_lif_forward_chain: bats_lif_forward of the first layer (same arguments and outputs up
+ * to pre_exp_tau), plus the second layer's constants next_tau_s, next_tau, next_c and its
+ * next_max_simulation, its per-slot exps next_pre_exp_tau_s / next_pre_exp_tau [B, J, K_post]
+ * (valid at the first layer's spikes, as bats_lif_forward writes pre exps) and its event lists
+ * (next_events, bats_event_lists_size(B, J * K_post) bytes, device memory the caller keeps
+ * until the second layer's call has run).
+ *
+ * bats_lif_forward_events: the second layer's forward on those event lists (events /
+ * events_bytes; I, K_pre = the first layer's J, K_post), with a workspace of
+ * bats_lif_forward_events_workspace_size(I, J) bytes; outputs out_n .. out_z as
+ * bats_lif_forward (its pre exps are the ones the chained call wrote). With next_events not
+ * NULL it also makes the event lists of the layer after it, as bats_lif_forward_chain does
+ * (next_events NULL: the next_* arguments are ignored).
+ */
+size_t bats_event_lists_size(int B, int S);
+int bats_lif_forward_chain(void *stream, int B, int I, int K_pre, int J, int K_post,
+                           const float *pre_times, const float *weights,
+                           float tau_s, float tau, float c, float delta_theta_tau,
+                           float max_simulation,
+                           int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                           float *pre_exp_tau_s, float *pre_exp_tau,
+                           float next_tau_s, float next_tau, float next_c,
+                           float next_max_simulation,
+                           float *next_pre_exp_tau_s, float *next_pre_exp_tau,
+                           void *next_events, size_t next_events_bytes,
+                           void *workspace, size_t workspace_bytes);
+size_t bats_lif_forward_events_workspace_size(int I, int J);
+int bats_lif_forward_events(void *stream, int B, int I, int K_pre, int J, int K_post,
+                            const void *events, size_t events_bytes, const float *weights,
+                            float tau_s, float tau, float c, float delta_theta_tau,
+                            int *out_n, float *out_t, float *out_a, float *out_x, float *out_z,
+                            float next_tau_s, float next_tau, float next_c,
+                            float next_max_simulation,
+                            float *next_pre_exp_tau_s, float *next_pre_exp_tau,
+                            void *next_events, size_t next_events_bytes,
+                            void *workspace, size_t workspace_bytes);
+
+/*
  * bats_lif_forward for a first layer fed by packed event lists, for inference: sample b's
  * input events are indices/times[row_offsets[b] .. row_offsets[b+1]) (row_offsets [B+1] int32
  * on the device), in ascending time, at most P_max per sample (later ones are dropped); an

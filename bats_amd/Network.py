@@ -48,9 +48,11 @@ class Network:
         for k, layer in enumerate(layers):
             nxt = layers[k + 1] if k + 1 < len(layers) else None
             # a dense LIF layer feeding the next dense LIF layer makes that layer's event
-            # lists in its own forward (bats_lif_forward_chain): same results, no sort pass
+            # lists in its own forward (bats_lif_forward_chain) when its solver can: same
+            # results, no sort pass
             if (type(layer) is LIFLayer and type(nxt) is LIFLayer
-                    and nxt.previous_layer is layer):
+                    and nxt.previous_layer is layer
+                    and ops.chain_fuses(len(spikes_per_neuron), layer.n_neurons)):
                 layer.forward(max_simulation, training, next_layer=nxt)
             else:
                 layer.forward(max_simulation, training)

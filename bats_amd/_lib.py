@@ -38,6 +38,9 @@ SIGNATURES = {
     "bats_lif_forward_packed_workspace_size": (_sz, [_i, _i, _i, _i]),
     "bats_lif_forward_packed": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _f, _f,
                                      _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "bats_lif_forward_packed_chain": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _f,
+                                           _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f,
+                                           _vp, _vp, _vp, _sz, _vp, _sz]),
     "bats_lif_forward_noalloc": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _f, _f, _f, _f, _f,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bats_lif_backward_workspace_size": (_sz, [_i, _i, _i, _i, _i]),

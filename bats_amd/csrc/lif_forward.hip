@@ -1287,13 +1287,13 @@ extern "C" size_t bats_lif_forward_packed_workspace_size(int B, int I, int P_max
   return align_up(sizeof(float) * (size_t)I * wt_stride(J), 256) + packed_lists_bytes(B, P_max);
 }
 
-extern "C" int bats_lif_forward_packed(void *stream, int B, int I, int J, int K_post, int P_max,
-                                       const int *row_offsets, const int *indices,
-                                       const float *times, const float *weights, float tau_s,
-                                       float tau, float c, float delta_theta_tau,
-                                       float max_simulation, int *out_n, float *out_t,
-                                       float *out_a, float *out_x, float *out_z,
-                                       void *workspace, size_t workspace_bytes) {
+static int lif_forward_packed_impl(void *stream, int B, int I, int J, int K_post, int P_max,
+                                   const int *row_offsets, const int *indices,
+                                   const float *times, const float *weights, float tau_s,
+                                   float tau, float c, float delta_theta_tau,
+                                   float max_simulation, int *out_n, float *out_t,
+                                   float *out_a, float *out_x, float *out_z, void *workspace,
+                                   size_t workspace_bytes, ChainOut *chain) {
   BATS_REQUIRE(B >= 0 && I >= 0 && J >= 0 && P_max >= 0, "negative size");
   BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
   BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
@@ -1336,7 +1336,42 @@ extern "C" int bats_lif_forward_packed(void *stream, int B, int I, int J, int K_
                                 Pcap, el.events, el.fast, el.idx, el.counts));
   if (int r = check_launch("packed_events_kernel")) return r;
   return solve_event_lists(st, B, I, J, K_post, el, any, weights, WT, Jp, wave_mode, 0, 0, c,
-                           delta_theta_tau, tau, out_n, out_t, out_a, out_x, out_z, true);
+                           delta_theta_tau, tau, out_n, out_t, out_a, out_x, out_z, true, chain);
+}
+
+extern "C" int bats_lif_forward_packed(void *stream, int B, int I, int J, int K_post, int P_max,
+                                       const int *row_offsets, const int *indices,
+                                       const float *times, const float *weights, float tau_s,
+                                       float tau, float c, float delta_theta_tau,
+                                       float max_simulation, int *out_n, float *out_t,
+                                       float *out_a, float *out_x, float *out_z,
+                                       void *workspace, size_t workspace_bytes) {
+  return lif_forward_packed_impl(stream, B, I, J, K_post, P_max, row_offsets, indices, times,
+                                 weights, tau_s, tau, c, delta_theta_tau, max_simulation, out_n,
+                                 out_t, out_a, out_x, out_z, workspace, workspace_bytes,
+                                 nullptr);
+}
+
+extern "C" int bats_lif_forward_packed_chain(
+    void *stream, int B, int I, int J, int K_post, int P_max, const int *row_offsets,
+    const int *indices, const float *times, const float *weights, float tau_s, float tau,
+    float c, float delta_theta_tau, float max_simulation, int *out_n, float *out_t,
+    float *out_a, float *out_x, float *out_z, float next_tau_s, float next_tau, float next_c,
+    float next_max_simulation, float *next_pre_exp_tau_s, float *next_pre_exp_tau,
+    void *next_events, size_t next_events_bytes, void *workspace, size_t workspace_bytes) {
+  BATS_REQUIRE(B >= 0 && J >= 0 && K_post >= 1, "bad sizes");
+  if (B == 0) return 0;
+  ChainOut ch;
+  if (int r = prepare_chain(B, J, K_post, next_tau_s, next_tau, next_c, next_max_simulation,
+                            next_pre_exp_tau_s, next_pre_exp_tau, next_events, next_events_bytes,
+                            &ch))
+    return r;
+  if (int r = lif_forward_packed_impl(stream, B, I, J, K_post, P_max, row_offsets, indices,
+                                      times, weights, tau_s, tau, c, delta_theta_tau,
+                                      max_simulation, out_n, out_t, out_a, out_x, out_z,
+                                      workspace, workspace_bytes, &ch))
+    return r;
+  return finish_chain(as_stream(stream), B, J, K_post, out_t, ch, next_events);
 }
 
 // ------------------------------------------------------------------ forward without allocation

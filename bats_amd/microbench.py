@@ -21,7 +21,8 @@ from ._constants import lif_constants
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--what", choices=("forward1", "forward2", "backward1", "backward2",
+    ap.add_argument("--what", choices=("forward1", "forward1_chain", "forward2", "backward1",
+                                       "backward2",
                                        "forward_hh", "backward_hh", "conv_forward1",
                                        "conv_forward2", "conv_backward1", "conv_backward2",
                                        "copy"),
@@ -77,6 +78,10 @@ def main():
     def run():
         if args.what == "forward1":
             ops.lif_forward(t, w1, *c1, 1.0, 30, solver_mode=sm)
+        elif args.what == "forward1_chain":
+            # the hidden layer chained to the output layer (its solver also builds the output
+            # layer's event lists), as the training step runs it
+            ops.lif_forward_chain(t, w1, *c1, 1.0, 30, (c2[0], c2[1], c2[2], 1.0))
         elif args.what == "forward2":
             ops.lif_forward(h[1], w2, *c2, 1.0, 30, solver_mode=sm)
         elif args.what == "backward1":

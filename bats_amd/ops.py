@@ -89,6 +89,20 @@ def lif_forward(pre_times: torch.Tensor, weights: torch.Tensor, tau_s: np.float3
     return n, t, a, x, z, es, e
 
 
+_CU_COUNT = {}
+
+
+def chain_fuses(batch: int, n_neurons: int) -> bool:
+    """Whether a chained dense layer of ``n_neurons`` builds the next layer's event lists inside
+    its solver (one workgroup per sample: at least as many samples as the device has compute
+    units, at most 1024 neurons); otherwise the chained call runs the per-sample sort, which is
+    what two plain calls do."""
+    dev = torch.cuda.current_device()
+    if dev not in _CU_COUNT:
+        _CU_COUNT[dev] = torch.cuda.get_device_properties(dev).multi_processor_count
+    return batch >= _CU_COUNT[dev] and n_neurons <= 1024
+
+
 class EventLists:
     """The next dense layer's pre exps and time-ordered event lists, made by the layer before
     it (:func:`lif_forward_chain`, or :func:`lif_forward_events` with ``chain``) from its
@@ -192,12 +206,14 @@ def lif_forward_events(lists: EventLists, weights: torch.Tensor, tau_s, tau, c,
 
 def lif_forward_packed(row_offsets: torch.Tensor, indices: torch.Tensor, times: torch.Tensor,
                        P_max: int, weights: torch.Tensor, tau_s, tau, c, delta_theta_tau,
-                       max_simulation: float, max_n_post: int, out=None):
+                       max_simulation: float, max_n_post: int, out=None, chain=None):
     """Dense LIF forward of a first layer fed by packed, time-ordered event lists (inference):
     sample b's events are ``indices/times[row_offsets[b]:row_offsets[b+1]]``, at most
     ``P_max`` each; an index outside [0, I) or a non-finite time ends a list. Returns
     (n, t, a, x, z) as :func:`lif_forward` (no pre exps). ``out`` may hold those five tensors
-    (written in place, for graph capture)."""
+    (written in place, for graph capture). With ``chain`` (the next dense layer's constants, as
+    for :func:`lif_forward_chain`; bats_lif_forward_packed_chain) it also returns that layer's
+    :class:`EventLists`."""
     _require_gpu()
     dev = weights.device
     off = as_device(row_offsets, I32, dev)
@@ -216,6 +232,14 @@ def lif_forward_packed(row_offsets: torch.Tensor, indices: torch.Tensor, times: 
     lib = _lib.load()
     ws_bytes = lib.bats_lif_forward_packed_workspace_size(B, I, int(P_max), J)
     ws = _workspace(ws_bytes, dev)
+    if chain is not None:
+        tau_s2, tau2, c2, tmax2, es2, e2, buf, nbytes, key = _chain_args(chain, B, J, K, dev)
+        _lib.check(lib.bats_lif_forward_packed_chain(
+            _stream(), B, I, J, K, int(P_max), _ptr(off), _ptr(idx), _ptr(tim),
+            _ptr(weights, F32), float(tau_s), float(tau), float(c), float(delta_theta_tau),
+            float(max_simulation), _ptr(n), _ptr(t), _ptr(a), _ptr(x), _ptr(z), tau_s2, tau2,
+            c2, tmax2, _ptr(es2), _ptr(e2), _ptr(buf), nbytes, _ptr(ws), ws_bytes))
+        return (n, t, a, x, z), EventLists(buf, nbytes, t, es2, e2, key)
     _lib.check(lib.bats_lif_forward_packed(
         _stream(), B, I, J, K, int(P_max), _ptr(off), _ptr(idx), _ptr(tim),
         _ptr(weights, F32), float(tau_s), float(tau), float(c), float(delta_theta_tau),

@@ -236,13 +236,38 @@ class _GraphBatch:
         self.done.record()
 
     def _run(self, server):
-        L0, w0 = server.layers[0], server.w_dev[0]
-        n, x, *_ = ops.lif_forward_packed(self.off, self.idx, self.t, self.P, w0,
-                                          *lif_constants(L0.tau_s, L0.theta, L0.delta_theta),
-                                          server.max_simulation, L0.max_n_spike)
-        for L, w in zip(server.layers[1:], server.w_dev[1:]):
-            n, x, *_ = ops.lif_forward(x, w, *lif_constants(L.tau_s, L.theta, L.delta_theta),
-                                       server.max_simulation, L.max_n_spike)
+        layers = list(zip(server.layers, server.w_dev))
+        tmax = server.max_simulation
+
+        def chain_to(k):
+            # the next layer's constants when layer k's solver builds its event lists
+            if k + 1 >= len(layers) or not ops.chain_fuses(self.B, layers[k][1].shape[0]):
+                return None
+            L = layers[k + 1][0]
+            return (*lif_constants(L.tau_s, L.theta, L.delta_theta)[:3], tmax)
+
+        L0, w0 = layers[0]
+        res = ops.lif_forward_packed(self.off, self.idx, self.t, self.P, w0,
+                                     *lif_constants(L0.tau_s, L0.theta, L0.delta_theta),
+                                     tmax, L0.max_n_spike, chain=chain_to(0))
+        lists = None
+        if isinstance(res[-1], ops.EventLists):
+            res, lists = res
+        n, x = res[0], res[1]
+        for k in range(1, len(layers)):
+            L, w = layers[k]
+            consts = lif_constants(L.tau_s, L.theta, L.delta_theta)
+            ch = chain_to(k)
+            if lists is not None:
+                res = ops.lif_forward_events(lists, w, *consts, L.max_n_spike, chain=ch)
+            elif ch is not None:
+                res = ops.lif_forward_chain(x, w, *consts, tmax, L.max_n_spike, ch)
+            else:
+                res = ops.lif_forward(x, w, *consts, tmax, L.max_n_spike)
+            lists = None
+            if ch is not None:
+                res, lists = res
+            n, x = res[0], res[1]
         _, _, pred = ops.count_loss(n, x.shape[2], targets=self.zero_targets, want_loss=False,
                                     want_errors=False, want_pred=True)
         return n, x, pred

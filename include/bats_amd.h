@@ -155,6 +155,23 @@ int bats_lif_forward_packed(void *stream, int B, int I, int J, int K_post, int P
                             void *workspace, size_t workspace_bytes);
 
 /*
+ * bats_lif_forward_packed chained to the next dense layer, as bats_lif_forward_chain is to
+ * bats_lif_forward: the next layer's pre exps and event lists are made in the same call (for
+ * bats_lif_forward_events). Same reference interfaces as bats_lif_forward_chain.
+ */
+int bats_lif_forward_packed_chain(void *stream, int B, int I, int J, int K_post, int P_max,
+                                  const int *row_offsets, const int *indices, const float *times,
+                                  const float *weights, float tau_s, float tau, float c,
+                                  float delta_theta_tau, float max_simulation,
+                                  int *out_n, float *out_t, float *out_a, float *out_x,
+                                  float *out_z,
+                                  float next_tau_s, float next_tau, float next_c,
+                                  float next_max_simulation,
+                                  float *next_pre_exp_tau_s, float *next_pre_exp_tau,
+                                  void *next_events, size_t next_events_bytes,
+                                  void *workspace, size_t workspace_bytes);
+
+/*
  * bats_lif_forward without a workspace argument and without any allocation: the workspace is
  * a file-scope device array of bats_lif_forward_noalloc_max_bytes() bytes (64 MiB) in the
  * library, one per GPU, held until the process ends. No counterpart in the reference: it is

@@ -163,3 +163,35 @@ def test_network_forward_chains_dense_layers():
     torch.cuda.synchronize()
     assert torch.equal(out.spike_trains[1], ref[0])
     assert torch.equal(out.spike_trains[0], ref[1])
+
+
+def test_packed_first_layer_chained():
+    """The inference server's first layer (packed, time-ordered event lists) chained to the
+    output layer against the packed forward followed by a plain forward."""
+    rng = np.random.default_rng(6)
+    B, I, J1, J2, K = 256, 784, 800, 10, 30
+    lists = []
+    for _ in range(B):
+        P = int(rng.integers(50, 2001))
+        idx = rng.integers(0, I, size=P).astype(np.int32)
+        tm = np.sort(rng.uniform(0.0, 1.0, size=P)).astype(np.float32)
+        lists.append((idx, tm))
+    off = np.zeros(B + 1, np.int32)
+    off[1:] = np.cumsum([len(i) for i, _ in lists])
+    idx = _dev(np.concatenate([i for i, _ in lists]), torch.int32)
+    tm = _dev(np.concatenate([t for _, t in lists]))
+    off = _dev(off, torch.int32)
+    P_max = 2048
+    w1 = _dev(rng.normal(0, 1.0, size=(J1, I)).astype(np.float32))
+    w2 = _dev(rng.normal(0, 1.0, size=(J2, J1)).astype(np.float32))
+    c1 = lif_constants(0.130, 1.0, 1.0)
+    c2 = lif_constants(0.130, 1.3, 1.3)
+    ref1 = ops.lif_forward_packed(off, idx, tm, P_max, w1, *c1, T_MAX, K)
+    ref2 = ops.lif_forward(ref1[1], w2, *c2, T_MAX, K)
+    got1, ev = ops.lif_forward_packed(off, idx, tm, P_max, w1, *c1, T_MAX, K,
+                                      chain=(c2[0], c2[1], c2[2], T_MAX))
+    got2 = ops.lif_forward_events(ev, w2, *c2, K)
+    torch.cuda.synchronize()
+    _assert_same_layer(got1, ref1, "first layer")
+    _assert_same_layer(got2, ref2, "second layer")
+    _assert_same_exps(got2[5], got2[6], ref2[5], ref2[6], ref1[0], "second layer")

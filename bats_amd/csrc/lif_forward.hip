@@ -728,6 +728,167 @@ static size_t seq_solver_lds_bytes(int nt, bool staged, int lds_keys = 0) {
          (lds_keys > 0 ? (size_t)lds_keys * sizeof(uint64_t) + sizeof(int) * (nt / 64) : 0);
 }
 
+// ------------------------------------------------------------------ solver, first spike only
+// Layers with one spike per neuron (K = 1) and few (sample, neuron) pairs per SIMD: before a
+// neuron's first spike no reset has happened, so its a and b at any event are the plain prefix
+// sums of the inputs, and its walk splits exactly into time segments. A workgroup is 64 post
+// neurons of one sample times nseg waves, wave s walking segment s of the event list:
+//   1. the segment's sums of w*es and w*e (no filter);
+//   2. the segment's start state, the sums of the earlier segments (in segment order);
+//   3. the lane solver's walk over the segment from that state (filter, closed form), which
+//      stops at the first accepted spike; z, a and x wait in LDS;
+//   4. the neuron's spike is that of its earliest segment with one (the later segments' walks
+//      assumed no earlier spike and are discarded).
+// The running sums are associated by segment instead of event by event (float64 rounding of
+// the last bits, as in the wave-per-neuron solver). kSegMax segments at most.
+constexpr int kSegMax = 16;
+
+__global__ __launch_bounds__(64 * kSegMax) void lif_solve_seg_kernel(
+    const Event *__restrict__ events, const FastEvent *__restrict__ fast,
+    const int *__restrict__ evidx, const int *__restrict__ counts, int Pcap,
+    const float *__restrict__ WT, int J, int Jp, int wt_bytes, double c, double dtt,
+    double tau, int nseg, int *__restrict__ out_n, float *__restrict__ out_t,
+    float *__restrict__ out_a, float *__restrict__ out_x, float *__restrict__ out_z) {
+  __shared__ double s_sa[kSegMax][64], s_sb[kSegMax][64], s_z[kSegMax][64];
+  __shared__ float s_a[kSegMax][64], s_x[kSegMax][64];
+  __shared__ int s_hit[kSegMax][64];
+  const int b = blockIdx.y;
+  const int sg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 64 + lane;
+  const bool active = j < J;
+  const int jc = active ? j : 0;
+  const int P = counts[b];
+  const int Ppad = (P + kGroup - 1) / kGroup * kGroup;
+  const int L = (Ppad / kGroup + nseg - 1) / nseg * kGroup;  // events per segment
+  const int q_lo = min(sg * L, Ppad), q_hi = min(q_lo + L, Ppad);
+  const Event *ev = events + (size_t)b * Pcap;
+  const FastEvent *fv = fast + (size_t)b * Pcap;
+  const int *ix = evidx + (size_t)b * Pcap;
+  const int lane_off = jc * (int)sizeof(float);
+  const int row_bytes = Jp * (int)sizeof(float);
+  const __amdgpu_buffer_rsrc_t wres =
+      __builtin_amdgcn_make_buffer_rsrc((void *)WT, (short)0, wt_bytes, kBufferWord3);
+  // 1. the segment's plain sums (the next group's weights in flight, its indices a group
+  // earlier still)
+  double sa = 0.0, sb = 0.0;
+  {
+    float pw[kGroup];
+    int pix[kGroup];
+    if (q_lo < q_hi) {
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) pw[u] = load_weight(wres, lane_off, ix[q_lo + u] * row_bytes);
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) pix[u] = q_lo + kGroup < q_hi ? ix[q_lo + kGroup + u] : 0;
+    for (int q0 = q_lo; q0 < q_hi; q0 += kGroup) {
+      float w[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) w[u] = pw[u];
+      if (q0 + kGroup < q_hi) {
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) pw[u] = load_weight(wres, lane_off, pix[u] * row_bytes);
+        if (q0 + 2 * kGroup < q_hi) {
+#pragma unroll
+          for (int u = 0; u < kGroup; ++u) pix[u] = ix[q0 + 2 * kGroup + u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const FastEvent f = fv[q0 + u];
+        sa = fma((double)w[u], f.es, sa);
+        sb = fma((double)w[u], f.e, sb);
+      }
+    }
+  }
+  s_sa[sg][lane] = sa;
+  s_sb[sg][lane] = sb;
+  __syncthreads();
+  // 2. the state before the segment's first event
+  double a = 0.0, bb = 0.0;
+  for (int s2 = 0; s2 < sg; ++s2) {
+    a += s_sa[s2][lane];
+    bb += s_sb[s2][lane];
+  }
+  if (!active) a = -kInfD;  // fails the filter's a > 0 for good
+  // 3. the walk over the segment (the next group's weights and indices in flight)
+  const double inv2c = 0.5 / c;
+  const SegStash stash{&s_z[sg][lane], &s_a[sg][lane], &s_x[sg][lane]};
+  int n = 0;
+  float wn[kGroup];
+  int ixn[kGroup];
+  if (q_lo < q_hi) {
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) wn[u] = load_weight(wres, lane_off, ix[q_lo + u] * row_bytes);
+  }
+#pragma unroll
+  for (int u = 0; u < kGroup; ++u) ixn[u] = q_lo + kGroup < q_hi ? ix[q_lo + kGroup + u] : 0;
+  FastEvent cur{};
+  if (q_lo < q_hi) cur = fv[q_lo];
+  for (int q0 = q_lo; q0 < q_hi; q0 += kGroup) {
+    float w[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) w[u] = wn[u];
+    if (q0 + kGroup < q_hi) {
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) wn[u] = load_weight(wres, lane_off, ixn[u] * row_bytes);
+      if (q0 + 2 * kGroup < q_hi) {
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) ixn[u] = ix[q0 + 2 * kGroup + u];
+      }
+    }
+    const bool more = q0 + kGroup < q_hi;
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      FastEvent nxt;
+      if (u + 1 < kGroup || more) nxt = fv[q0 + u + 1];
+      const double w64 = (double)w[u];
+      a = fma(w64, cur.es, a);
+      bb = fma(w64, cur.e, bb);
+      const bool pass = clamp_filter(a, bb, cur, c, inv2c);
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(pass) != 0, 0)) {
+        if (pass) {
+          // the potential already at the threshold where the interval starts (g <= 0 a little
+          // before it): the smaller root lies before the interval, which the closed form
+          // would reject, so it is not formed. This is what keeps a segment that starts after
+          // the neuron's first spike (its state has no reset) from taking the closed form at
+          // every event while the potential stays above the threshold.
+          const double gq = fma(fma(c, cur.zq_lo, -bb), cur.zq_lo, a);
+          if (!(gq <= 0.0)) {
+            const SolveStateZ r = solve_crossings_deferred_stash(
+                a, bb, n, cur.zq_lo, cur.zl_hi, EventEnds{ev + q0 + u}, c, inv2c, dtt, tau, 1,
+                0, nullptr, nullptr, stash);
+            bb = r.b;
+            n = r.n;
+            if (!r.live) a = -kInfD;
+          }
+        }
+      }
+      if (u + 1 < kGroup || more) cur = nxt;
+    }
+  }
+  s_hit[sg][lane] = n;
+  __syncthreads();
+  // 4. the earliest segment's spike
+  if (sg == 0 && active) {
+    int win = -1;
+    for (int s2 = nseg - 1; s2 >= 0; --s2)
+      if (s_hit[s2][lane]) win = s2;
+    const size_t row = (size_t)b * J + j;
+    if (win >= 0) {
+      const double z = s_z[win][lane];
+      out_t[row] = (float)(tau * log(z));
+      out_z[row] = (float)z;
+      out_a[row] = s_a[win][lane];
+      out_x[row] = s_x[win][lane];
+      out_n[row] = 1;
+    } else {
+      out_t[row] = kInfF;
+      out_n[row] = 0;
+    }
+  }
+}
+
 // Sort keys of a chained sample held in LDS at most (beyond: the global key region).
 constexpr int kChainLdsKeys = 8192;
 
@@ -1033,11 +1194,28 @@ static int solve_event_lists(hipStream_t st, int B, int I, int J, int K_post,
                                   out_n, out_t, out_a, out_x, out_z));
     return check_launch("lif_solve_wave_kernel");
   }
+  const int cus = cu_count();
+  // one spike per neuron and fewer than two lane-solver waves per SIMD: the walks split into
+  // time segments (lif_solve_seg_kernel), about four waves per SIMD in all
+  const long long lane_waves = (long long)B * ((J + 63) / 64);
+  if (K_post == 1 && !always_exact && any_events &&
+      (solver_mode == 5 || (solver_mode == 0 && lane_waves < 2LL * 4 * cus))) {
+    int nseg = (int)std::max<long long>(
+        2, std::min<long long>(kSegMax, (16LL * cus + lane_waves - 1) / std::max(1LL, lane_waves)));
+    if (solver_blocks > 0) nseg = std::min(solver_blocks, kSegMax);  // segments, for tuning
+    const size_t wt_bytes = sizeof(float) * (size_t)I * Jp;
+    BATS_REQUIRE(wt_bytes < 0x7fffffffULL, "W^T above 2 GiB");
+    BATS_TIMED_IF(timed, "lif_solve_seg_kernel", st,
+               hipLaunchKernelGGL(lif_solve_seg_kernel, dim3((J + 63) / 64, B), dim3(64 * nseg), 0,
+                                  st, el.events, el.fast, el.idx, el.counts, el.Pcap, WT, J, Jp,
+                                  (int)wt_bytes, (double)c, (double)delta_theta_tau, (double)tau,
+                                  nseg, out_n, out_t, out_a, out_x, out_z));
+    return check_launch("lif_solve_seg_kernel");
+  }
   // blocks per sample, and lanes per block rounded up to whole waves; a batch with fewer
   // samples than the device has CUs spreads each sample's neurons over more workgroups (of
   // at least one wave), so that the launch reaches every CU
   int nblk = (J + kSolveMaxThreads - 1) / kSolveMaxThreads;
-  const int cus = cu_count();
   if ((long long)B * nblk < cus) nblk = std::max(nblk, std::min((cus + B - 1) / B, (J + 63) / 64));
   if (solver_blocks > 0) nblk = std::max(1, std::min(solver_blocks, (J + 63) / 64));
   const int per = (J + nblk - 1) / nblk;
@@ -1086,9 +1264,10 @@ static int lif_forward_impl(void *stream, int B, int I, int K_pre, int J, int K_
   BATS_REQUIRE(K_post >= 1, "max_n_spike (K_post) must be >= 1");
   BATS_REQUIRE((double)B * J * K_post < 2147483647.0 * 8, "output too large");
   BATS_REQUIRE(B <= 65535, "batch above 65535");
-  BATS_REQUIRE(solver_mode >= 0 && solver_mode <= 4,
-               "solver_mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron) or 3 / 4 "
-               "(lane per neuron, staged / scalar event records)");
+  BATS_REQUIRE(solver_mode >= 0 && solver_mode <= 5,
+               "solver_mode must be 0 (auto), 1 (lane per neuron), 2 (wave per neuron), 3 / 4 "
+               "(lane per neuron, staged / scalar event records) or 5 (time segments, one "
+               "spike per neuron)");
   BATS_REQUIRE(solver_blocks >= 0, "solver_blocks must be >= 0 (0: automatic)");
   if (B == 0 || J == 0) return 0;
   BATS_REQUIRE(out_n && out_t && out_a && out_x && out_z, "null output pointer");

@@ -147,10 +147,21 @@ __device__ __forceinline__ void stash_z(size_t slot, double z, float *__restrict
 // Where an accepted spike's float64 z waits for its logarithm. GlobalStash parks it in the
 // spike's own float32 t and z slots; LaneStash keeps a lane's first kLaneStash spikes in LDS
 // (one float64 per slot, [slot][lane]) and parks the later ones like GlobalStash.
+// An accepted spike's a and x go straight to its slots (streamed past the caches).
+__device__ __forceinline__ void store_ax(size_t slot, double a, double x, float *out_a,
+                                         float *out_x) {
+  __builtin_nontemporal_store((float)a, out_a + slot);
+  __builtin_nontemporal_store((float)x, out_x + slot);
+}
+
 struct GlobalStash {
   float *out_t, *out_z;
   __device__ __forceinline__ void operator()(size_t slot, int, double z) const {
     stash_z(slot, z, out_t, out_z);
+  }
+  __device__ __forceinline__ void ax(size_t slot, double a, double x, float *out_a,
+                                     float *out_x) const {
+    store_ax(slot, a, x, out_a, out_x);
   }
 };
 
@@ -164,6 +175,22 @@ struct LaneStash {
       lds[k * stride + lane] = z;
     else
       stash_z(slot, z, out_t, out_z);
+  }
+  __device__ __forceinline__ void ax(size_t slot, double a, double x, float *out_a,
+                                     float *out_x) const {
+    store_ax(slot, a, x, out_a, out_x);
+  }
+};
+
+// A segment lane's one spike (layers with one spike per neuron): z, a and x in its LDS slots
+// until the lanes of the neuron's other segments are known.
+struct SegStash {
+  double *z;
+  float *a, *x;
+  __device__ __forceinline__ void operator()(size_t, int, double zv) const { *z = zv; }
+  __device__ __forceinline__ void ax(size_t, double av, double xv, float *, float *) const {
+    *a = (float)av;
+    *x = (float)xv;
   }
 };
 
@@ -219,8 +246,7 @@ __device__ __forceinline__ SolveStateZ solve_crossings_deferred_stash(
     }
     if (!(valid & above & below)) break;
     const size_t slot = row + n;
-    __builtin_nontemporal_store((float)a, out_a + slot);
-    __builtin_nontemporal_store((float)x, out_x + slot);
+    stash.ax(slot, a, x, out_a, out_x);
     stash(slot, n, z);
     zlast = z;
     last_is_event = false;

@@ -68,7 +68,10 @@ int bats_lif_forward(void *stream, int B, int I, int K_pre, int J, int K_post,
  * (sample, post neuron) when there are at least 32768 such pairs, else one wave per pair),
  * 1 = always one lane per pair, 2 = always one wave per pair, 3 / 4 = one lane per pair with
  * the event records staged in LDS / read with scalar loads (1 chooses by the launch's waves
- * per SIMD); solver_blocks = workgroups per sample of the lane-per-pair solver (0: one per
+ * per SIMD), 5 = for K_post = 1 (one spike per neuron; otherwise as 1) the event walk split
+ * into time segments walked at once (0 takes it when the lane-per-pair form would have fewer
+ * than two waves per SIMD; solver_blocks > 0 then sets the number of segments, at most 16);
+ * solver_blocks = workgroups per sample of the lane-per-pair solver (0: one per
  * 1024 neurons, more when the batch has fewer samples than the device has CUs). Every form
  * gives the same results; the arguments exist for testing and tuning. Same reference
  * interfaces as bats_lif_forward (bats/Layers/LIFLayer.py:63-91).

@@ -251,6 +251,30 @@ def test_forward_both_solver_forms(mode):
     check_forward(tb, W5, 0.05, 0.0, 0.2, 5, 0.6, f"odd J theta 0 mode {mode}", mode)
 
 
+@pytest.mark.parametrize("mode", [0, 5])
+def test_forward_first_spike_time_segments(mode):
+    # one spike per neuron with few (sample, neuron) pairs: the walk split into time segments
+    # (mode 5 forces it; mode 0 takes it below two lane-solver waves per SIMD): single and
+    # multi-spike inputs, spikes early and late in the list, silent neurons, an odd number of
+    # post neurons, one sample, and a list shorter than one group per segment
+    rng = np.random.default_rng(12)
+    t, _ = latency_inputs(rng, 50, 784)
+    W = rng.normal(0, 1.0, size=(800, 784)).astype(F32)
+    ref, _ = check_forward(t, W, 0.130, 1.0, 1.0, 1, 1.0, f"TTFS hidden mode {mode}", mode)
+    assert 0 < ref.n.sum() < ref.n.size
+    W2 = rng.normal(0, 1.0, size=(10, 800)).astype(F32)
+    check_forward(ref.t, W2, 0.130, 1.3, 1.3, 1, 1.0, f"TTFS output mode {mode}", mode)
+    tb, _ = multi_spike_inputs(rng, 8, 300, 6, t_max=1.0)
+    for scale, label in ((0.5, "late spikes"), (3.0, "early spikes")):
+        Wm = rng.normal(0.1, scale, size=(97, 300)).astype(F32)
+        r, _ = check_forward(tb, Wm, 0.05, 1.0, 1.0, 1, 1.0, f"{label} mode {mode}", mode)
+        assert r.n.sum() > 0
+    check_forward(t[:1], W, 0.130, 1.0, 1.0, 1, 0.5, f"one sample, t_max 0.5, mode {mode}", mode)
+    short = np.full((4, 784, 1), np.inf, F32)
+    short[:, :20, 0] = rng.uniform(0, 1, size=(4, 20)).astype(F32)
+    check_forward(short, np.abs(W), 0.130, 1.0, 1.0, 1, 1.0, f"short lists mode {mode}", mode)
+
+
 # ------------------------------------------------------------------ input encoding
 def test_latency_encoder_matches_reference_formula():
     # experiments/mnist/Dataset.py:66-71 on uint8 and float pixels, zeros included

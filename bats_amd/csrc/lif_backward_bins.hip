@@ -234,12 +234,15 @@ __device__ __forceinline__ void neuron_bins(const float *__restrict__ t, const f
   }
 }
 
-// One workgroup (up to 1024 threads, one neuron per thread per round) per sample. Pass 1
-// counts every bin's spikes; pass 2 writes per neuron the exclusive per-bin prefix sums of
-// es and e (bins -1 .. r-1) as rows b*2*kBins + r and b*2*kBins + kBins + r, column i, of Bm,
-// and lists the neuron's spikes in the sample's bin-major list: bins -1 .. kBins in turn,
-// within a bin by neuron and spike. spk entries are {t, es, e, i | (bin + 1) << 24}, slot the
-// index i*K_pre + m of the spike, poff[b][0 .. kBins+2] the start of each bin and the total.
+// One workgroup (up to 1024 threads, one neuron per thread per round) per sample. With one
+// round (I <= threads) a single pass does it all: each thread reads its neuron's spikes once
+// and one block scan of the per-bin counts gives the spikes' positions and the bins' starts.
+// Otherwise pass 1 counts every bin's spikes and pass 2 does the rest. Per neuron: the
+// exclusive per-bin prefix sums of es and e (bins -1 .. r-1) as rows b*2*kBins + r and
+// b*2*kBins + kBins + r, column i, of Bm, and the neuron's spikes in the sample's bin-major
+// list: bins -1 .. kBins in turn, within a bin by neuron and spike. spk entries are
+// {t, es, e, i | (bin + 1) << 24}, slot the index i*K_pre + m of the spike, poff[b][0 ..
+// kBins+2] the start of each bin and the total.
 __global__ __launch_bounds__(kPreMaxThreads) void pre_bins_kernel(
     int I, int K_pre, const float *__restrict__ pre_t, const float *__restrict__ pre_es,
     const float *__restrict__ pre_e, const float4 *__restrict__ range, int ldb,
@@ -252,6 +255,76 @@ __global__ __launch_bounds__(kPreMaxThreads) void pre_bins_kernel(
   const float4 rg = range[b];
   int cnt[kPreBinsN];
   double hs[kBins + 1], he[kBins + 1];
+  float4 *o = spk + (size_t)b * cap;
+  int *os = slot + (size_t)b * cap;
+  // the neuron's spikes in the list from their positions pos[bin + 1] on
+  auto list_spikes = [&](int i, size_t row, int n, int (&pos)[kPreBinsN]) {
+    for (int m = 0; m < n; ++m) {
+      const float tm = pre_t[row + m];
+      const int bin = pre_bin(tm, rg.x, rg.y, rg.z);
+      int at = 0;
+#pragma unroll
+      for (int q = 0; q < kPreBinsN; ++q) {
+        if (q == bin + 1) {
+          at = pos[q];
+          pos[q] += 1;
+        }
+      }
+      o[at] = make_float4(tm, pre_es[row + m], pre_e[row + m],
+                          __int_as_float(i | ((bin + 1) << 24)));
+      os[at] = i * K_pre + m;
+    }
+  };
+  auto write_prefix = [&](int i) {
+    double ps = 0.0, pe = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBins; ++q) {  // P[r] = bins -1 .. r-1
+      ps += hs[q];
+      pe += he[q];
+      Bm[((size_t)b * 2 * kBins + q) * ldb + i] = (float)ps;
+      Bm[((size_t)b * 2 * kBins + kBins + q) * ldb + i] = (float)pe;
+    }
+  };
+  if (I <= nth) {
+    // one neuron per thread: its counts and sums in one read, one block scan gives both its
+    // positions and the bins' totals (the same list as the two passes below)
+    const int i = tid;
+    const bool act = i < I;
+    const size_t row = ((size_t)b * I + (act ? i : 0)) * K_pre;
+    const int n = act ? leading_finite_row(pre_t + row, K_pre) : 0;
+    neuron_bins<true>(pre_t + row, pre_es + row, pre_e + row, n, rg, cnt, hs, he);
+    if (act) write_prefix(i);
+    int pos[kPreBinsN];
+#pragma unroll
+    for (int q = 0; q < kPreBinsN; ++q) {
+      int incl = cnt[q];
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+      }
+      pos[q] = incl - cnt[q];
+      if (lane == 63) s_wave[wv][q] = incl;
+    }
+    __syncthreads();
+    int start = 0;
+    int *po = poff + (size_t)b * (kPreBinsN + 1);
+#pragma unroll
+    for (int q = 0; q < kPreBinsN; ++q) {
+      int woff = 0, tq = 0;
+      for (int w = 0; w < nw; ++w) {
+        const int c = s_wave[w][q];
+        woff += w < wv ? c : 0;
+        tq += c;
+      }
+      pos[q] += woff + start;
+      if (tid == 0) po[q] = start;
+      start += tq;
+    }
+    if (tid == 0) po[kPreBinsN] = start;
+    if (act) list_spikes(i, row, n, pos);
+    return;
+  }
   // pass 1: per-bin totals
   int tot[kPreBinsN];
 #pragma unroll
@@ -291,16 +364,7 @@ __global__ __launch_bounds__(kPreMaxThreads) void pre_bins_kernel(
     const size_t row = ((size_t)b * I + (act ? i : 0)) * K_pre;
     const int n = act ? leading_finite_row(pre_t + row, K_pre) : 0;
     neuron_bins<true>(pre_t + row, pre_es + row, pre_e + row, n, rg, cnt, hs, he);
-    if (act) {
-      double ps = 0.0, pe = 0.0;
-#pragma unroll
-      for (int q = 0; q < kBins; ++q) {  // P[r] = bins -1 .. r-1
-        ps += hs[q];
-        pe += he[q];
-        Bm[((size_t)b * 2 * kBins + q) * ldb + i] = (float)ps;
-        Bm[((size_t)b * 2 * kBins + kBins + q) * ldb + i] = (float)pe;
-      }
-    }
+    if (act) write_prefix(i);
     int pos[kPreBinsN];
 #pragma unroll
     for (int q = 0; q < kPreBinsN; ++q) {
@@ -328,23 +392,7 @@ __global__ __launch_bounds__(kPreMaxThreads) void pre_bins_kernel(
     }
     __syncthreads();  // every thread has read s_wave and s_run
     if (tid < kPreBinsN) s_run[tid] += round_tot[tid];
-    float4 *o = spk + (size_t)b * cap;
-    int *os = slot + (size_t)b * cap;
-    for (int m = 0; m < n; ++m) {
-      const float tm = pre_t[row + m];
-      const int bin = pre_bin(tm, rg.x, rg.y, rg.z);
-      int at = 0;
-#pragma unroll
-      for (int q = 0; q < kPreBinsN; ++q) {
-        if (q == bin + 1) {
-          at = pos[q];
-          pos[q] += 1;
-        }
-      }
-      o[at] = make_float4(tm, pre_es[row + m], pre_e[row + m],
-                          __int_as_float(i | ((bin + 1) << 24)));
-      os[at] = i * K_pre + m;
-    }
+    if (act) list_spikes(i, row, n, pos);
   }
 }
 

@@ -22,8 +22,8 @@ constexpr int kRecHead = 1;  // first spike slot of a tile record
 __host__ __device__ inline int tile_rec_slots(int K) { return kRecHead + kWT * K; }
 
 struct BinLayout {
-  size_t range, A, Bm, Q, wt, recb, boff, spk, slot, poff, D, n_eff, partial, total;
-  int lda, ldb, ldt, gj, gemm_splits, q_splits, walk_slices, walk_b_per_slice;
+  size_t range, A, Bm, Q, wt, recb, boff, spk, slot, poff, D, n_eff, partial, gpart, total;
+  int lda, ldb, ldt, gj, gemm_splits, q_splits, walk_slices, walk_b_per_slice, pe_groups;
 };
 
 // Workspace layout of the binned path for a layer (B samples, I pre neurons with up to K_pre

@@ -582,15 +582,19 @@ __global__ __launch_bounds__(256) void transpose_pad_kernel(int J, int I, int ld
 
 // Lane per pre spike of the sample's bin-major list, no LDS. The bins after the spike's come
 // from Q (rows b*2*kBins + r and b*2*kBins + kBins + r, column i, summed over the GEMM's K
-// slices); the pairs inside the spike's bin from the tile records: a wave covers the bins [blo, bhi] of its 64 spikes (a
-// narrow range in the bin-major list) and per tile reads the record entries of those bins,
+// slices); the pairs inside the spike's bin from the tile records: a wave covers the bins
+// [blo, bhi] of its 64 spikes (a narrow range in the bin-major list) and per tile reads the
+// record entries of those bins,
 // their offsets and the entries themselves through scalar loads (the same for every lane), and
 // its lane's 16 weights from W^T (four 16-byte loads, one tile ahead). An entry adds
 // w[row] (es' d1 - e' d2) where its bin is the lane's and t_pre < t_post. The grid is
-// (chunk slots, B); slot s takes chunks s, s + slots, ... of its sample's list.
-// Measured on the 800 -> 800 layers of the 784-800-800-800-10 network, two thirds of the time
-// is the weight loads: a chunk's spikes belong to neurons spread over the layer, so each lane
-// reads its own line of W^T per tile.
+// (chunk slots x G tile groups, B); slot s takes chunks s, s + slots, ... of its sample's
+// list and walks the tiles of its group only. A chunk's spikes belong to neurons spread over
+// the layer, so each lane reads its own line of W^T per tile: with G = 8 and the group taken
+// from blockIdx.x % 8 (workgroups are dealt to the 8 XCDs round robin), every XCD reads one
+// eighth of W^T and of the records, which stays in its L2. With G > 1 each group writes its
+// partial sums (float64; group 0 adds the bins after the spike's) and pre_err_groups_kernel
+// adds them in group order.
 constexpr int kPbWaves = 4;
 constexpr int kPbSlots = 8;
 
@@ -599,24 +603,27 @@ __global__ __launch_bounds__(64 * kPbWaves) void pre_err_bin_kernel(
     const int *__restrict__ slot, const int *__restrict__ poff, const float *__restrict__ WT,
     int ldt, const float4 *__restrict__ recb, const int *__restrict__ boff,
     const float *__restrict__ Q, int ldq, int q_splits, int64_t q_stride, double inv_tau_s,
-    double inv_tau, float *__restrict__ pre_errors) {
+    double inv_tau, float *__restrict__ pre_errors, int G, double *__restrict__ gpart) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.y;
+  // tile group g of G (blocks of one group run on one XCD when G divides 8)
+  const int g = blockIdx.x % G, nslot = gridDim.x / G, sl = blockIdx.x / G;
+  const int tj0 = g * gj / G, tj1 = (g + 1) * gj / G;
   const int P = poff[(size_t)b * (kPreBinsN + 1) + kPreBinsN];
   const int R = tile_rec_slots(K);
   uniform_f32 *srec = (uniform_f32 *)(recb + (size_t)b * gj * R);
   uniform_i32 *sbo = (uniform_i32 *)(boff + (size_t)b * gj * (kBins + 1));
-  for (int c = blockIdx.x * kPbWaves + wv; c * 64 < P; c += gridDim.x * kPbWaves) {
+  for (int c = sl * kPbWaves + wv; c * 64 < P; c += nslot * kPbWaves) {
     const int sidx = c * 64 + lane;
     const bool valid = sidx < P;
     const float4 S = valid ? spk[(size_t)b * cap + sidx] : make_float4(kInfF, 0.f, 0.f, 0.f);
     const int sw = __float_as_int(S.w);
     const int i = valid ? (sw & 0xffffff) : 0;
     const int nb = valid ? (sw >> 24) - 1 : kBins;  // -1 .. kBins
-    // the bins after the spike's
+    // the bins after the spike's (by tile group 0)
     double q1 = 0.0, q2 = 0.0;
-    for (int sp = 0; sp < q_splits; ++sp) {
+    for (int sp = 0; sp < (g == 0 ? q_splits : 0); ++sp) {
       const float *q = Q + sp * q_stride + (size_t)b * 2 * kBins * ldq + i;
 #pragma unroll
       for (int r = 0; r < kBins; ++r) {
@@ -639,14 +646,14 @@ __global__ __launch_bounds__(64 * kPbWaves) void pre_err_bin_kernel(
       float w[kBT], wn[kBT];
 #pragma unroll
       for (int h = 0; h < kBT / 4; ++h) {
-        const float4 v = wt[h];
+        const float4 v = wt[tj0 * (kBT / 4) + h];
         w[4 * h] = v.x;
         w[4 * h + 1] = v.y;
         w[4 * h + 2] = v.z;
         w[4 * h + 3] = v.w;
       }
-      for (int tj = 0; tj < gj; ++tj) {
-        const int tn = min(tj + 1, gj - 1);
+      for (int tj = tj0; tj < tj1; ++tj) {
+        const int tn = min(tj + 1, tj1 - 1);
 #pragma unroll
         for (int h = 0; h < kBT / 4; ++h) {
           const float4 v = wt[tn * (kBT / 4) + h];
@@ -682,11 +689,35 @@ __global__ __launch_bounds__(64 * kPbWaves) void pre_err_bin_kernel(
         for (int u = 0; u < kBT; ++u) w[u] = wn[u];
       }
     }
-    if (valid) pre_errors[(size_t)b * I * K_pre + slot[(size_t)b * cap + sidx]] = (float)pe;
+    if (valid) {
+      if (G == 1)
+        pre_errors[(size_t)b * I * K_pre + slot[(size_t)b * cap + sidx]] = (float)pe;
+      else
+        gpart[((size_t)g * gridDim.y + b) * cap + sidx] = pe;
+    }
+  }
+}
+
+// pre_err = the tile groups' partial sums, added in group order.
+__global__ __launch_bounds__(256) void pre_err_groups_kernel(
+    int I, int K_pre, int cap, int G, const int *__restrict__ slot, const int *__restrict__ poff,
+    const double *__restrict__ gpart, float *__restrict__ pre_errors) {
+  const int b = blockIdx.y, B = gridDim.y;
+  const int P = poff[(size_t)b * (kPreBinsN + 1) + kPreBinsN];
+  for (int sidx = blockIdx.x * 256 + threadIdx.x; sidx < P; sidx += gridDim.x * 256) {
+    double pe = 0.0;
+    for (int g = 0; g < G; ++g) pe += gpart[((size_t)g * B + b) * cap + sidx];
+    pre_errors[(size_t)b * I * K_pre + slot[(size_t)b * cap + sidx]] = (float)pe;
   }
 }
 
 // ------------------------------------------------------------------ host side
+// Tile groups of the pre-spike error walk: one per XCD, so that each XCD's L2 holds its
+// group's columns of W^T and records (the deep network's 800 -> 800 layers: 227 -> 144 us per
+// launch plus 12 us for the group sums; 4 groups 204 us, 16 groups 155 us, 32 groups 175 us).
+constexpr int kPeGroups = 8;
+static int pre_err_groups(int gj) { return std::max(1, std::min(kPeGroups, gj)); }
+
 BinLayout bin_layout(int B, int I, int K_pre, int J, int K) {
   BinLayout L{};
   L.lda = (J + 3) & ~3;
@@ -727,6 +758,8 @@ BinLayout bin_layout(int B, int I, int K_pre, int J, int K) {
   L.D = take(sizeof(float2) * (size_t)B * J * K);
   L.n_eff = take(sizeof(int) * (size_t)B * J);
   L.partial = take(sizeof(float) * (size_t)(L.gemm_splits + L.walk_slices) * J * I);
+  L.pe_groups = pre_err_groups(L.gj);
+  L.gpart = take(L.pe_groups > 1 ? sizeof(double) * (size_t)L.pe_groups * B * cap : 0);
   L.total = off;
   return L;
 }
@@ -805,12 +838,19 @@ int bins_backward(hipStream_t st, const BinLayout &L, int B, int I, int K_pre, i
                                           (int64_t)rows * L.ldb, L.q_splits, 1.f));
   if (grc) return grc;
   const int slots = std::max(1, std::min(kPbSlots, (cap + 64 * kPbWaves - 1) / (64 * kPbWaves)));
+  const int G = L.pe_groups;
+  double *gpart = reinterpret_cast<double *>(ws + L.gpart);
   BATS_TIMED("pre_err_bin_kernel", st,
-             hipLaunchKernelGGL(pre_err_bin_kernel, dim3(slots, B), dim3(64 * kPbWaves), 0, st, I,
-                                K_pre, K, L.gj, cap, spk, slot, poff, WT, L.ldt, recb, boff, Q,
-                                L.ldb, L.q_splits, (int64_t)rows * L.ldb, inv_tau_s, inv_tau,
-                                pre_errors));
-  return check_launch("pre_err_bin_kernel");
+             hipLaunchKernelGGL(pre_err_bin_kernel, dim3(slots * G, B), dim3(64 * kPbWaves), 0,
+                                st, I, K_pre, K, L.gj, cap, spk, slot, poff, WT, L.ldt, recb,
+                                boff, Q, L.ldb, L.q_splits, (int64_t)rows * L.ldb, inv_tau_s,
+                                inv_tau, pre_errors, G, gpart));
+  if (int r = check_launch("pre_err_bin_kernel")) return r;
+  if (G == 1) return 0;
+  BATS_TIMED("pre_err_groups_kernel", st,
+             hipLaunchKernelGGL(pre_err_groups_kernel, dim3(8, B), dim3(256), 0, st, I, K_pre,
+                                cap, G, slot, poff, gpart, pre_errors));
+  return check_launch("pre_err_groups_kernel");
 }
 
 }  // namespace bats

@@ -105,3 +105,15 @@ def test_binned_tied_times_many_slices_and_batch_sum():
     errors = rng.normal(size=ref.t.shape).astype(F32)
     check_binned(ref, errors, "tied times, several slices")
     check_binned(ref, errors, "tied times, batch sum", batch_sum=True)
+
+
+def test_binned_pre_layer_wider_than_one_workgroup_round():
+    """More pre neurons than the pre-spike listing's workgroup has threads (1024): the
+    listing counts the bins in a first pass and lists the spikes round by round."""
+    rng = np.random.default_rng(13)
+    B, I, Kp, J, K = 6, 1100, 3, 24, 4
+    pre, _ = multi_spike_inputs(rng, B, I, Kp, t_max=1.0)
+    W = (0.8 * rng.normal(0, 1.0, size=(J, I))).astype(F32)
+    ref = olif.lif_forward(pre, W, 0.130, 1.0, 1.0, K, tmax=1.0)
+    assert ref.n.sum() > 30
+    check_binned(ref, rng.normal(size=ref.t.shape).astype(F32), "pre layer of 1100 neurons")

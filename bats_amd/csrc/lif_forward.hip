@@ -4,7 +4,7 @@
 //   1. transpose_kernel    W [J,I] -> W^T [I,J] so that the lanes of a wave (consecutive
 //                          post neurons) read one input's weights with one coalesced load.
 //   2. sort_events_kernel  one workgroup per sample: compacts the finite input slots,
-//                          bitonic-sorts (time, slot) keys in LDS and writes the sample's
+//                          sorts (time, slot) keys in LDS and writes the sample's
 //                          time-ordered Event list (exps in float32 exactly as stored for
 //                          the backward, exp(t/tau) in float64 for the crossing filter).
 //                          It also writes the per-slot exps the backward reads.
@@ -71,43 +71,6 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *keys, int N, bool global_
   }
 }
 
-// Bitonic network over 1024 keys held two per thread (keys 2t and 2t+1) by a 512-thread
-// block: the partner of a compare-exchange at distance j >= 2 is thread t ^ (j/2), reached
-// with a lane shuffle inside the wave (j/2 < 64) or through LDS (j/2 >= 64, 6 of the 55
-// stages, each between two barriers). Ascending order on exit.
-__device__ __forceinline__ void bitonic1024_regs(uint64_t &x0, uint64_t &x1,
-                                                 uint64_t *__restrict__ lds) {
-  const int t = threadIdx.x;
-  for (int k = 2; k <= 1024; k <<= 1) {
-    const bool up = ((2 * t) & k) == 0;
-    for (int j = k >> 1; j >= 1; j >>= 1) {
-      if (j == 1) {
-        const uint64_t lo = x0 < x1 ? x0 : x1, hi = x0 < x1 ? x1 : x0;
-        x0 = up ? lo : hi;
-        x1 = up ? hi : lo;
-        continue;
-      }
-      const int pj = j >> 1;
-      uint64_t y0, y1;
-      if (pj < 64) {
-        y0 = __shfl_xor(x0, pj, 64);
-        y1 = __shfl_xor(x1, pj, 64);
-      } else {
-        __syncthreads();
-        lds[2 * t] = x0;
-        lds[2 * t + 1] = x1;
-        __syncthreads();
-        const int pt = t ^ pj;
-        y0 = lds[2 * pt];
-        y1 = lds[2 * pt + 1];
-      }
-      const bool keep_min = ((t & pj) == 0) == up;
-      x0 = keep_min ? (x0 < y0 ? x0 : y0) : (x0 < y0 ? y0 : x0);
-      x1 = keep_min ? (x1 < y1 ? x1 : y1) : (x1 < y1 ? y1 : x1);
-    }
-  }
-}
-
 // Bitonic network over the 128 keys of one wave, two per lane (keys 2l and 2l+1): every
 // compare-exchange partner is reached with a lane shuffle (no LDS, no barrier). Ascending.
 __device__ __forceinline__ void bitonic128_wave(uint64_t &x0, uint64_t &x1, int l) {
@@ -135,8 +98,10 @@ __device__ __forceinline__ void bitonic128_wave(uint64_t &x0, uint64_t &x1, int 
 // of two): each pass merges run pairs into runs of twice the length, each of the first nthr
 // threads producing N / nthr consecutive outputs from its merge-path split (a binary search on
 // its diagonal); every thread of the block takes part in the barriers. Keys are distinct except
-// the ~0 padding. Returns the buffer holding the sorted keys.
-__device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0, int nthr) {
+// the ~0 padding. With `global` one of the two buffers is in global memory and each pass ends
+// with a device fence before the barrier. Returns the buffer holding the sorted keys.
+__device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0, int nthr,
+                                bool global = false) {
   const int E = N / nthr;
   for (int L = L0; L < N; L <<= 1) {
     if ((int)threadIdx.x < nthr) {
@@ -161,6 +126,7 @@ __device__ uint64_t *merge_runs(uint64_t *src, uint64_t *dst, int N, int L0, int
       ib += take_a ? 0 : 1;
     }
     }
+    if (global) __threadfence();
     __syncthreads();
     uint64_t *tmp = src;
     src = dst;
@@ -253,10 +219,11 @@ __device__ __forceinline__ void write_sorted_events(
 
 // One workgroup per sample: compact the finite slots, sort (time, slot) keys, write the
 // sample's Event list padded to a multiple of kEventPad, the event indices, the count, and
-// the per-slot float32 exps the backward reads. Up to 1024 events each wave sorts 128 keys in
-// registers (bitonic128_wave) and merge_runs merges the runs; longer lists that fit twice in
-// the LDS are sorted as 1024-key register runs (bitonic1024_regs) merged by merge_runs, the
-// rest with the LDS (or global) bitonic network.
+// the per-slot float32 exps the backward reads. The waves sort 128-key runs in registers
+// (bitonic128_wave) and merge_runs merges them: up to 1024 events one run per wave; longer
+// lists that fit the LDS in runs taken wave by wave, with the merge passes' second buffer in
+// the LDS when the list fits twice, else in the sample's global key region; lists beyond the
+// LDS with the global bitonic network.
 __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
     const float *__restrict__ pre_times, int S, int K_pre, float tau_s, float tau, float c,
     float tmax, int Pcap, Event *__restrict__ events, FastEvent *__restrict__ fast,
@@ -381,20 +348,24 @@ __global__ __launch_bounds__(kSortThreads) void sort_events_kernel(
       __syncthreads();
       keys = merge_runs(keys, keys + 1024, Ns, 128, (int)blockDim.x);
     }
-  } else if (!use_global && 2 * N <= lds_cap) {
-    // 1024-key runs sorted in registers, then merge-path passes between two LDS buffers
+  } else if (!use_global) {
+    // 128-key runs sorted in registers by the waves independently (wave w takes runs w,
+    // w + waves, ...), then merge-path passes between two LDS buffers, or between the LDS and
+    // the sample's global key region when the LDS holds the keys once only (the host then
+    // sizes that region for the list)
+    const bool alt_global = 2 * N > lds_cap;
+    uint64_t *alt = alt_global ? gkeys + (size_t)b * gcap : keys + N;
     for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
     __syncthreads();
-    const int t = threadIdx.x;
-    for (int c0 = 0; c0 < N; c0 += 1024) {
-      uint64_t x0 = keys[c0 + 2 * t], x1 = keys[c0 + 2 * t + 1];
-      bitonic1024_regs(x0, x1, keys + c0);  // exchanges through the run's own slots
-      __syncthreads();
-      keys[c0 + 2 * t] = x0;
-      keys[c0 + 2 * t + 1] = x1;
-      __syncthreads();
+    const int l = threadIdx.x & 63, nwv = blockDim.x >> 6;
+    for (int r0 = (threadIdx.x >> 6) * 128; r0 < N; r0 += nwv * 128) {
+      uint64_t x0 = keys[r0 + 2 * l], x1 = keys[r0 + 2 * l + 1];
+      bitonic128_wave(x0, x1, l);
+      keys[r0 + 2 * l] = x0;  // the wave's own run: its reads above are done
+      keys[r0 + 2 * l + 1] = x1;
     }
-    keys = merge_runs(keys, keys + N, N, 1024, (int)blockDim.x);
+    __syncthreads();
+    keys = merge_runs(keys, alt, N, 128, (int)blockDim.x, alt_global);
   } else {
     for (int q = P + threadIdx.x; q < N; q += blockDim.x) keys[q] = ~0ull;
     if (use_global) __threadfence();
@@ -1086,11 +1057,15 @@ static int event_capacity(int S) {
   return std::max(kEventPad, (S + kEventPad - 1) / kEventPad * kEventPad);
 }
 
+// Global sort keys per sample: needed when a list may not fit the LDS twice (the second
+// buffer of the merge passes, or the whole list beyond the LDS).
+static int global_key_capacity(int N, int cap) { return (2 * N > cap) ? N : 0; }
+
 size_t event_lists_bytes(int B, int S) {
   const int cap = lds_capacity_entries();
   int N = 1;
   while (N < S) N <<= 1;
-  const int gcap = (N > cap) ? N : 0;
+  const int gcap = global_key_capacity(N, cap);
   const int Pcap = event_capacity(S);
   size_t off = 0;
   off = align_up(off + sizeof(Event) * (size_t)B * Pcap, 256);
@@ -1108,7 +1083,7 @@ static EventLists event_lists_at(void *ws, int B, int S, uint64_t **gkeys, int *
   const int cap = lds_capacity_entries();
   int N = 1;
   while (N < S) N <<= 1;
-  const int gcap = (N > cap) ? N : 0;
+  const int gcap = global_key_capacity(N, cap);
   const int Pcap = event_capacity(S);
   char *base = static_cast<char *>(ws);
   EventLists el;
@@ -1138,7 +1113,9 @@ int build_event_lists(hipStream_t st, int B, int S, int K_pre, const float *pre_
   *out = event_lists_at(ws, B, S, &gkeys, &gcap);
   const int Pcap = out->Pcap;
   if (S > 0) {
-    const int lds_entries = std::max(2048, std::min(N, cap));  // the short-list merge buffers
+    // room for the list twice (the merge passes) up to the LDS limit; 2048 at least for the
+    // short-list merge buffers
+    const int lds_entries = std::max(2048, std::min(2 * N, cap));
     BATS_TIMED_IF(timed, "sort_events_kernel", st,
                hipLaunchKernelGGL(sort_events_kernel, dim3(B), dim3(kSortThreads),
                                   sizeof(uint64_t) * lds_entries, st, pre_times, S, K_pre, tau_s,

@@ -127,6 +127,20 @@ def test_forward_event_list_sort_paths(n_events):
     assert ref.n.sum() > 0
 
 
+@pytest.mark.parametrize("n_in,k_in,lo,hi", [(800, 10, 4097, 8192), (1000, 12, 8193, 16384)])
+def test_forward_long_event_lists_merge_sort(n_in, k_in, lo, hi):
+    # lists of 4097..8192 events (merged between two LDS buffers) and of 8193..16384 events in
+    # a layer of more slots (the merge passes alternate between the LDS and global memory)
+    rng = np.random.default_rng(8)
+    n = rng.integers(k_in - 3, k_in + 1, size=(2, n_in))
+    t = np.sort(rng.uniform(0.0, 1.0, size=(2, n_in, k_in)), axis=2).astype(F32)
+    t[np.arange(k_in)[None, None, :] >= n[:, :, None]] = np.inf
+    counts = np.isfinite(t).sum(axis=(1, 2))
+    assert ((counts >= lo) & (counts <= hi)).all(), counts
+    W = rng.normal(0, 0.3, size=(10, n_in)).astype(F32)
+    check_forward(t, W, 0.130, 1.3, 1.3, 30, 1.0, f"merge sort, {counts.tolist()} events")
+
+
 def test_forward_large_event_list_global_sort():
     # 800 x 30 slots per sample with ~20k spikes: exceeds the LDS sort capacity
     rng = np.random.default_rng(6)

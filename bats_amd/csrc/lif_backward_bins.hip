@@ -728,11 +728,13 @@ BinLayout bin_layout(int B, int I, int K_pre, int J, int K) {
   const size_t cap = (size_t)I * K_pre;
   // GEMMs: K split so that the grid is at most one round of 2 workgroups per CU (the LDS of
   // two fits a CU), each split at least 64 deep. The weight gradient has K = 2 kBins B, Q has
-  // K = J (its slices are summed by pre_err_bin_kernel).
+  // K = J; Q's slices are summed per pre spike by pre_err_bin_kernel, so it takes at most
+  // two (deep network's 800 -> 800 layers: the Q GEMM takes 47 us with two or four slices, 82
+  // with one; the walk 143 us reading four slices, 125 us reading two).
   const int tiles = ((J + kGemmBM - 1) / kGemmBM) * ((I + kGemmBN - 1) / kGemmBN);
   L.gemm_splits = std::max(1, std::min((int)(rows / 64), kGemmSlots / tiles));
   const int qtiles = (int)((rows + kGemmBM - 1) / kGemmBM) * ((I + kGemmBN - 1) / kGemmBN);
-  L.q_splits = std::max(1, std::min(std::min(J / 64, 4), kGemmSlots / qtiles));
+  L.q_splits = std::max(1, std::min(std::min(J / 64, 2), kGemmSlots / qtiles));
   // in-bin walk: slices so that the grid fills the 256 CUs about once (one workgroup each, as
   // the LDS accumulators take most of a CU's LDS), with at most kWalkMaxSamples per slice
   int sl = std::max(1, std::min(std::max(B, 1), 256 / L.gj));

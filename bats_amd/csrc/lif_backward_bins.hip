@@ -399,53 +399,64 @@ __global__ __launch_bounds__(kPreMaxThreads) void pre_bins_kernel(
 // ------------------------------------------------------------------ in-bin weight gradient
 // The spike-lane walk restricted to pairs in one bin: one workgroup per (tile of kBT rows,
 // batch slice of up to kWalkMaxSamples samples) keeps acc[row][pre neuron] in float64 LDS.
-// The slice's samples are taken G at a time (a round): their tile records are copied to LDS
-// together, and the waves share out the 64-spike chunks of all G samples' pre spikes of bins
-// 0 .. kBins-1 (bin-major, so a chunk spans few bins), each walking only the record entries of
-// its chunk's bins and adding es d1 - e d2 where the bins match and t_pre < t_post. A lane
+// The slice's samples are taken in rounds of as many consecutive samples as the LDS beside acc
+// holds the real entries of (packed, each record at its start in the round): their tile
+// records are copied to LDS together, and the waves share out the 64-spike chunks of all the
+// round's samples' pre spikes of bins 0 .. kBins-1 (bin-major, so a chunk spans few bins),
+// each walking only the record entries of its chunk's bins and adding es d1 - e d2 where the
+// bins match and t_pre < t_post. A lane
 // sums one row's terms in a register (entries of one bin are row-ordered, so a row change is
 // uniform) and adds them to acc with one no-return LDS atomic when the row ends. A wave
 // requests its next chunk's spikes before walking the current one; there are two barriers
 // per round, not per sample.
 constexpr int kWalkMaxSamples = 64;  // samples per in-bin walk slice (one lane each in a scan)
 constexpr int kWalkThreads = 1024;
-constexpr int kWalkCopy = 4;  // record entries per thread copied per round (G * kBT * K bound)
+constexpr int kWalkCopy = 4;  // record entries per thread in flight while a round is copied
 constexpr int kWalkLds = 160 * 1024 - 1024;
-constexpr int kSInfo = 3 + kBins + 1;  // per sample: p0, p1, chunks, bin offsets 0 .. kBins
+// per sample: p0, p1, chunks, bin offsets 0 .. kBins, the record's start in the round's LDS
+constexpr int kSInfo = 3 + kBins + 2;
+constexpr int kSBase = 3 + kBins + 1;
 
 __host__ __device__ inline size_t bin_walk_fixed_bytes(int I) {
-  return (size_t)kBT * I * sizeof(double) + (size_t)kWalkMaxSamples * kSInfo * sizeof(int);
+  const size_t b = (size_t)kBT * I * sizeof(double) +
+                   (size_t)(kWalkMaxSamples * kSInfo + kWalkMaxSamples + 1 + 1) * sizeof(int);
+  return (b + 15) / 16 * 16;
 }
 
-// Samples per round: as many records as fit beside acc (at least one) and kWalkCopy entries
-// per thread.
-__host__ __device__ inline int bin_walk_group(int I, int K) {
-  const int rb = kBT * K;
+// Record entries the LDS holds beside acc: a round takes consecutive samples while their
+// records (only their real entries, packed) fit.
+__host__ __device__ inline int bin_walk_entries(int I) {
   const long avail = (long)kWalkLds - (long)bin_walk_fixed_bytes(I);
-  int g = (int)(avail / ((long)rb * (long)sizeof(float4)));
-  g = min(g, kWalkCopy * kWalkThreads / rb);
-  return min(g, kWalkMaxSamples);
+  return avail > 0 ? (int)(avail / (long)sizeof(float4)) : 0;
+}
+
+// Whole records (kBT * K entries) the LDS holds: at least one is needed.
+__host__ __device__ inline int bin_walk_group(int I, int K) {
+  return min(bin_walk_entries(I) / (kBT * K), kWalkMaxSamples);
 }
 
 __host__ __device__ inline size_t bin_wgrad_lds_bytes(int I, int K) {
-  return bin_walk_fixed_bytes(I) + (size_t)max(bin_walk_group(I, K), 1) * kBT * K * sizeof(float4);
+  (void)K;
+  return bin_walk_fixed_bytes(I) + (size_t)bin_walk_entries(I) * sizeof(float4);
 }
 
 __global__ __launch_bounds__(kWalkThreads) void wgrad_bin_kernel(
-    int B, int I, int J, int K, int gj, int b_per_slice, int G, int cap,
+    int B, int I, int J, int K, int gj, int b_per_slice, int capE, int cap,
     const float4 *__restrict__ spk, const int *__restrict__ poff, const float4 *__restrict__ recb,
     const int *__restrict__ boff, float *__restrict__ partial) {
   extern __shared__ __align__(16) char smem[];
   double *acc = reinterpret_cast<double *>(smem);  // [kBT][I]
   int *info = reinterpret_cast<int *>(acc + (size_t)kBT * I);  // [kWalkMaxSamples][kSInfo]
-  float4 *rb = reinterpret_cast<float4 *>(info + kWalkMaxSamples * kSInfo);  // [G][kBT * K]
+  int *rstart = info + kWalkMaxSamples * kSInfo;  // first sample of each round, then nS
+  int *nrounds = rstart + kWalkMaxSamples + 1;
+  float4 *rb = reinterpret_cast<float4 *>(smem + bin_walk_fixed_bytes(I));  // [capE]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int nth = kWalkThreads, nw = kWalkThreads / 64;
   const int tj = blockIdx.x, sl = blockIdx.y;
   const int j0 = tj * kBT, jn = min(kBT, J - j0);
   const int b0 = sl * b_per_slice, nS = max(0, min(B, b0 + b_per_slice) - b0);
-  const int RB = kBT * K, R = tile_rec_slots(K);
+  const int R = tile_rec_slots(K);
   for (int e = tid; e < kBT * I; e += nth) acc[e] = 0.0;
   if (tid < nS) {
     const int *po = poff + (size_t)(b0 + tid) * (kPreBinsN + 1);
@@ -459,22 +470,50 @@ __global__ __launch_bounds__(kWalkThreads) void wgrad_bin_kernel(
     info[s * kSInfo + 3 + q] = boff[((size_t)(b0 + s) * gj + tj) * (kBins + 1) + q];
   }
   __syncthreads();
-  for (int g0 = 0; g0 < nS; g0 += G) {
-    const int gn = min(G, nS - g0);
-    // the round's records, kWalkCopy entries per thread in flight together
-    {
+  // rounds: consecutive samples whose records fit capE entries together (a record has at most
+  // kBT * K <= capE entries), each record's start in the round
+  if (tid == 0) {
+    int nr = 0, used = 0;
+    rstart[0] = 0;
+    for (int s = 0; s < nS; ++s) {
+      const int c = info[s * kSInfo + 3 + kBins];
+      if (used + c > capE) {
+        rstart[++nr] = s;
+        used = 0;
+      }
+      info[s * kSInfo + kSBase] = used;
+      used += c;
+    }
+    rstart[++nr] = nS;
+    *nrounds = nS > 0 ? nr : 0;
+  }
+  __syncthreads();
+  const int nr = *nrounds;
+  for (int r = 0; r < nr; ++r) {
+    const int g0 = rstart[r], gn = rstart[r + 1] - g0;
+    const int *last = info + (g0 + gn - 1) * kSInfo;
+    const int E = last[kSBase] + last[3 + kBins];
+    // the round's records, packed, kWalkCopy entries per thread in flight together; an
+    // entry's sample is the last of the round whose start is at or before it
+    for (int e0 = 0; e0 < E; e0 += kWalkCopy * nth) {
       float4 v[kWalkCopy];
       int at[kWalkCopy];
 #pragma unroll
       for (int h = 0; h < kWalkCopy; ++h) {
-        const int e = tid + h * nth;
+        const int e = e0 + tid + h * nth;
         at[h] = -1;
-        if (e < gn * RB) {
-          const int s = e / RB, k = e - s * RB;
-          if (k < info[(g0 + s) * kSInfo + 3 + kBins]) {
-            v[h] = recb[((size_t)(b0 + g0 + s) * gj + tj) * R + kRecHead + k];
-            at[h] = e;
+        if (e < E) {
+          int lo = 0, hi = gn - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (info[(g0 + mid) * kSInfo + kSBase] <= e)
+              lo = mid;
+            else
+              hi = mid - 1;
           }
+          const int k = e - info[(g0 + lo) * kSInfo + kSBase];
+          v[h] = recb[((size_t)(b0 + g0 + lo) * gj + tj) * R + kRecHead + k];
+          at[h] = e;
         }
       }
 #pragma unroll
@@ -521,7 +560,7 @@ __global__ __launch_bounds__(kWalkThreads) void wgrad_bin_kernel(
       const int blo = __builtin_amdgcn_readfirstlane(nb);
       const int bhi = __builtin_amdgcn_readlane(nb, 63 - __clzll((long long)vm));
       const int *sb = info + (g0 + cur.s) * kSInfo + 3;
-      const float4 *rr = rb + (size_t)cur.s * RB;
+      const float4 *rr = rb + info[(g0 + cur.s) * kSInfo + kSBase];
       const int e0 = sb[blo], e1 = sb[bhi + 1];
       double racc = 0.0;
       int crow = e0 < e1 ? __builtin_amdgcn_readfirstlane(__float_as_int(rr[e0].w) >> 9) : 0;
@@ -824,7 +863,7 @@ int bins_backward(hipStream_t st, const BinLayout &L, int B, int I, int K_pre, i
   BATS_TIMED("wgrad_bin_kernel", st,
              hipLaunchKernelGGL(wgrad_bin_kernel, dim3(L.gj, L.walk_slices), dim3(kWalkThreads),
                                 lds, st, B, I, J, K, L.gj, L.walk_b_per_slice,
-                                bin_walk_group(I, K), cap, spk, poff, recb, boff,
+                                bin_walk_entries(I), cap, spk, poff, recb, boff,
                                 partial + (size_t)L.gemm_splits * J * I));
   if (int r = check_launch("wgrad_bin_kernel")) return r;
   if (!pre_errors) return 0;
